@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""PageRank benchmark on one MI355X: GTEPS and fraction of the HBM roofline.
+
+Workload (one "step" = one full PageRank solve to tolerance on the resident
+graph): directed R-MAT scale 22, edge factor 16 (a=0.57, b=0.19, c=0.19),
+self-loops and duplicates removed, seed 27491095, uint32 in-CSR; damping
+0.85, uniform teleport, dangling mass redistributed uniformly, tolerance 1e-6
+(L1), at most 100 sweeps; FP32 ranks with FP32 accumulation.
+
+value = GTEPS = edges x sweeps / wall time of the timed solves (inputs
+resident in HBM; each solve includes its init, all sweeps and the output
+permutation).  roofline.achieved = algorithmic bytes per sweep
+(8*M + 16*N + 4: a uint32 index and an FP32 gathered contribution per edge;
+offset, degree, old and new rank per vertex) x sweeps / summed device time of
+the sweep kernels (HIP events around every captured chunk).
+
+cpu_baseline: the OpenMP restatement of the reference's default page_rank
+(v11, in-place Gauss-Seidel) on the same graph, FP32, timed on a bounded
+number of sweeps on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PageRank GTEPS & % HBM roofline, R-MAT s22; fwd+adjoint vs fwd time"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=27491095)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iters", type=int, default=100)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU time of the baseline sample (0 disables it)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
+    return ap.parse_args()
+
+
+def load_traffic(scale, ef):
+    """HBM bytes per sweep launch from committed rocprofv3 --pmc summaries."""
+    path = os.path.join(ROOT, "profiles", f"k_sweep_traffic_rmat_s{scale}_ef{ef}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_sweep")
+
+
+def cpu_baseline(g, deg, target_s, threads):
+    import oracle  # the CPU restatement: baseline leg only
+    # calibrate on 2 sweeps, then time a bounded number of sweeps
+    t0 = time.perf_counter()
+    oracle.gs_par(g.offsets, g.indices, deg, 0.85, 0.0, 2, nthreads=threads, dtype=np.float32)
+    per = (time.perf_counter() - t0) / 2
+    sweeps = int(max(2, min(100, target_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, 0.0, sweeps, nthreads=threads, dtype=np.float32)
+    dt = time.perf_counter() - t0
+    return {"value": g.m * it / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"R-MAT s{int(np.log2(g.n))} ef16 (same graph), {it} in-place Gauss-Seidel sweeps of the "
+                      f"OpenMP restatement of page_rank v11 (FP32, schedule(dynamic,64)), {dt:.2f} s",
+            "seconds_per_sweep": dt / it}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "
+              "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import nwgr_amd
+    from nwgr_amd import PageRankGraph
+
+    t0 = time.perf_counter()
+    g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
+    deg = nwgr_amd.build_degrees(g)
+    log(f"generated R-MAT s{args.scale} ef{args.edge_factor}: n={g.n} m={g.m} in {time.perf_counter() - t0:.1f}s")
+    pg = PageRankGraph(g, deg)
+    log(f"device graph: {pg.device_bytes / 2**20:.0f} MiB, setup {pg.setup_ms:.1f} ms")
+    kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform", chunk=args.chunk)
+    out = np.empty(g.n, np.float32)
+    for i in range(args.warmup):
+        _, res = pg.run(out=out, **kw)
+        log(f"warmup {i}: sweeps={res.iters} err={res.error:.3e} {res.ms:.3f} ms")
+    sweeps = 0
+    sweep_ms = 0.0
+    launches = 0
+    dev_ms = 0.0
+    iters = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, res = pg.run(out=out, **kw)
+        prof = pg.last_profile()
+        sweeps += res.iters
+        iters.append(res.iters)
+        sweep_ms += prof["sweep_ms"]
+        launches += prof["launches"]
+        dev_ms += res.ms
+    wall = time.perf_counter() - t0
+    mass = float(out.sum(dtype=np.float64))
+    log(f"timed {args.steps} solves: {wall * 1e3:.2f} ms wall, {dev_ms:.2f} ms device, sweeps/solve={iters[0]}, "
+        f"sum(r)={mass:.7f}")
+    if abs(mass - 1.0) > 1e-4:
+        log("WARNING: ranks do not sum to 1")
+
+    n, m = g.n, g.m
+    b_iter = 8 * m + 16 * n + 4
+    achieved = b_iter * sweeps / (sweep_ms / 1e3) / 1e9
+    traffic = load_traffic(args.scale, args.edge_factor)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_sweep", "algorithmic_bytes_per_sweep": b_iter,
+            "avg_launch_us": round(sweep_ms / launches * 1e3, 2), "launches": launches, "sweeps": sweeps}
+    value = m * sweeps / wall / 1e9
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads (~{args.cpu_seconds:.0f}s)")
+        cpu = cpu_baseline(g, deg, args.cpu_seconds, threads)
+        cpu["value"] = round(cpu["value"], 4)
+        log(f"cpu baseline {cpu['value']} GTEPS")
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"pagerank rmat s{args.scale} ef{args.edge_factor} (a=0.57,b=0.19,c=0.19, seed "
+                               f"{args.seed}, dedup, no self-loops), damping 0.85, dangling uniform, tol "
+                               f"{args.tol:g} L1, max {args.max_iters} sweeps, FP32",
+                   "n": n, "m": m, "sweeps_per_solve": iters[0], "device_ms_per_solve": round(dev_ms / args.steps, 4),
+                   "setup_ms": round(pg.setup_ms, 1)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_vs_cpu_per_sweep"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

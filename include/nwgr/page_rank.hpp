@@ -1,0 +1,132 @@
+/*
+ * nwgr/page_rank.hpp -- C++20 drop-in for nw::graph::page_rank on the MI355X.
+ *
+ * Include this header instead of <nwgraph/algorithms/page_rank.hpp> and link
+ * libnwgr_pagerank.so.  The template keeps the reference signature
+ * (include/nwgraph/algorithms/page_rank.hpp:89-91):
+ *
+ *   template <adjacency_list_graph Graph, typename Real>
+ *   void page_rank(const Graph& graph,
+ *                  const std::vector<typename Graph::vertex_id_type>& degrees,
+ *                  std::vector<Real>& page_rank, Real damping_factor,
+ *                  Real threshold, size_t max_iters, size_t num_threads);
+ *
+ * Graph must expose the CSR the way NWGraph's indexed_struct_of_arrays does
+ * (containers/compressed.hpp:67-69): `indices_` (offsets, N+1 entries) and
+ * `std::get<0>(to_be_indexed_)` (neighbours), both contiguous uint32, and
+ * size().  These are the members the reference's own raw-CSR code reads
+ * (experimental/algorithms/page_rank.hpp:118-120, apb/spmv.cpp:48-50).  A
+ * plain `nw::graph::csr_view` overload serves callers without NWGraph.
+ *
+ * Semantics: Jacobi sweeps in FP32 on the GPU (the reference's v11 sweeps in
+ * place, Gauss-Seidel; both reach the same fixed point, the sweep count
+ * differs); dangling mass leaks as in the reference; the result is written
+ * into the caller's vector, which must already hold graph.size() entries
+ * (page_rank.hpp:102).  num_threads is ignored (unused in the reference too,
+ * page_rank.hpp:91).  Failures (no GPU, malformed CSR) throw
+ * std::runtime_error; not converging within max_iters returns normally.
+ */
+#ifndef NWGR_PAGE_RANK_HPP
+#define NWGR_PAGE_RANK_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "../nwgr_pagerank.h"
+
+namespace nw::graph {
+
+/* CSR given as raw arrays: offsets[n+1], indices[m] (row i = in-neighbours). */
+struct csr_view {
+  const uint32_t* offsets;
+  const uint32_t* indices;
+  uint32_t n;
+  uint64_t m;
+  using vertex_id_type = uint32_t;
+  std::size_t size() const { return n; }
+};
+
+template <class G>
+concept nwgr_csr_graph = requires(const G& g) {
+  { g.indices_.data() } -> std::convertible_to<const uint32_t*>;
+  { std::get<0>(g.to_be_indexed_).data() } -> std::convertible_to<const uint32_t*>;
+  { g.size() } -> std::convertible_to<std::size_t>;
+};
+
+namespace nwgr_detail {
+
+inline void check(int rc) {
+  if (rc != NWGR_OK) throw std::runtime_error(std::string("nwgr: ") + nwgr_last_error());
+}
+
+/* Device used by the drop-in (default 0). */
+inline int& device() {
+  static int dev = 0;
+  return dev;
+}
+
+template <typename Real>
+void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+         const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank, Real damping_factor,
+         Real threshold, std::size_t max_iters) {
+  static_assert(std::is_floating_point_v<Real>, "Real must be a floating-point type");
+  if (page_rank.size() < n) throw std::runtime_error("nwgr: page_rank must hold graph.size() entries");
+  if (degrees.size() < n) throw std::runtime_error("nwgr: degrees must hold graph.size() entries");
+  nwgr_pr_graph* g = nullptr;
+  check(nwgr_pr_graph_create(offsets, indices, n, m, degrees.data(), device(), &g));
+  struct Guard {
+    nwgr_pr_graph* g;
+    ~Guard() { nwgr_pr_graph_destroy(g); }
+  } guard{g};
+  nwgr_pr_params p;
+  nwgr_pr_params_default(&p);
+  p.damping = static_cast<double>(damping_factor);
+  p.tol = static_cast<double>(threshold);
+  p.max_iters = max_iters > 0xffffffffu ? 0xffffffffu : static_cast<uint32_t>(max_iters);
+  p.dangling = NWGR_DANGLING_LEAK;
+  nwgr_pr_result res;
+  if constexpr (std::is_same_v<Real, float>) {
+    check(nwgr_pr_run_f32(g, &p, page_rank.data(), &res));
+  } else {
+    std::vector<float> tmp(n);
+    check(nwgr_pr_run_f32(g, &p, tmp.data(), &res));
+    for (uint32_t i = 0; i < n; ++i) page_rank[i] = static_cast<Real>(tmp[i]);
+  }
+}
+
+}  // namespace nwgr_detail
+
+/* Select the GPU used by page_rank (default 0). */
+inline void page_rank_set_device(int device) { nwgr_detail::device() = device; }
+
+template <nwgr_csr_graph Graph, typename Real>
+void page_rank(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+               std::vector<Real>& page_rank, Real damping_factor, Real threshold, std::size_t max_iters,
+               std::size_t /*num_threads*/) {
+  static_assert(sizeof(typename Graph::vertex_id_type) == sizeof(uint32_t),
+                "the device path takes uint32 vertex ids (util/defaults.hpp:23-24)");
+  const auto& offs = graph.indices_;
+  const auto& nbrs = std::get<0>(graph.to_be_indexed_);
+  const uint32_t n = static_cast<uint32_t>(graph.size());
+  std::vector<uint32_t> deg(degrees.begin(), degrees.end());
+  nwgr_detail::run<Real>(reinterpret_cast<const uint32_t*>(offs.data()),
+                         reinterpret_cast<const uint32_t*>(nbrs.data()), n,
+                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), deg, page_rank, damping_factor,
+                         threshold, max_iters);
+}
+
+template <typename Real>
+void page_rank(const csr_view& graph, const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank,
+               Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
+  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
+                         threshold, max_iters);
+}
+
+}  // namespace nw::graph
+
+#endif  // NWGR_PAGE_RANK_HPP

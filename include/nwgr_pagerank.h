@@ -1,0 +1,146 @@
+/*
+ * nwgr_pagerank.h -- C ABI of the MI355X PageRank library (libnwgr_pagerank.so).
+ *
+ * Drop-in boundary for NWGraph's PageRank hot path.  The reference entry
+ * point is the C++20 template
+ *
+ *   template <adjacency_list_graph Graph, typename Real>
+ *   void nw::graph::page_rank(const Graph& graph,
+ *                             const std::vector<vertex_id_type>& degrees,
+ *                             std::vector<Real>& page_rank, Real damping_factor,
+ *                             Real threshold, size_t max_iters, size_t num_threads);
+ *   (include/nwgraph/algorithms/page_rank.hpp:89-91, driven by
+ *    bench/pr.cpp:137-139 with Graph = adjacency<1>, Real = float)
+ *
+ * which the C++ header include/nwgr/page_rank.hpp re-declares on top of this
+ * ABI.  Plain pointers and sizes only; no HIP, torch or C++ types cross it
+ * (streams are passed as void*, which is what hipStream_t is).
+ *
+ * Graph input ("in-CSR", the reference's adjacency<1>): offsets[n+1] with
+ * offsets[0] = 0 and offsets[n] = m (uint32, as default_index_t,
+ * util/defaults.hpp:23-24), indices[m] uint32 = the in-neighbours of each row
+ * (compressed.hpp:67-69, 179-186).  degrees[n] = the per-vertex divisor of
+ * the contribution, the out-degree when built by build_degrees
+ * (bench/common.hpp:109-120); NULL means "count out-degrees from indices".
+ *
+ * Iteration semantics: Jacobi sweeps as page_rank_v10
+ * (experimental/algorithms/page_rank.hpp:450-505): from r_0 = 1/n (or init),
+ *   r'_i = (1-d)*t_i + d*( sum_{j in row i} r_j/deg_j + D*u_i ),
+ *   D = sum_{deg_j = 0} r_j,  u = 0 (LEAK, the reference), 1/n (UNIFORM) or t
+ *   (TELEPORT);  error = sum_i |r'_i - r_i| accumulated in double;
+ * stop after the first sweep with error < tol, or after max_iters sweeps
+ * (page_rank.hpp:121-147).  Ranks are FP32 with FP32 accumulation.  Not
+ * converging within max_iters is not an error (as in the reference): the
+ * call returns NWGR_OK with result.converged = 0.
+ *
+ * Errors: every function returns an nwgr_status; on failure a message is
+ * available from nwgr_last_error() (thread-local).  The reference has no
+ * error reporting on this path (page_rank returns void).
+ *
+ * Threading: a graph handle is used by one host thread at a time; distinct
+ * handles may be used concurrently.  num_threads of the reference has no
+ * counterpart (it is unused there as well, page_rank.hpp:91).
+ */
+#ifndef NWGR_PAGERANK_H
+#define NWGR_PAGERANK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nwgr_status {
+  NWGR_OK = 0,
+  NWGR_EINVAL = 1,   /* invalid argument or malformed CSR          */
+  NWGR_ENOMEM = 2,   /* device or host allocation failed            */
+  NWGR_EDEVICE = 3,  /* HIP runtime error / no usable device        */
+  NWGR_ENOTSUP = 4,  /* feature not supported for this handle       */
+} nwgr_status;
+
+typedef enum nwgr_dangling {
+  NWGR_DANGLING_LEAK = 0,     /* reference behaviour: mass of deg-0 vertices is lost */
+  NWGR_DANGLING_UNIFORM = 1,  /* redistributed as 1/n                                */
+  NWGR_DANGLING_TELEPORT = 2, /* redistributed along the teleport vector             */
+} nwgr_dangling;
+
+typedef struct nwgr_pr_graph nwgr_pr_graph; /* device-resident graph, opaque */
+
+typedef struct nwgr_pr_params {
+  double damping;       /* d, the reference's damping_factor (0.85)                   */
+  double tol;           /* L1 threshold, the reference's threshold                    */
+  uint32_t max_iters;   /* sweep cap, the reference's max_iters                       */
+  uint32_t dangling;    /* nwgr_dangling                                              */
+  const float* teleport;/* n entries in caller vertex order, or NULL (= 1/n).         */
+                        /* Host pointer for nwgr_pr_run_f32, device pointer for the   */
+                        /* _device entry points.                                      */
+  const float* init;    /* initial ranks (n), same pointer rules, or NULL (= 1/n)     */
+  uint32_t chunk;       /* sweeps per captured hipGraph launch; 0 = default           */
+  uint32_t trace;       /* 1: print the reference's per-sweep trace table to stdout   */
+} nwgr_pr_params;
+
+typedef struct nwgr_pr_result {
+  uint32_t iters;     /* sweeps performed                                              */
+  uint32_t converged; /* 1 if the last sweep's error < tol                             */
+  double error;       /* L1 change of the last sweep                                   */
+  double ms;          /* device time of the call (init + sweeps + output), hipEvents   */
+} nwgr_pr_result;
+
+/* Fills the reference defaults of bench/pr.cpp: damping 0.85, tol 1e-4
+ * (pr.cpp:27), max_iters 20 (pr.cpp:26), LEAK, no teleport/init. */
+void nwgr_pr_params_default(nwgr_pr_params* p);
+
+/* Upload an in-CSR (host arrays) to `device` and prepare it for sweeps.
+ * Replaces the graph/degrees arguments of page_rank (page_rank.hpp:89-91):
+ * the caller keeps ownership of its arrays; the handle owns device memory.
+ * Validates offsets (monotone, offsets[0]=0, offsets[n]=m) and indices < n. */
+int nwgr_pr_graph_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                         const uint32_t* degrees, int device, nwgr_pr_graph** out);
+
+/* Same from device arrays (e.g. torch tensors) on `stream`.  The arrays are
+ * read during the call only.  Indices are range-checked on the device. */
+int nwgr_pr_graph_create_device(const uint32_t* d_offsets, const uint32_t* d_indices, uint32_t n,
+                                uint64_t m, const uint32_t* d_degrees, int device, void* stream,
+                                nwgr_pr_graph** out);
+
+void nwgr_pr_graph_destroy(nwgr_pr_graph* g);
+
+uint32_t nwgr_pr_graph_n(const nwgr_pr_graph* g);
+uint64_t nwgr_pr_graph_m(const nwgr_pr_graph* g);
+uint64_t nwgr_pr_graph_device_bytes(const nwgr_pr_graph* g);
+/* Time (ms) the handle spent building its device layout (relabel, bins). */
+double nwgr_pr_graph_setup_ms(const nwgr_pr_graph* g);
+
+/* Plain PageRank; the ranks are written to the host array r_out[n] in the
+ * caller's vertex order.  Replaces the body of page_rank
+ * (page_rank.hpp:92-147); r_out plays the role of the page_rank vector. */
+int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res);
+
+/* Same with the output in device memory (d_r_out[n]) and all work enqueued
+ * on `stream` (NULL = the null stream).  Synchronises `stream` before
+ * returning, since the sweep count is decided on the device. */
+int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
+                           nwgr_pr_result* res, void* stream);
+
+/* Per-sweep L1 errors of the last run (up to `cap` entries); returns the
+ * number written.  Backs the reference's trace table (page_rank.hpp:42-52). */
+uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap);
+
+/* Device timing of the last run's sweep kernels: summed event time of the
+ * captured sweep chunks (*sweep_ms), sweep-kernel launches (*launches: the
+ * sweeps plus the launch that found the stop condition, plus any launched
+ * past it) and chunks.  Backs the per-sweep time of the trace table
+ * (page_rank.hpp:123,142).  Any output pointer may be NULL. */
+int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* launches, uint32_t* chunks);
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* nwgr_last_error(void);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int nwgr_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NWGR_PAGERANK_H */

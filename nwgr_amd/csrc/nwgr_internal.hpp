@@ -1,0 +1,238 @@
+// Internal declarations shared by the HIP translation units of libnwgr_pagerank.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nwgr_pagerank.h"
+
+namespace nwgr {
+
+// ---------------------------------------------------------------------------
+// errors: internal code throws Error; the C boundary converts it to a status.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define NWGR_HIP(call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      int code_ = (e_ == hipErrorOutOfMemory) ? NWGR_ENOMEM : NWGR_EDEVICE;                   \
+      throw ::nwgr::Error(code_, std::string(#call) + ": " + hipGetErrorString(e_) + " (" +    \
+                                     __FILE__ + ":" + std::to_string(__LINE__) + ")");        \
+    }                                                                                         \
+  } while (0)
+
+#define NWGR_REQUIRE(cond, msg)                                   \
+  do {                                                            \
+    if (!(cond)) throw ::nwgr::Error(NWGR_EINVAL, (msg));          \
+  } while (0)
+
+// Converts exceptions to a status at the ABI boundary.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return NWGR_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return NWGR_ENOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return NWGR_EDEVICE;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device memory owned by a handle
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void alloc(size_t n) {
+    reset();
+    if (n == 0) n = 16;
+    NWGR_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Work decomposition of one sweep.
+//
+// Rows (destination vertices) are relabelled so that in-degree is
+// non-increasing with the new id.  The rows then fall into contiguous
+// segments; each segment is processed by groups of G lanes per row
+// (G = 1 .. 64), or, for hub rows, by whole workgroups over fixed-size edge
+// chunks.  A "unit" is the work one workgroup takes from the queue at a time.
+constexpr int kBlock = 256;      // threads per workgroup of the sweep kernels
+constexpr int kMaxSegs = 16;
+
+struct Seg {
+  uint32_t row_lo, row_hi;    // rows [row_lo, row_hi)
+  uint32_t G;                 // lanes per row (power of two); 0 = hub chunks
+  uint32_t rows_per_unit;     // rows a unit covers (multiple of kBlock / G)
+  uint32_t unit_lo, unit_hi;  // global unit ids [unit_lo, unit_hi)
+};
+
+struct SegTable {
+  Seg s[kMaxSegs];
+  uint32_t nseg;
+  uint32_t total_units;
+};
+
+// One chunk of a hub row.  Chunks of one row are consecutive units.
+struct HubUnit {
+  uint32_t row;
+  uint32_t e_lo, e_hi;    // edge range of this chunk
+  uint32_t first;         // index of the row's first chunk in the hub unit list
+  uint32_t nchunks;       // chunks of this row
+  uint32_t hub;           // hub row ordinal (ticket slot)
+};
+
+// Device-side solver state.  Cross-workgroup values are exchanged across
+// kernel boundaries (ping-pong on the sweep parity), except the work queue
+// counters and hub tickets, which use read-modify-write atomics only.
+struct State {
+  uint32_t ctr[2];    // ctr[p]: sweeps completed when the sweep of parity p ran
+  uint32_t work[2];   // work queue counters, one per parity
+  uint32_t done;      // 1 once converged or capped
+  uint32_t iters;     // sweeps performed (valid once done)
+  double err;         // error of the last sweep (valid once done)
+  double pad;
+};
+
+// Host-mapped mirror the solver publishes into: progress every sweep, the
+// result when it stops.
+struct HostFlag {
+  volatile uint32_t done;
+  volatile uint32_t iters;
+  volatile double err;
+  volatile uint32_t progress;   // sweeps completed as of the latest launch
+  volatile uint32_t pad;
+  volatile double last_err;     // error of sweep `progress`
+};
+
+// Timing of the last solve, from events around each captured chunk.
+struct SolveProfile {
+  double chunk_ms = 0;      // summed device time of the sweep chunks
+  uint32_t launches = 0;    // sweep kernel launches (sweeps + the stopping launch + idle ones)
+  uint32_t chunks = 0;
+};
+
+struct Tuning {
+  uint32_t hub_threshold = 4096;   // rows with in-degree > this are hubs
+  uint32_t hub_chunk = 4096;       // edges per hub unit
+  uint32_t unit_edges = 8192;      // target edges per non-hub unit
+  uint32_t zero_rows_per_unit = 8192;
+  uint32_t blocks_per_cu = 8;
+  uint32_t chunk = 16;             // sweeps per captured graph
+};
+
+struct Graph {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t num_cu = 0;
+  uint32_t grid = 0;              // sweep workgroups
+  double setup_ms = 0;
+  Tuning tune;
+
+  // relabelled layout
+  DevBuf perm;      // u32[n]  new -> original id
+  DevBuf iperm;     // u32[n]  original -> new id
+  DevBuf offs;      // u32[n+1]
+  DevBuf idx;       // u32[m]  new source ids
+  DevBuf epos;      // u32[m]  new edge position -> original edge position
+  DevBuf degf;      // f32[n]  divisor of the contribution (caller degrees), new order
+  DevBuf hub_units; // HubUnit[]
+  DevBuf hub_part;  // u32[] (float bits) hub chunk partial sums
+  DevBuf hub_ticket;// u32[] per hub row
+  std::vector<uint32_t> seg_rows;  // host copy of in-degrees (new order), for planning
+  SegTable segs{};
+  uint32_t n_hub_units = 0, n_hubs = 0;
+
+  // iteration buffers
+  DevBuf r[2], c[2];
+  DevBuf tel;       // f32[n] permuted teleport (when given)
+  DevBuf partials;  // double[2][grid][2]
+  DevBuf state;     // State
+  DevBuf hist;      // double[hist_cap]
+  uint32_t hist_cap = 0;
+  HostFlag* hflag = nullptr;   // pinned, mapped
+  HostFlag* hflag_dev = nullptr;
+
+  // captured chunk graphs of the plain solve: [parity of the first sweep][length]
+  static constexpr uint32_t kMaxChunk = 32;
+  hipGraphExec_t exec[2][kMaxChunk + 1] = {};
+  uint64_t exec_key = 0;
+  hipStream_t own_stream = nullptr;
+
+  std::vector<double> last_errs;
+  SolveProfile last_profile;
+
+  void drop_graphs() {
+    for (auto& row : exec)
+      for (auto& e : row)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+  }
+
+  ~Graph();
+};
+
+inline uint64_t bytes_of(const Graph& g) {
+  uint64_t b = 0;
+  for (const DevBuf* d : {&g.perm, &g.iperm, &g.offs, &g.idx, &g.epos, &g.degf, &g.hub_units,
+                          &g.hub_part, &g.hub_ticket, &g.r[0], &g.r[1], &g.c[0], &g.c[1], &g.tel,
+                          &g.partials, &g.state, &g.hist})
+    b += d->bytes;
+  return b;
+}
+
+// graph construction (graph.hip)
+Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                    const uint32_t* degrees, int device, hipStream_t stream, bool inputs_on_device);
+
+// plain solve (pagerank.hip)
+void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
+
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    NWGR_HIP(hipGetDevice(&prev));
+    if (prev != dev) NWGR_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+}  // namespace nwgr

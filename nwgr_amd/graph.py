@@ -1,0 +1,146 @@
+"""Host-side graph types and builders (the data either side of the hot path).
+
+``CSRGraph`` is the reference's ``adjacency<1>`` ("in-CSR"): row ``i`` lists
+the in-neighbours ``j`` of vertex ``i`` (edges ``j -> i``); uint32 offsets
+(n+1) and indices (m), as ``indexed_struct_of_arrays``
+(include/nwgraph/containers/compressed.hpp:60-90, 179-186, 215-218).
+
+* ``build_adjacency``  -- fill_directed<1> (include/nwgraph/build.hpp:209-265),
+  with the optional uniq (:425-435) and remove_self_loops (:437-443).
+* ``build_degrees``    -- bench/common.hpp:109-120 (out-degree by counting
+  every neighbour occurrence).
+* ``rmat`` / ``erdos_renyi`` / ``rmat_many`` -- synthetic inputs (no
+  reference counterpart; the recipes are specified in include/nwgr_graphgen.h).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import graphgen_lib
+
+CSR_DEDUP = 1
+CSR_DROP_SELF_LOOPS = 2
+
+RMAT_A, RMAT_B, RMAT_C = 0.57, 0.19, 0.19
+DEFAULT_SEED = 27491095  # the reference drivers' default --seed (bench/bfs.cpp:32)
+
+
+@dataclass
+class CSRGraph:
+    """In-CSR adjacency: offsets[n+1], indices[m] (uint32)."""
+
+    offsets: np.ndarray
+    indices: np.ndarray
+
+    def __post_init__(self):
+        self.offsets = np.ascontiguousarray(self.offsets, dtype=np.uint32)
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32)
+
+    @property
+    def n(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def m(self) -> int:
+        return len(self.indices)
+
+    def size(self) -> int:
+        """Number of vertices, as graph.size() (compressed.hpp:218)."""
+        return self.n
+
+    def __len__(self) -> int:
+        return self.n
+
+    def __getitem__(self, i: int) -> np.ndarray:
+        """Row view graph[i] (compressed.hpp:215-216)."""
+        return self.indices[self.offsets[i]:self.offsets[i + 1]]
+
+    def in_degrees(self) -> np.ndarray:
+        return np.diff(self.offsets).astype(np.uint32)
+
+
+def _take(h) -> CSRGraph:
+    L = graphgen_lib()
+    try:
+        n = L.nwgr_host_csr_n(h)
+        m = L.nwgr_host_csr_m(h)
+        off = np.ctypeslib.as_array(L.nwgr_host_csr_offsets(h), (n + 1,)).copy()
+        ind = np.ctypeslib.as_array(L.nwgr_host_csr_indices(h), (m,)).copy() if m else np.zeros(0, np.uint32)
+        return CSRGraph(off, ind)
+    finally:
+        L.nwgr_host_csr_free(h)
+
+
+def _check_gen(rc: int) -> None:
+    if rc != 0:
+        raise ValueError(graphgen_lib().nwgr_gen_last_error().decode(errors="replace"))
+
+
+def build_adjacency(src, dst, n: int | None = None, dedup: bool = False, drop_self_loops: bool = False,
+                    nthreads: int = 0) -> CSRGraph:
+    """In-CSR of the directed edge list src[i] -> dst[i] (rows sorted)."""
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    if src.shape != dst.shape:
+        raise ValueError("src and dst differ in length")
+    if n is None:  # N = max id + 1, as edge_list::push_back (edge_list.hpp:143-154)
+        n = int(max(src.max(initial=0), dst.max(initial=0))) + 1 if len(src) else 0
+    flags = (CSR_DEDUP if dedup else 0) | (CSR_DROP_SELF_LOOPS if drop_self_loops else 0)
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_csr_build(n, len(src), src.ctypes.data, dst.ctypes.data, flags, nthreads,
+                                             ctypes.byref(h)))
+    return _take(h)
+
+
+def build_degrees(graph: CSRGraph) -> np.ndarray:
+    """Out-degrees: each neighbour j of each row counts once (common.hpp:109-120)."""
+    return np.bincount(graph.indices, minlength=graph.n).astype(np.uint32)
+
+
+def rmat(scale: int, edge_factor: int = 16, seed: int = DEFAULT_SEED, a: float = RMAT_A, b: float = RMAT_B,
+         c: float = RMAT_C, nthreads: int = 0) -> CSRGraph:
+    """Directed R-MAT, N = 2**scale, self-loops and duplicates removed."""
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_gen_rmat(scale, edge_factor, seed, a, b, c, nthreads, ctypes.byref(h)))
+    return _take(h)
+
+
+def erdos_renyi(n: int, m_draws: int, seed: int = DEFAULT_SEED, nthreads: int = 0) -> CSRGraph:
+    """Directed G(n, m) drawn with replacement, self-loops and duplicates removed."""
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_gen_er(n, m_draws, seed, nthreads, ctypes.byref(h)))
+    return _take(h)
+
+
+def rmat_many(ns, edge_factor: int = 8, seed: int = DEFAULT_SEED, a: float = RMAT_A, b: float = RMAT_B,
+              c: float = RMAT_C, nthreads: int = 0):
+    """Independent R-MAT graphs stored block-diagonally in one CSR.
+
+    Returns (graph, vbase) with vbase[g] the first global vertex of graph g
+    (len(ns) + 1 entries)."""
+    ns = np.ascontiguousarray(ns, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_gen_rmat_many(len(ns), ns.ctypes.data, edge_factor, seed, a, b, c, nthreads,
+                                                 ctypes.byref(h)))
+    vbase = np.zeros(len(ns) + 1, np.uint32)
+    vbase[1:] = np.cumsum(ns, dtype=np.uint64).astype(np.uint32)
+    return _take(h), vbase
+
+
+def transpose(graph: CSRGraph, nthreads: int = 0):
+    """Out-CSR of an in-CSR: (offsets, indices, perm) with perm[k] the in-CSR
+    position of out-CSR entry k."""
+    n, m = graph.n, graph.m
+    off = np.zeros(n + 1, np.uint32)
+    ind = np.zeros(max(m, 1), np.uint32)
+    perm = np.zeros(max(m, 1), np.uint32)
+    _check_gen(graphgen_lib().nwgr_csr_transpose(n, m, graph.offsets.ctypes.data, graph.indices.ctypes.data,
+                                                 off.ctypes.data, ind.ctypes.data, perm.ctypes.data, nthreads))
+    return off, ind[:m], perm[:m]
+
+
+def splitmix64_at(key: int, k: int) -> int:
+    return int(graphgen_lib().nwgr_splitmix64_at(key, k))

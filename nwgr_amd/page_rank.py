@@ -1,0 +1,165 @@
+"""PageRank on the MI355X through the C ABI (include/nwgr_pagerank.h).
+
+``page_rank`` mirrors the reference call shape
+``nw::graph::page_rank(graph, degrees, page_rank, damping_factor, threshold,
+max_iters, num_threads)`` (include/nwgraph/algorithms/page_rank.hpp:89-91):
+it writes the ranks into the caller's pre-sized array and returns None; not
+converging within ``max_iters`` is not an error (page_rank.hpp:121-147).
+Arithmetic is FP32 on the device with Jacobi sweeps
+(experimental/algorithms/page_rank.hpp:450-505).
+
+``PageRankGraph`` is the device-resident handle for repeated solves, the
+dangling-mass / teleport extensions and the device-memory (torch) path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import DANGLING, PrParams, PrResult, check, pagerank_lib
+from .graph import CSRGraph
+
+
+@dataclass
+class RunResult:
+    iters: int        # sweeps performed
+    converged: bool   # last sweep's L1 change < tol
+    error: float      # L1 change of the last sweep
+    ms: float         # device time of the call
+
+
+def _params(damping, tol, max_iters, dangling, chunk, trace) -> PrParams:
+    p = PrParams()
+    pagerank_lib().nwgr_pr_params_default(ctypes.byref(p))
+    p.damping = float(damping)
+    p.tol = float(tol)
+    p.max_iters = int(max_iters)
+    if isinstance(dangling, str):
+        if dangling not in DANGLING:
+            raise ValueError(f"dangling must be one of {sorted(DANGLING)}")
+        dangling = DANGLING[dangling]
+    p.dangling = int(dangling)
+    p.chunk = int(chunk)
+    p.trace = 1 if trace else 0
+    return p
+
+
+def device_count() -> int:
+    return int(pagerank_lib().nwgr_device_count())
+
+
+class PageRankGraph:
+    """A graph uploaded to one GPU, ready for PageRank sweeps."""
+
+    def __init__(self, graph: CSRGraph | None = None, degrees=None, device: int = 0, *, _handle=None):
+        L = pagerank_lib()
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            off = np.ascontiguousarray(graph.offsets, dtype=np.uint32)
+            ind = np.ascontiguousarray(graph.indices, dtype=np.uint32)
+            deg = None if degrees is None else np.ascontiguousarray(degrees, dtype=np.uint32)
+            if deg is not None and len(deg) != graph.n:
+                raise ValueError("degrees must have one entry per vertex")
+            check(L.nwgr_pr_graph_create(off.ctypes.data, ind.ctypes.data if len(ind) else None, graph.n,
+                                         graph.m, None if deg is None else deg.ctypes.data, device,
+                                         ctypes.byref(self._h)))
+        self.n = int(L.nwgr_pr_graph_n(self._h))
+        self.m = int(L.nwgr_pr_graph_m(self._h))
+        self.device = device
+
+    @classmethod
+    def from_device(cls, offsets, indices, n: int, m: int, degrees=None, device: int = 0, stream=None):
+        """Create from device arrays given as raw pointers (ints), e.g.
+        torch tensors' data_ptr(); `stream` is a hipStream_t as int or None."""
+        L = pagerank_lib()
+        h = ctypes.c_void_p()
+        check(L.nwgr_pr_graph_create_device(offsets, indices, n, m, degrees, device, stream, ctypes.byref(h)))
+        return cls(_handle=h, device=device)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            pagerank_lib().nwgr_pr_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(pagerank_lib().nwgr_pr_graph_device_bytes(self._h))
+
+    @property
+    def setup_ms(self) -> float:
+        return float(pagerank_lib().nwgr_pr_graph_setup_ms(self._h))
+
+    def last_profile(self) -> dict:
+        """Sweep-kernel timing of the last solve: summed event time of the
+        captured sweep chunks, number of sweep launches and chunks."""
+        ms, launches, chunks = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(pagerank_lib().nwgr_pr_last_profile(self._h, ctypes.byref(ms), ctypes.byref(launches),
+                                                  ctypes.byref(chunks)))
+        return {"sweep_ms": ms.value, "launches": launches.value, "chunks": chunks.value}
+
+    def last_errors(self) -> np.ndarray:
+        buf = np.zeros(4096, np.float64)
+        k = pagerank_lib().nwgr_pr_last_errors(self._h, buf.ctypes.data, len(buf))
+        return buf[:k].copy()
+
+    def run(self, damping=0.85, tol=1e-4, max_iters=20, dangling="leak", teleport=None, init=None,
+            chunk=0, trace=False, out=None):
+        """Solve; returns (ranks float32[n] in caller order, RunResult)."""
+        p = _params(damping, tol, max_iters, dangling, chunk, trace)
+        keep = []
+        for name, vec in (("teleport", teleport), ("init", init)):
+            if vec is not None:
+                v = np.ascontiguousarray(vec, dtype=np.float32)
+                if v.shape != (self.n,):
+                    raise ValueError(f"{name} must have shape ({self.n},)")
+                keep.append(v)
+                setattr(p, name, v.ctypes.data)
+        if out is None:
+            out = np.empty(self.n, np.float32)
+        elif out.dtype != np.float32 or out.shape != (self.n,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float32 array of n entries")
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_run_f32(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
+        return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run_device(self, out_ptr: int, damping=0.85, tol=1e-4, max_iters=20, dangling="leak",
+                   teleport_ptr=None, init_ptr=None, chunk=0, stream=None) -> RunResult:
+        """Solve with device-resident vectors given as raw pointers."""
+        p = _params(damping, tol, max_iters, dangling, chunk, False)
+        p.teleport = teleport_ptr
+        p.init = init_ptr
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_run_device_f32(self._h, ctypes.byref(p), out_ptr, ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+
+def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, max_iters=20,
+              num_threads=None, *, device: int = 0) -> None:
+    """Drop-in for nw::graph::page_rank (page_rank.hpp:89-91).
+
+    ``page_rank`` must be a pre-sized float array of graph.size() entries
+    (the reference writes without resizing, page_rank.hpp:102); it is filled
+    in place.  ``num_threads`` is accepted and ignored, as in the reference.
+    Dangling vertices leak their mass, as in the reference."""
+    if len(page_rank) != graph.size():
+        raise ValueError("page_rank must be pre-sized to graph.size()")
+    with PageRankGraph(graph, degrees, device) as g:
+        r, _ = g.run(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak")
+    page_rank[:] = r

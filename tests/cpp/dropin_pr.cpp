@@ -1,0 +1,40 @@
+// Calls the C++20 drop-in nw::graph::page_rank the way bench/pr.cpp:137-139
+// calls the reference: float ranks, damping 0.85f, tolerance and max_iters
+// from the command line.  Input file: "n m" then n+1 offsets, m indices and
+// n degrees (whitespace separated).  Prints one rank per line.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "nwgr/page_rank.hpp"
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::cerr << "usage: dropin_pr <graph.txt> <tol> <max_iters>\n";
+    return 2;
+  }
+  std::ifstream in(argv[1]);
+  uint64_t n = 0, m = 0;
+  in >> n >> m;
+  std::vector<uint32_t> offs(n + 1), idx(m), deg(n);
+  for (auto& x : offs) in >> x;
+  for (auto& x : idx) in >> x;
+  for (auto& x : deg) in >> x;
+  if (!in) {
+    std::cerr << "bad input\n";
+    return 2;
+  }
+  nw::graph::csr_view g{offs.data(), idx.data(), static_cast<uint32_t>(n), m};
+  std::vector<float> rankings(n);
+  try {
+    nw::graph::page_rank(g, deg, rankings, 0.85f, std::strtof(argv[2], nullptr),
+                         static_cast<size_t>(std::atol(argv[3])), 1);
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  }
+  for (float r : rankings) std::printf("%.9g\n", r);
+  return 0;
+}

@@ -1,0 +1,258 @@
+"""Parity of the HIP PageRank path against the FP64 oracle (needs the GPU).
+
+Gates (per workload): FP32 device ranks within L1 <= 1e-6 of the FP64 Jacobi
+oracle run for the SAME number of sweeps; sweep count within +-1 of the
+oracle's at the same tolerance; identical top-100 order (vertex-id
+tie-break).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import nwgr_amd
+import oracle
+from nwgr_amd import PageRankGraph
+from nwgr_amd._lib import NWGR_EINVAL, NwgrError
+from tests.conftest import ROOT, l1
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"leak": oracle.LEAK, "uniform": oracle.UNIFORM, "teleport": oracle.TELEPORT}
+L1_GATE = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def parity(g, deg, mode="uniform", tol=1e-6, max_iters=100, teleport=None, init=None, topk=100,
+           nthreads=16, device_graph=None):
+    pg = device_graph or PageRankGraph(g, deg)
+    r, res = pg.run(damping=0.85, tol=tol, max_iters=max_iters, dangling=mode, teleport=teleport, init=init)
+    tel64 = None if teleport is None else np.asarray(teleport, np.float64)
+    ini64 = None if init is None else np.asarray(init, np.float64)
+    r64, k64 = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 0.0, res.iters, dangling=MODES[mode],
+                             teleport=tel64, init=ini64, nthreads=nthreads)
+    _, k_ref = oracle.jacobi(g.offsets, g.indices, deg, 0.85, tol, max_iters, dangling=MODES[mode],
+                             teleport=tel64, init=ini64, nthreads=nthreads)
+    err = l1(r, r64)
+    assert err <= L1_GATE, f"L1 {err:.3e} at sweep {res.iters}"
+    assert abs(int(res.iters) - int(k_ref)) <= 1, (res.iters, k_ref)
+    k = min(topk, g.n)
+    np.testing.assert_array_equal(oracle.top_k(r, k), oracle.top_k(r64, k))
+    return r, res, r64, err
+
+
+def test_karate_golden_and_oracle(karate):
+    d, g, deg = karate
+    say("karate")
+    pg = PageRankGraph(g, deg)
+    r, res = pg.run(damping=d["damping"], tol=d["threshold"], max_iters=d["max_iters"], dangling="leak")
+    ans = np.array(d["answer"])
+    assert np.all(np.abs(r - ans) <= d["rel_tolerance"] * ans)
+    parity(g, deg, "leak", d["threshold"], d["max_iters"], topk=34)
+
+
+def test_dropin_function_fills_caller_array(karate):
+    d, g, deg = karate
+    ranks = np.zeros(g.size(), np.float32)
+    assert nwgr_amd.page_rank(g, deg, ranks, 0.85, 1e-7, 100, 8) is None
+    r64, _ = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-7, 100)
+    assert l1(ranks, r64) < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["leak", "uniform", "teleport"])
+@pytest.mark.parametrize("scale", [10, 14])
+def test_rmat_small_parity(scale, mode):
+    say(f"rmat s{scale} {mode}")
+    g = nwgr_amd.rmat(scale, 16)
+    deg = nwgr_amd.build_degrees(g)
+    tel = np.random.default_rng(scale).dirichlet(np.ones(g.n)).astype(np.float32) if mode == "teleport" else None
+    parity(g, deg, mode, 1e-6, 100, teleport=tel)
+
+
+def test_config1_rmat_s16_tol_1e4():
+    """R-MAT s16 ef16, uniform teleport, dangling UNIFORM, tol 1e-4, max 20."""
+    say("rmat s16 config1")
+    g = nwgr_amd.rmat(16, 16)
+    deg = nwgr_amd.build_degrees(g)
+    parity(g, deg, "uniform", 1e-4, 20)
+
+
+def test_er_parity():
+    say("er 2^16")
+    g = nwgr_amd.erdos_renyi(1 << 16, 16 << 16)
+    deg = nwgr_amd.build_degrees(g)
+    parity(g, deg, "uniform", 1e-6, 100)
+
+
+def test_hub_rows_split_in_chunks():
+    """In-degree far above the hub threshold: chunked hub rows combine exactly."""
+    say("hubs")
+    n = 200_000
+    rng = np.random.default_rng(3)
+    # two hubs with 150k and 60k in-edges plus a random background
+    src = np.concatenate([np.arange(1, 150_001), np.arange(100, 60_100), rng.integers(0, n, 400_000)])
+    dst = np.concatenate([np.zeros(150_000, np.int64), np.ones(60_000, np.int64), rng.integers(0, n, 400_000)])
+    g = nwgr_amd.build_adjacency(src, dst, n=n, dedup=True, drop_self_loops=True)
+    deg = nwgr_amd.build_degrees(g)
+    assert np.diff(g.offsets).max() > 100_000
+    parity(g, deg, "uniform", 1e-7, 100)
+    parity(g, deg, "leak", 1e-7, 100)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_edgeless_graphs(n):
+    g = nwgr_amd.build_adjacency([], [], n=n)
+    deg = np.zeros(n, np.uint32)
+    r, res = PageRankGraph(g, deg).run(tol=1e-9, max_iters=10, dangling="leak")
+    np.testing.assert_allclose(r, 0.15 / n, rtol=1e-6)
+    assert res.iters == 2
+    r, res = PageRankGraph(g, deg).run(tol=1e-9, max_iters=10, dangling="uniform")
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-6)
+    assert res.iters == 1
+
+
+def test_iteration_cap_and_zero_sweeps():
+    g = nwgr_amd.rmat(10, 8)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    r, res = pg.run(tol=0.0, max_iters=3, dangling="uniform")
+    assert res.iters == 3 and not res.converged
+    r64, _ = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 0.0, 3, dangling=oracle.UNIFORM)
+    assert l1(r, r64) < 1e-7
+    r0, res0 = pg.run(tol=1e-6, max_iters=0)
+    assert res0.iters == 0
+    np.testing.assert_allclose(r0, 1.0 / g.n, rtol=1e-7)
+    for cap in (1, 2, 15, 16, 17, 33):  # around the captured chunk length
+        _, rc = pg.run(tol=0.0, max_iters=cap, chunk=16)
+        assert rc.iters == cap
+
+
+def test_per_sweep_errors_match_oracle():
+    g = nwgr_amd.rmat(12, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    _, res = pg.run(tol=1e-8, max_iters=60, dangling="uniform")
+    errs = pg.last_errors()
+    _, k, h = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-8, 60, dangling=oracle.UNIFORM, return_hist=True)
+    n = min(len(errs), len(h))
+    assert n >= res.iters - 1
+    np.testing.assert_allclose(errs[:10], h[:10], rtol=1e-4)
+
+
+def test_init_vector_and_warm_start():
+    g = nwgr_amd.rmat(12, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    r_conv, res1 = pg.run(tol=1e-7, max_iters=100, dangling="uniform")
+    init = np.random.default_rng(0).dirichlet(np.ones(g.n)).astype(np.float32)
+    parity(g, deg, "uniform", 1e-6, 100, init=init, device_graph=pg)
+    _, res2 = pg.run(tol=1e-6, max_iters=100, dangling="uniform", init=r_conv)
+    assert res2.iters <= 2
+
+
+def test_deterministic_bits():
+    g = nwgr_amd.rmat(14, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    a, ra = pg.run(tol=1e-6, max_iters=100, dangling="uniform")
+    b, rb = pg.run(tol=1e-6, max_iters=100, dangling="uniform")
+    np.testing.assert_array_equal(a, b)
+    assert ra.iters == rb.iters and ra.error == rb.error
+
+
+def test_caller_degrees_used_as_given(karate):
+    """The reference divides by whatever degrees it is given (page_rank.hpp:116,135)."""
+    d, g, deg = karate
+    deg2 = deg * 2
+    r, res = PageRankGraph(g, deg2).run(tol=1e-9, max_iters=200, dangling="leak")
+    r64, _ = oracle.jacobi(g.offsets, g.indices, deg2, 0.85, 0.0, res.iters)
+    assert l1(r, r64) < 1e-6
+
+
+def test_null_degrees_means_out_degree(karate):
+    d, g, deg = karate
+    a, _ = PageRankGraph(g, None).run(tol=1e-7, max_iters=100)
+    b, _ = PageRankGraph(g, deg).run(tol=1e-7, max_iters=100)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_malformed_csr_rejected():
+    bad = nwgr_amd.CSRGraph(np.array([0, 2, 1, 3], np.uint32), np.array([0, 1, 2], np.uint32))
+    with pytest.raises(NwgrError) as e:
+        PageRankGraph(bad)
+    assert e.value.code == NWGR_EINVAL
+    bad = nwgr_amd.CSRGraph(np.array([0, 1, 2], np.uint32), np.array([0, 7], np.uint32))
+    with pytest.raises(NwgrError) as e:
+        PageRankGraph(bad)
+    assert e.value.code == NWGR_EINVAL
+
+
+def test_bad_params_rejected(karate):
+    _, g, deg = karate
+    pg = PageRankGraph(g, deg)
+    with pytest.raises(NwgrError):
+        pg.run(damping=1.5)
+    with pytest.raises(NwgrError):
+        pg.run(dangling=2)  # TELEPORT without a teleport vector
+
+
+def test_device_pointers_from_torch():
+    import torch
+    g = nwgr_amd.rmat(12, 16)
+    deg = nwgr_amd.build_degrees(g)
+    dev = torch.device("cuda", 0)
+    off = torch.from_numpy(g.offsets.view(np.int32)).to(dev)
+    ind = torch.from_numpy(g.indices.view(np.int32)).to(dev)
+    dg = torch.from_numpy(deg.view(np.int32)).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    pg = PageRankGraph.from_device(off.data_ptr(), ind.data_ptr(), g.n, g.m, dg.data_ptr(), 0, stream)
+    out = torch.empty(g.n, dtype=torch.float32, device=dev)
+    tel = torch.full((g.n,), 1.0 / g.n, dtype=torch.float32, device=dev)
+    res = pg.run_device(out.data_ptr(), tol=1e-6, max_iters=100, dangling="teleport",
+                        teleport_ptr=tel.data_ptr(), stream=stream)
+    r = out.cpu().numpy()
+    r_host, res_host = PageRankGraph(g, deg).run(tol=1e-6, max_iters=100, dangling="uniform")
+    np.testing.assert_array_equal(r, r_host)  # TELEPORT with t = 1/n is UNIFORM
+    assert res.iters == res_host.iters
+
+
+def test_cpp_dropin_runs(karate, tmp_path):
+    d, g, deg = karate
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ missing")
+    from nwgr_amd import _lib
+    exe = tmp_path / "dropin_pr"
+    subprocess.run([gxx, "-std=c++20", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dropin_pr.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(_lib.PAGERANK_SO), "-lnwgr_pagerank",
+                    "-Wl,-rpath," + os.path.dirname(_lib.PAGERANK_SO), "-Wl,--allow-shlib-undefined"],
+                   check=True)
+    gf = tmp_path / "karate.txt"
+    with open(gf, "w") as f:
+        f.write(f"{g.n} {g.m}\n" + " ".join(map(str, g.offsets)) + "\n" + " ".join(map(str, g.indices)) + "\n"
+                + " ".join(map(str, deg)) + "\n")
+    out = subprocess.run([str(exe), str(gf), "1e-7", "100"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    r = np.array([float(x) for x in out.stdout.split()])
+    ans = np.array(d["answer"])
+    assert np.all(np.abs(r - ans) <= 0.005 * ans)
+
+
+@pytest.mark.slow
+def test_config2_rmat_s22_full_size():
+    """R-MAT s22 ef16, FP32, tol 1e-6, max 100, vs FP64 at the matched sweep."""
+    say("rmat s22 (generate)")
+    g = nwgr_amd.rmat(22, 16)
+    deg = nwgr_amd.build_degrees(g)
+    say(f"rmat s22 n={g.n} m={g.m} (solve + oracle)")
+    r, res, r64, err = parity(g, deg, "uniform", 1e-6, 100, nthreads=16)
+    say(f"rmat s22 sweeps={res.iters} L1={err:.3e} ms={res.ms:.2f}")
+    assert abs(float(r.sum(dtype=np.float64)) - 1.0) < 1e-5

@@ -1,0 +1,240 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+Every check here is independent of the oracle's own formula: the reference's
+known-answer vector, closed forms, dense linear solves, invariants of the
+iteration, and finite differences.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import l1
+
+D = 0.85
+
+
+def csr_from_edges(n, src, dst):
+    import nwgr_amd
+    return nwgr_amd.build_adjacency(np.asarray(src, np.uint32), np.asarray(dst, np.uint32), n=n)
+
+
+def random_graph(n, m, seed, dangling_frac=0.2):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    # make some vertices dangling (no out-edges)
+    dang = rng.random(n) < dangling_frac
+    keep &= ~dang[src]
+    import nwgr_amd
+    return nwgr_amd.build_adjacency(src[keep], dst[keep], n=n, dedup=True)
+
+
+def dense_fixed_point(g, deg, mode, t=None, d=D):
+    """Solve r = (1-d) t + d (P^T r + (1_dangling . r) u) exactly."""
+    n = g.n
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in g[i]:
+            A[i, j] += 1.0 / deg[j]
+    t = np.full(n, 1.0 / n) if t is None else np.asarray(t, np.float64)
+    if mode == oracle.UNIFORM:
+        u = np.full(n, 1.0 / n)
+    elif mode == oracle.TELEPORT:
+        u = t
+    else:
+        u = np.zeros(n)
+    A += np.outer(u, (deg == 0).astype(np.float64))
+    return np.linalg.solve(np.eye(n) - d * A, (1 - d) * t)
+
+
+# --- reference known-answer test ------------------------------------------------
+
+def test_karate_golden(karate):
+    """page_rank_v1 on karate (test/page_rank_test.cpp:93-101): 0.5 % relative."""
+    d, g, deg = karate
+    r, it = oracle.jacobi(g.offsets, g.indices, deg, d["damping"], d["threshold"], d["max_iters"])
+    ans = np.array(d["answer"])
+    assert np.all(np.abs(r - ans) <= d["rel_tolerance"] * ans)
+    assert it < d["max_iters"]
+
+
+def test_karate_gs_same_fixed_point(karate):
+    d, g, deg = karate
+    rj, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-13, 1000)
+    rg, _ = oracle.gs(g.offsets, g.indices, deg, D, 1e-13, 1000)
+    assert l1(rj, rg) < 1e-11
+
+
+def test_karate_iteration_counts_tol_1e4(karate):
+    """Jacobi (v1/v10) and Gauss-Seidel (v11) reach tol 1e-4 in different numbers of sweeps."""
+    _, g, deg = karate
+    _, itj = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-4, 100)
+    _, itg = oracle.gs(g.offsets, g.indices, deg, D, 1e-4, 100)
+    assert itg < itj
+
+
+# --- closed forms ---------------------------------------------------------------
+
+def test_edgeless_leak_and_uniform():
+    n = 5
+    g = csr_from_edges(n, [], [])
+    deg = np.zeros(n, np.uint32)
+    r, it, h = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-12, 10, dangling=oracle.LEAK, return_hist=True)
+    np.testing.assert_allclose(r, (1 - D) / n, rtol=1e-15)
+    assert it == 2 and abs(h[0] - D) < 1e-15 and h[1] == 0.0
+    r, it = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-12, 10, dangling=oracle.UNIFORM)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-15)
+    assert it == 1
+
+
+@pytest.mark.parametrize("kind", ["cycle", "complete"])
+def test_regular_graphs_stay_uniform(kind):
+    n = 7
+    if kind == "cycle":
+        src, dst = np.arange(n), (np.arange(n) + 1) % n
+    else:
+        src, dst = np.nonzero(~np.eye(n, dtype=bool))
+    g = csr_from_edges(n, src, dst)
+    deg = oracle.out_degree(n, g.indices)
+    r, it = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-12, 10)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-14)
+    assert it == 1
+    rg, itg = oracle.gs(g.offsets, g.indices, deg, D, 1e-12, 10)
+    np.testing.assert_allclose(rg, 1.0 / n, rtol=1e-14)
+
+
+def test_star_leaves_to_hub():
+    n = 9
+    g = csr_from_edges(n, np.arange(1, n), np.zeros(n - 1))
+    deg = oracle.out_degree(n, g.indices)
+    r1, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 0.0, 1)
+    assert abs(r1[0] - ((1 - D) / n + D * (n - 1) / n)) < 1e-15
+    np.testing.assert_allclose(r1[1:], (1 - D) / n, rtol=1e-15)
+    r, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-15, 50)
+    assert abs(r[0] - ((1 - D) / n + D * (n - 1) * (1 - D) / n)) < 1e-15
+
+
+# --- dense solves, invariants ---------------------------------------------------
+
+@pytest.mark.parametrize("mode", [oracle.LEAK, oracle.UNIFORM, oracle.TELEPORT])
+def test_fixed_point_matches_dense_solve(mode):
+    g = random_graph(60, 240, seed=5 + mode)
+    deg = oracle.out_degree(g.n, g.indices)
+    assert (deg == 0).any()
+    t = np.random.default_rng(1).dirichlet(np.ones(g.n)) if mode == oracle.TELEPORT else None
+    r, it = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-15, 2000, dangling=mode, teleport=t)
+    exact = dense_fixed_point(g, deg, mode, t)
+    assert l1(r, exact) < 1e-12
+    if mode != oracle.LEAK:
+        assert abs(r.sum() - 1.0) < 1e-12
+
+
+def test_planted_mistake_is_caught_by_dense_solve():
+    """A dropped dangling term (UNIFORM solved as LEAK) must fail the check."""
+    g = random_graph(60, 240, seed=9)
+    deg = oracle.out_degree(g.n, g.indices)
+    r_leak, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-15, 2000, dangling=oracle.LEAK)
+    assert l1(r_leak, dense_fixed_point(g, deg, oracle.UNIFORM)) > 1e-3
+
+
+@pytest.mark.parametrize("mode", [oracle.LEAK, oracle.UNIFORM])
+def test_l1_contraction_and_mass(mode):
+    g = random_graph(300, 2000, seed=11)
+    deg = oracle.out_degree(g.n, g.indices)
+    r, it, h = oracle.jacobi(g.offsets, g.indices, deg, D, 0.0, 40, dangling=mode, return_hist=True)
+    big = h[:-1] > 1e-12  # below that the error is rounding noise
+    assert big.sum() > 10
+    assert np.all(h[1:][big] <= D * h[:-1][big] * (1 + 1e-9))
+
+
+def test_parallel_jacobi_equals_serial():
+    g = random_graph(5000, 40000, seed=3)
+    deg = oracle.out_degree(g.n, g.indices)
+    r1, it1 = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-10, 100, dangling=oracle.UNIFORM)
+    r8, it8 = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-10, 100, dangling=oracle.UNIFORM, nthreads=8)
+    assert it1 == it8
+    assert l1(r1, r8) < 1e-13
+
+
+def test_matched_iterate_semantics():
+    """tol = 0 runs exactly max_iters sweeps; sweeps k and k+1 differ by err_hist[k]."""
+    g = random_graph(200, 1500, seed=4)
+    deg = oracle.out_degree(g.n, g.indices)
+    r5, it5 = oracle.jacobi(g.offsets, g.indices, deg, D, 0.0, 5)
+    r6, it6, h = oracle.jacobi(g.offsets, g.indices, deg, D, 0.0, 6, return_hist=True)
+    assert it5 == 5 and it6 == 6
+    assert abs(l1(r5, r6) - h[5]) < 1e-15
+
+
+# --- the Gauss-Seidel restatements (CPU baseline) --------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gs_parallel_one_thread_equals_serial(dtype):
+    g = random_graph(3000, 20000, seed=21)
+    deg = oracle.out_degree(g.n, g.indices)
+    rs, its = oracle.gs(g.offsets, g.indices, deg, D, 1e-6, 100, dtype=dtype)
+    rp, itp = oracle.gs_par(g.offsets, g.indices, deg, D, 1e-6, 100, nthreads=1, dtype=dtype, fast=False)
+    assert its == itp
+    np.testing.assert_array_equal(rs, rp)
+
+
+def test_gs_parallel_many_threads_same_fixed_point():
+    g = random_graph(20000, 200000, seed=22)
+    deg = oracle.out_degree(g.n, g.indices)
+    rj, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-13, 1000)
+    rp, _ = oracle.gs_par(g.offsets, g.indices, deg, D, 1e-12, 1000, nthreads=8, dtype=np.float64)
+    assert l1(rj, rp) < 1e-10
+    r32, _ = oracle.jacobi_par(g.offsets, g.indices, deg, D, 1e-7, 1000, nthreads=8)
+    assert l1(rj, r32) < 1e-5
+
+
+# --- weighted PageRank and its adjoint ---------------------------------------------
+
+def _weighted_case(seed, mode):
+    g = random_graph(40, 200, seed=seed)
+    rng = np.random.default_rng(seed)
+    w = 1.0 - rng.random(g.m)                       # (0, 1]
+    t = rng.dirichlet(np.ones(g.n))
+    wl = rng.standard_normal(g.n)
+    return g, w, t, wl
+
+
+def test_weighted_with_unit_weights_equals_plain():
+    g = random_graph(100, 700, seed=30)
+    deg = oracle.out_degree(g.n, g.indices)
+    t = np.full(g.n, 1.0 / g.n)
+    rw, _ = oracle.wpr_forward(g.offsets, g.indices, np.ones(g.m), t, D, 1e-14, 2000, dangling=oracle.UNIFORM)
+    rp, _ = oracle.jacobi(g.offsets, g.indices, deg, D, 1e-14, 2000, dangling=oracle.UNIFORM)
+    assert l1(rw, rp) < 1e-13
+
+
+@pytest.mark.parametrize("mode", [oracle.LEAK, oracle.UNIFORM, oracle.TELEPORT])
+def test_adjoint_matches_central_differences(mode):
+    g, w, t, wl = _weighted_case(31 + mode, mode)
+
+    def loss(w_, t_):
+        r, _ = oracle.wpr_forward(g.offsets, g.indices, w_, t_, D, 1e-15, 5000, dangling=mode)
+        return float(wl @ r)
+
+    r, _ = oracle.wpr_forward(g.offsets, g.indices, w, t, D, 1e-15, 5000, dangling=mode)
+    lam, dw, dt, _ = oracle.wpr_adjoint(g.offsets, g.indices, w, t, r, wl, D, 1e-15, 5000, dangling=mode)
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for k in rng.choice(g.m, 12, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (loss(wp, t) - loss(wm, t)) / (2 * h)
+        assert abs(fd - dw[k]) <= 1e-6 * max(1e-3, abs(fd)), (k, fd, dw[k])
+    for v in rng.choice(g.n, 8, replace=False):
+        tp, tm = t.copy(), t.copy()
+        tp[v] += h
+        tm[v] -= h
+        fd = (loss(w, tp) - loss(w, tm)) / (2 * h)
+        assert abs(fd - dt[v]) <= 1e-6 * max(1e-3, abs(fd)), (v, fd, dt[v])
+
+
+def test_top_k_tie_break():
+    r = np.array([0.1, 0.3, 0.3, 0.2, 0.3])
+    assert list(oracle.top_k(r, 4)) == [1, 2, 4, 3]
