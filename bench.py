@@ -55,7 +55,21 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (3 times) and exit: the known-size "
+                         "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    import torch
+    n = mib * 2**20 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
 def load_traffic(scale, ef):
@@ -86,6 +100,9 @@ def cpu_baseline(g, deg, target_s, threads):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "
               "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)

@@ -122,6 +122,54 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
                            nwgr_pr_result* res, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Differentiable weighted PageRank (no reference counterpart: the weighted
+ * form of the same Jacobi iteration, for gradient-based callers such as the
+ * torch autograd bridge in nwgr_amd/autograd.py).
+ *
+ * Edge k of the caller's in-CSR (row v, neighbour u = indices[k]) is u -> v
+ * with weight w[k] > 0.  P_uv = w_uv / s_u, s_u = sum of u's out-edge weights;
+ *   r_v = (1-d) t_v + d (sum_{u->v} P_uv r_u + D q_v),  D = sum_{s_u=0} r_u,
+ * q = 0 (LEAK), 1/n (UNIFORM), t (TELEPORT); t = p->teleport (device
+ * pointer, caller order) or 1/n when NULL.  All vectors are device arrays in
+ * the caller's vertex / edge order; the transposed CSR is built on the device
+ * on first use and kept by the handle.
+ * ------------------------------------------------------------------------- */
+
+/* Forward fixed point; ranks to d_r_out[n].  Stop rule as the plain solve. */
+int nwgr_wpr_forward_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, const float* d_w, float* d_r_out,
+                                nwgr_pr_result* res, void* stream);
+
+/* Adjoint of L(r) at the forward fixed point d_r (as returned by the
+ * forward with the same w, p): given d_grad_r = dL/dr, writes
+ * d_grad_w[m] = dL/dw (caller edge order) and d_grad_t[n] = dL/dt.
+ * The adjoint iteration lambda = d (P lambda + 1_dangling (q . lambda)) + g
+ * stops when the L1 change of lambda is below p->tol, an ABSOLUTE threshold
+ * (scale it by ||dL/dr||_1 for a relative one), or after p->max_iters sweeps. */
+int nwgr_wpr_adjoint_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, const float* d_w, const float* d_r,
+                                const float* d_grad_r, float* d_grad_w, float* d_grad_t, nwgr_pr_result* res,
+                                void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Work decomposition of the sweeps (no reference counterpart: the knobs of
+ * the degree-binned device layout, set by an autotuner or by hand).
+ * ------------------------------------------------------------------------- */
+typedef struct nwgr_pr_tuning {
+  uint32_t lds_cache;          /* vertices (the hottest after the degree relabelling) whose
+                                  contributions every sweep workgroup stages in LDS; 0 = none */
+  uint32_t hub_threshold;      /* rows with more entries are split into chunks */
+  uint32_t hub_chunk;          /* entries per hub chunk */
+  uint32_t unit_edges;         /* target entries per work unit of the other rows */
+  uint32_t zero_rows_per_unit; /* rows per work unit among rows without entries */
+  uint32_t blocks_per_cu;      /* sweep workgroups (of 1024 threads) per compute unit */
+  uint32_t chunk;              /* sweeps per captured hipGraph launch */
+} nwgr_pr_tuning;
+
+int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t);
+/* Validates and applies `t`: re-plans the work units of the handle (cheap,
+ * host side; the relabelled CSR is kept). */
+int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t);
+
 /* Per-sweep L1 errors of the last run (up to `cap` entries); returns the
  * number written.  Backs the reference's trace table (page_rank.hpp:42-52). */
 uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap);

@@ -53,6 +53,18 @@ class PrResult(ctypes.Structure):
     ]
 
 
+class PrTuning(ctypes.Structure):
+    _fields_ = [
+        ("lds_cache", ctypes.c_uint32),
+        ("hub_threshold", ctypes.c_uint32),
+        ("hub_chunk", ctypes.c_uint32),
+        ("unit_edges", ctypes.c_uint32),
+        ("zero_rows_per_unit", ctypes.c_uint32),
+        ("blocks_per_cu", ctypes.c_uint32),
+        ("chunk", ctypes.c_uint32),
+    ]
+
+
 _pr = None
 _gen = None
 
@@ -76,6 +88,10 @@ def _declare_pagerank(L):
     L.nwgr_pr_graph_device_bytes.argtypes = [vp]
     L.nwgr_pr_graph_setup_ms.restype = f64
     L.nwgr_pr_graph_setup_ms.argtypes = [vp]
+    L.nwgr_pr_graph_get_tuning.restype = i32
+    L.nwgr_pr_graph_get_tuning.argtypes = [vp, P(PrTuning)]
+    L.nwgr_pr_graph_set_tuning.restype = i32
+    L.nwgr_pr_graph_set_tuning.argtypes = [vp, P(PrTuning)]
     L.nwgr_pr_run_f32.restype = i32
     L.nwgr_pr_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
     L.nwgr_pr_run_device_f32.restype = i32
@@ -84,6 +100,10 @@ def _declare_pagerank(L):
     L.nwgr_pr_last_errors.argtypes = [vp, vp, u32]
     L.nwgr_pr_last_profile.restype = i32
     L.nwgr_pr_last_profile.argtypes = [vp, P(f64), P(u32), P(u32)]
+    L.nwgr_wpr_forward_device_f32.restype = i32
+    L.nwgr_wpr_forward_device_f32.argtypes = [vp, P(PrParams), vp, vp, P(PrResult), vp]
+    L.nwgr_wpr_adjoint_device_f32.restype = i32
+    L.nwgr_wpr_adjoint_device_f32.argtypes = [vp, P(PrParams), vp, vp, vp, vp, vp, P(PrResult), vp]
     L.nwgr_last_error.restype = ctypes.c_char_p
     L.nwgr_last_error.argtypes = []
     L.nwgr_device_count.restype = i32

@@ -101,88 +101,232 @@ uint32_t bits_for(uint32_t n) {
   return b;
 }
 
-// Segment plan from the non-increasing in-degree sequence `deg` (new order).
-void plan_segments(Graph& g, const std::vector<uint32_t>& deg, const std::vector<uint32_t>& offs_host,
-                   std::vector<HubUnit>& hubs) {
+}  // namespace
+
+void build_plan(Graph& g, Plan& plan, const std::vector<uint32_t>& deg, const std::vector<uint32_t>& offs_h,
+                const std::vector<uint32_t>& row_of, hipStream_t stream) {
   const Tuning& t = g.tune;
   const uint32_t n = g.n;
+  auto row_at = [&](uint32_t pos) { return row_of.empty() ? pos : row_of[pos]; };
   auto first_below_or_eq = [&](uint32_t x) -> uint32_t {
-    // first row with deg <= x (deg is non-increasing)
+    // first position with deg <= x (deg is non-increasing)
     return static_cast<uint32_t>(
         std::partition_point(deg.begin(), deg.end(), [&](uint32_t d) { return d > x; }) - deg.begin());
   };
+  std::vector<uint64_t> pre(static_cast<size_t>(n) + 1, 0);  // edges before a position
+  for (uint32_t i = 0; i < n; ++i) pre[i + 1] = pre[i] + deg[i];
   SegTable st{};
   uint32_t unit = 0;
   // hub rows: whole workgroups over chunks of hub_chunk edges
   const uint32_t h_end = first_below_or_eq(t.hub_threshold);
-  hubs.clear();
-  for (uint32_t r = 0; r < h_end; ++r) {
-    const uint32_t d = deg[r];
+  std::vector<HubUnit> hubs;
+  for (uint32_t pos = 0; pos < h_end; ++pos) {
+    const uint32_t row = row_at(pos);
+    const uint32_t d = deg[pos];
     const uint32_t nch = (d + t.hub_chunk - 1) / t.hub_chunk;
     const uint32_t first = static_cast<uint32_t>(hubs.size());
     for (uint32_t c = 0; c < nch; ++c) {
       HubUnit u;
-      u.row = r;
-      u.e_lo = offs_host[r] + c * t.hub_chunk;
-      u.e_hi = std::min(offs_host[r] + (c + 1) * t.hub_chunk, offs_host[r + 1]);
+      u.row = row;
+      u.e_lo = offs_h[row] + c * t.hub_chunk;
+      u.e_hi = std::min(offs_h[row] + (c + 1) * t.hub_chunk, offs_h[row + 1]);
       u.first = first;
       u.nchunks = nch;
-      u.hub = r;
+      u.hub = pos;
       hubs.push_back(u);
     }
   }
   if (!hubs.empty()) {
-    Seg& s = st.s[st.nseg++];
-    s.row_lo = 0;
-    s.row_hi = h_end;
-    s.G = 0;
-    s.rows_per_unit = 1;
-    s.unit_lo = unit;
+    Seg& sg = st.s[st.nseg++];
+    sg.row_lo = 0;
+    sg.row_hi = h_end;
+    sg.G = 0;
+    sg.rows_per_unit = 1;
+    sg.unit_lo = unit;
     unit += static_cast<uint32_t>(hubs.size());
-    s.unit_hi = unit;
+    sg.unit_hi = unit;
   }
-  // lane-group segments: G lanes per row for degrees in (G, 2G]; G = 64
-  // takes everything in (64, hub_threshold]; G = 1 takes degrees 1 and 2.
-  uint32_t row = h_end;
-  struct Band { uint32_t G, dlo; };  // rows with deg > dlo (and <= previous band's dlo)
-  const Band bands[] = {{64, 64}, {32, 32}, {16, 16}, {8, 8}, {4, 4}, {2, 2}, {1, 0}};
-  for (const Band& b : bands) {
-    const uint32_t end = first_below_or_eq(b.dlo);
-    if (end > row) {
-      Seg& s = st.s[st.nseg++];
-      s.row_lo = row;
-      s.row_hi = end;
-      s.G = b.G;
-      const uint64_t edges = static_cast<uint64_t>(offs_host[end]) - offs_host[row];
-      const double avg = static_cast<double>(edges) / (end - row);
-      const uint32_t per_pass = kBlock / b.G;
-      double passes = t.unit_edges / std::max(1.0, per_pass * avg);
-      uint32_t R = static_cast<uint32_t>(std::clamp(passes, 1.0, 256.0));
-      s.rows_per_unit = per_pass * R;
-      s.unit_lo = unit;
-      unit += (end - row + s.rows_per_unit - 1) / s.rows_per_unit;
-      s.unit_hi = unit;
-      row = end;
+  // Lane-group segments.  A band takes the rows with dlo < degree <= the
+  // previous band's dlo; its rows get G lanes each, every lane loads U
+  // entries per batch (one batch covers the band's longest row except in the
+  // first band, which loops), and every lane group works on P rows at once so
+  // that P independent load chains are in flight.
+  uint32_t pos = h_end;
+  struct Band { uint32_t G, dlo, U, P; };
+  const Band bands[] = {{64, 128, 8, 1}, {64, 64, 2, 2}, {32, 32, 2, 2}, {16, 16, 2, 4}, {8, 8, 2, 4},
+                        {4, 4, 2, 4},    {2, 2, 2, 4},   {1, 0, 2, 4}};
+  for (const Band& bd : bands) {
+    const uint32_t end = first_below_or_eq(bd.dlo);
+    if (end > pos) {
+      Seg& sg = st.s[st.nseg++];
+      sg.row_lo = pos;
+      sg.row_hi = end;
+      sg.G = bd.G;
+      sg.U = bd.U;
+      sg.P = bd.P;
+      const double avg = static_cast<double>(pre[end] - pre[pos]) / (end - pos);
+      const uint32_t per_pass = kWave / bd.G * bd.P;
+      const double passes = t.unit_edges / std::max(1.0, per_pass * avg);
+      const uint32_t R = static_cast<uint32_t>(std::clamp(passes, 1.0, 256.0));
+      sg.rows_per_unit = per_pass * R;
+      sg.unit_lo = unit;
+      unit += (end - pos + sg.rows_per_unit - 1) / sg.rows_per_unit;
+      sg.unit_hi = unit;
+      pos = end;
     }
   }
-  // rows without in-edges: teleport/dangling terms only
-  if (n > row) {
-    Seg& s = st.s[st.nseg++];
-    s.row_lo = row;
-    s.row_hi = n;
-    s.G = 1;
-    s.rows_per_unit = std::max<uint32_t>(kBlock, t.zero_rows_per_unit / kBlock * kBlock);
-    s.unit_lo = unit;
-    unit += (n - row + s.rows_per_unit - 1) / s.rows_per_unit;
-    s.unit_hi = unit;
+  // rows without entries: teleport/dangling terms only
+  if (n > pos) {
+    Seg& sg = st.s[st.nseg++];
+    sg.row_lo = pos;
+    sg.row_hi = n;
+    sg.G = 1;
+    sg.U = 0;
+    sg.P = 4;
+    sg.rows_per_unit = std::max<uint32_t>(kWave * 4, t.zero_rows_per_unit / (kWave * 4) * (kWave * 4));
+    sg.unit_lo = unit;
+    unit += (n - pos + sg.rows_per_unit - 1) / sg.rows_per_unit;
+    sg.unit_hi = unit;
   }
   st.total_units = unit;
-  g.segs = st;
-  g.n_hub_units = static_cast<uint32_t>(hubs.size());
-  g.n_hubs = h_end;
+  plan.segs = st;
+  plan.n_hub_units = static_cast<uint32_t>(hubs.size());
+  plan.n_hubs = h_end;
+  plan.hub_units.alloc(sizeof(HubUnit) * std::max<size_t>(hubs.size(), 1));
+  if (!hubs.empty())
+    NWGR_HIP(hipMemcpyAsync(plan.hub_units.p, hubs.data(), sizeof(HubUnit) * hubs.size(), hipMemcpyHostToDevice,
+                            stream));
+  plan.hub_part.alloc(sizeof(uint32_t) * std::max<size_t>(hubs.size(), 1));
+  plan.hub_ticket.alloc(sizeof(uint32_t) * std::max<uint32_t>(h_end, 1));
+  NWGR_HIP(hipMemsetAsync(plan.hub_ticket.p, 0, plan.hub_ticket.bytes, stream));
+  if (!row_of.empty()) {
+    plan.rows.alloc(sizeof(uint32_t) * n);
+    NWGR_HIP(hipMemcpyAsync(plan.rows.p, row_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+  } else {
+    plan.rows.reset();
+  }
+  // the host vectors must outlive the copies
+  NWGR_HIP(hipStreamSynchronize(stream));
+}
+
+namespace {
+
+// Out-CSR keys of the new in-CSR: edge k (row v, source u) -> (u << 32 | v), value k.
+__global__ void k_transpose_keys(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ idx, uint32_t n,
+                                 uint64_t m, uint64_t* keys, uint32_t* vals, uint32_t* outdeg) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < m; k += stride) {
+    uint32_t lo = 0, hi = n;  // offs[lo] <= k < offs[hi]
+    while (hi - lo > 1) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (offs[mid] <= k) lo = mid; else hi = mid;
+    }
+    const uint32_t u = idx[k];
+    keys[k] = (static_cast<uint64_t>(u) << 32) | lo;
+    vals[k] = static_cast<uint32_t>(k);
+    atomicAdd(&outdeg[u], 1u);
+  }
+}
+
+__global__ void k_iota(uint32_t n, uint32_t* v) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    v[i] = static_cast<uint32_t>(i);
 }
 
 }  // namespace
+
+void ensure_transpose(Graph& g, hipStream_t stream) {
+  if (g.has_t) return;
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  const size_t nb = sizeof(uint32_t) * n, mb = sizeof(uint32_t) * m;
+  DevBuf outdeg, ids, outdeg_sorted, row_list;
+  outdeg.alloc(nb);
+  NWGR_HIP(hipMemsetAsync(outdeg.p, 0, nb, stream));
+  g.t_idx.alloc(mb + sizeof(uint32_t) * kEdgePad);
+  NWGR_HIP(hipMemsetAsync(g.t_idx.p, 0, g.t_idx.bytes, stream));
+  g.t_pos.alloc(mb);
+  if (m) {
+    DevBuf k_in, k_out, v_in;
+    k_in.alloc(sizeof(uint64_t) * m);
+    k_out.alloc(sizeof(uint64_t) * m);
+    v_in.alloc(mb);
+    k_transpose_keys<<<grid_for(m), kT, 0, stream>>>(g.offs.as<uint32_t>(), g.idx.as<uint32_t>(), n, m,
+                                                     k_in.as<uint64_t>(), v_in.as<uint32_t>(), outdeg.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    const int end_bit = 32 + static_cast<int>(bits_for(n));
+    size_t tmp = 0;
+    NWGR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k_in.as<uint64_t>(), k_out.as<uint64_t>(),
+                                                v_in.as<uint32_t>(), g.t_pos.as<uint32_t>(), static_cast<int>(m), 0,
+                                                end_bit, stream));
+    DevBuf t;
+    t.alloc(tmp);
+    NWGR_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k_in.as<uint64_t>(), k_out.as<uint64_t>(),
+                                                v_in.as<uint32_t>(), g.t_pos.as<uint32_t>(), static_cast<int>(m), 0,
+                                                end_bit, stream));
+    k_split_keys<<<grid_for(m), kT, 0, stream>>>(k_out.as<uint64_t>(), m, g.t_idx.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  g.t_offs.alloc(nb + sizeof(uint32_t));
+  {
+    size_t tmp = 0;
+    NWGR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, outdeg.as<uint32_t>(), g.t_offs.as<uint32_t>(),
+                                              static_cast<int>(n), stream));
+    DevBuf t;
+    t.alloc(tmp);
+    NWGR_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, outdeg.as<uint32_t>(), g.t_offs.as<uint32_t>(),
+                                              static_cast<int>(n), stream));
+    k_set_last<<<1, 1, 0, stream>>>(g.t_offs.as<uint32_t>(), n, static_cast<uint32_t>(m));
+    NWGR_HIP(hipGetLastError());
+  }
+  // rows of the out-CSR in non-increasing out-degree order (stable)
+  ids.alloc(nb);
+  outdeg_sorted.alloc(nb);
+  row_list.alloc(nb);
+  k_iota<<<grid_for(n), kT, 0, stream>>>(n, ids.as<uint32_t>());
+  {
+    size_t tmp = 0;
+    NWGR_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, outdeg.as<uint32_t>(),
+                                                         outdeg_sorted.as<uint32_t>(), ids.as<uint32_t>(),
+                                                         row_list.as<uint32_t>(), static_cast<int>(n), 0, 32, stream));
+    DevBuf t;
+    t.alloc(tmp);
+    NWGR_HIP(hipcub::DeviceRadixSort::SortPairsDescending(t.p, tmp, outdeg.as<uint32_t>(),
+                                                         outdeg_sorted.as<uint32_t>(), ids.as<uint32_t>(),
+                                                         row_list.as<uint32_t>(), static_cast<int>(n), 0, 32, stream));
+  }
+  std::vector<uint32_t> deg_h(n), rows_h(n), toffs_h(static_cast<size_t>(n) + 1);
+  NWGR_HIP(hipMemcpyAsync(deg_h.data(), outdeg_sorted.p, nb, hipMemcpyDeviceToHost, stream));
+  NWGR_HIP(hipMemcpyAsync(rows_h.data(), row_list.p, nb, hipMemcpyDeviceToHost, stream));
+  NWGR_HIP(hipMemcpyAsync(toffs_h.data(), g.t_offs.p, nb + sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  NWGR_HIP(hipStreamSynchronize(stream));
+  build_plan(g, g.adj, deg_h, toffs_h, rows_h, stream);
+  g.t_deg_h.swap(deg_h);
+  g.t_rows_h.swap(rows_h);
+  g.t_offs_h.swap(toffs_h);
+  g.has_t = true;
+}
+
+void replan(Graph& g, hipStream_t stream) {
+  Tuning& t = g.tune;
+  NWGR_REQUIRE(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2, "blocks_per_cu must be 1 or 2");
+  NWGR_REQUIRE(t.lds_cache <= kLdsCache / t.blocks_per_cu,
+               "lds_cache exceeds the LDS share of a sweep workgroup (36864 / blocks_per_cu)");
+  NWGR_REQUIRE(t.hub_threshold >= 128, "hub_threshold must be >= 128");
+  NWGR_REQUIRE(t.hub_chunk >= kWave, "hub_chunk must be >= 64");
+  NWGR_REQUIRE(t.unit_edges >= 1 && t.zero_rows_per_unit >= 1, "unit sizes must be positive");
+  NWGR_REQUIRE(t.chunk >= 1 && t.chunk <= Graph::kMaxChunk, "chunk must be in [1, 32]");
+  build_plan(g, g.fwd, g.in_deg_h, g.offs_h, {}, stream);
+  if (g.has_t) build_plan(g, g.adj, g.t_deg_h, g.t_offs_h, g.t_rows_h, stream);
+  const uint32_t grid = std::max<uint32_t>(
+      1, std::min<uint32_t>(g.num_cu * t.blocks_per_cu, std::max<uint32_t>(g.fwd.segs.total_units, 64)));
+  if (grid != g.grid || !g.partials.p) {
+    g.grid = grid;
+    g.partials.alloc(sizeof(double) * 2 * 2 * g.grid);
+  }
+  g.drop_graphs();
+}
 
 Graph::~Graph() {
   drop_graphs();
@@ -302,7 +446,8 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
     NWGR_HIP(hipGetLastError());
   }
   // renamed, row-sorted edges
-  g->idx.alloc(mb);
+  g->idx.alloc(mb + sizeof(uint32_t) * kEdgePad);
+  NWGR_HIP(hipMemsetAsync(g->idx.p, 0, g->idx.bytes, stream));
   g->epos.alloc(mb);
   if (m) {
     DevBuf ek_in, ek_out, ev_in;
@@ -334,31 +479,24 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   NWGR_HIP(hipMemcpyAsync(deg_h.data(), key_out.p, nb, hipMemcpyDeviceToHost, stream));
   NWGR_HIP(hipMemcpyAsync(offs_h.data(), g->offs.p, nb + sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   NWGR_HIP(hipStreamSynchronize(stream));
-  std::vector<HubUnit> hubs;
-  plan_segments(*g, deg_h, offs_h, hubs);
-  g->hub_units.alloc(sizeof(HubUnit) * std::max<size_t>(hubs.size(), 1));
-  if (!hubs.empty())
-    NWGR_HIP(hipMemcpyAsync(g->hub_units.p, hubs.data(), sizeof(HubUnit) * hubs.size(),
-                            hipMemcpyHostToDevice, stream));
-  g->hub_part.alloc(sizeof(uint32_t) * std::max<size_t>(hubs.size(), 1));
-  g->hub_ticket.alloc(sizeof(uint32_t) * std::max<uint32_t>(g->n_hubs, 1));
-  NWGR_HIP(hipMemsetAsync(g->hub_ticket.p, 0, g->hub_ticket.bytes, stream));
+  g->in_deg_h.swap(deg_h);
+  g->offs_h.swap(offs_h);
+  replan(*g, stream);  // plans and sweep grid for the default tuning
 
   // --- iteration buffers
-  g->grid = std::max<uint32_t>(1, std::min<uint32_t>(g->num_cu * g->tune.blocks_per_cu,
-                                                      std::max<uint32_t>(g->segs.total_units, 1)));
+  // one extra entry per vector: index n is the zero sentinel of the gathers
   for (int b = 0; b < 2; ++b) {
-    g->r[b].alloc(sizeof(float) * n);
-    g->c[b].alloc(sizeof(float) * n);
+    g->r[b].alloc(sizeof(float) * (static_cast<size_t>(n) + 1));
+    g->c[b].alloc(sizeof(float) * (static_cast<size_t>(n) + 1));
+    NWGR_HIP(hipMemsetAsync(g->r[b].p, 0, g->r[b].bytes, stream));
+    NWGR_HIP(hipMemsetAsync(g->c[b].p, 0, g->c[b].bytes, stream));
   }
-  g->partials.alloc(sizeof(double) * 2 * 2 * g->grid);
   g->state.alloc(sizeof(State));
   NWGR_HIP(hipMemsetAsync(g->state.p, 0, sizeof(State), stream));
   NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->hflag), sizeof(HostFlag), hipHostMallocMapped));
   NWGR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g->hflag_dev), g->hflag, 0));
   std::memset(const_cast<HostFlag*>(g->hflag), 0, sizeof(HostFlag));
   NWGR_HIP(hipStreamSynchronize(stream));
-  g->seg_rows.swap(deg_h);
   g->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return g.release();
 }

@@ -92,14 +92,27 @@ struct DevBuf {
 // segments; each segment is processed by groups of G lanes per row
 // (G = 1 .. 64), or, for hub rows, by whole workgroups over fixed-size edge
 // chunks.  A "unit" is the work one workgroup takes from the queue at a time.
-constexpr int kBlock = 256;      // threads per workgroup of the sweep kernels
+constexpr int kBlock = 256;      // threads per workgroup of the auxiliary kernels
+constexpr int kWave = 64;        // one work unit is processed by one wavefront
+constexpr int kSweepThreads = 1024;  // sweep workgroup: one per CU, 16 wavefronts
+// Contributions of vertices [0, kLdsCache) -- the hottest ones after the
+// degree relabelling -- are staged in LDS by every sweep workgroup (144 KiB
+// of the 160 KiB per CU).
+constexpr uint32_t kLdsCache = 36864;
 constexpr int kMaxSegs = 16;
+// Edge arrays (indices, weights) carry this many extra entries so that the
+// batched gather loops may load past the last edge of a row without a bounds
+// branch; vertex arrays gathered from carry one extra entry (index n) that
+// stays 0, the sentinel for loads past a row's end.
+constexpr uint32_t kEdgePad = 4096;
 
 struct Seg {
   uint32_t row_lo, row_hi;    // rows [row_lo, row_hi)
   uint32_t G;                 // lanes per row (power of two); 0 = hub chunks
-  uint32_t rows_per_unit;     // rows a unit covers (multiple of kBlock / G)
+  uint32_t rows_per_unit;     // rows a unit covers (multiple of P * kBlock / G)
   uint32_t unit_lo, unit_hi;  // global unit ids [unit_lo, unit_hi)
+  uint32_t U;                 // edges per lane loaded in one batch (0: rows without entries)
+  uint32_t P;                 // rows per lane group processed together
 };
 
 struct SegTable {
@@ -126,7 +139,7 @@ struct State {
   uint32_t done;      // 1 once converged or capped
   uint32_t iters;     // sweeps performed (valid once done)
   double err;         // error of the last sweep (valid once done)
-  double pad;
+  double dang;        // dangling mass (or adjoint Q) of the final iterate (valid once done)
 };
 
 // Host-mapped mirror the solver publishes into: progress every sweep, the
@@ -138,6 +151,7 @@ struct HostFlag {
   volatile uint32_t progress;   // sweeps completed as of the latest launch
   volatile uint32_t pad;
   volatile double last_err;     // error of sweep `progress`
+  volatile double prev_err;     // error of sweep `progress - 1` (0 if none)
 };
 
 // Timing of the last solve, from events around each captured chunk.
@@ -148,12 +162,47 @@ struct SolveProfile {
 };
 
 struct Tuning {
+  uint32_t lds_cache = kLdsCache;  // hot vertices staged in LDS per sweep workgroup
   uint32_t hub_threshold = 4096;   // rows with in-degree > this are hubs
-  uint32_t hub_chunk = 4096;       // edges per hub unit
-  uint32_t unit_edges = 8192;      // target edges per non-hub unit
-  uint32_t zero_rows_per_unit = 8192;
-  uint32_t blocks_per_cu = 8;
+  uint32_t hub_chunk = 2048;       // edges per hub unit (one wavefront)
+  uint32_t unit_edges = 2048;      // target edges per non-hub unit (one wavefront)
+  uint32_t zero_rows_per_unit = 2048;
+  uint32_t blocks_per_cu = 1;
   uint32_t chunk = 16;             // sweeps per captured graph
+};
+
+// Work decomposition of one CSR orientation: segments over row positions and
+// the hub chunk list.  `rows` maps a position to a row when the rows are not
+// already stored in non-increasing degree order (the transposed CSR).
+struct Plan {
+  SegTable segs{};
+  DevBuf hub_units;  // HubUnit[]
+  DevBuf hub_part;   // u32[] (float bits) hub chunk partial sums
+  DevBuf hub_ticket; // u32[] per hub row
+  DevBuf rows;       // u32[n] position -> row, or empty (identity)
+  uint32_t n_hub_units = 0, n_hubs = 0;
+};
+
+// Kinds of fixed-point solve sharing the sweep machinery.
+enum SolveKind : uint32_t {
+  kPlain = 0,     // r' = (1-d) t + d (sum c_j + D u),  c = r / deg
+  kWeighted = 1,  // same with per-edge weights, c = r / s (s = out-weight sums)
+  kAdjoint = 2,   // lambda' = d (s > 0 ? (sum w lambda)/s : Q) + g   on the transpose
+  kNumKinds = 3,
+  kCoef = 3,      // one pass on the transpose: gradient coefficients from the final lambda
+};
+
+// What one fixed-point solve iterates over.
+struct SolveSpec {
+  uint32_t kind = kPlain;
+  const Plan* plan = nullptr;
+  const uint32_t* offs = nullptr;  // CSR the sweep gathers over
+  const uint32_t* idx = nullptr;
+  const float* w = nullptr;        // per-edge weights in that CSR's order (weighted kinds)
+  const float* div = nullptr;      // per-row divisor: degree (plain) or out-weight sum
+  const float* gsrc = nullptr;     // adjoint source term, new labels
+  const float* init = nullptr;     // forward initial ranks in caller order, or null
+  const float* tel_src = nullptr;  // teleport in caller order (permuted at init), or null
 };
 
 struct Graph {
@@ -172,12 +221,24 @@ struct Graph {
   DevBuf idx;       // u32[m]  new source ids
   DevBuf epos;      // u32[m]  new edge position -> original edge position
   DevBuf degf;      // f32[n]  divisor of the contribution (caller degrees), new order
-  DevBuf hub_units; // HubUnit[]
-  DevBuf hub_part;  // u32[] (float bits) hub chunk partial sums
-  DevBuf hub_ticket;// u32[] per hub row
-  std::vector<uint32_t> seg_rows;  // host copy of in-degrees (new order), for planning
-  SegTable segs{};
-  uint32_t n_hub_units = 0, n_hubs = 0;
+  Plan fwd;         // plan of the in-CSR (rows already in degree order)
+  std::vector<uint32_t> in_deg_h;  // host copy of in-degrees (new order)
+  std::vector<uint32_t> offs_h;    // host copy of offs
+
+  // transposed (out-)CSR in the new labels, built on first use
+  bool has_t = false;
+  DevBuf t_offs;    // u32[n+1]
+  DevBuf t_idx;     // u32[m]  destinations
+  DevBuf t_pos;     // u32[m]  out-CSR entry -> in-CSR (new order) position
+  Plan adj;         // plan of the out-CSR (row list by out-degree)
+  std::vector<uint32_t> t_deg_h, t_rows_h, t_offs_h;  // host copies for re-planning
+
+  // weighted / adjoint work arrays
+  DevBuf w_in, w_out;  // f32[m] weights in in-CSR (new) and out-CSR order
+  DevBuf s;            // f32[n] out-weight sums (new labels)
+  DevBuf gsrc;         // f32[n] adjoint source term (new labels)
+  DevBuf rsave;        // f32[n] forward ranks (new labels) for the gradient
+  DevBuf coef_a, coef_b; // f32[n] per-source gradient coefficients
 
   // iteration buffers
   DevBuf r[2], c[2];
@@ -189,22 +250,26 @@ struct Graph {
   HostFlag* hflag = nullptr;   // pinned, mapped
   HostFlag* hflag_dev = nullptr;
 
-  // captured chunk graphs of the plain solve: [parity of the first sweep][length]
+  // captured chunk graphs: [kind][parity of the first sweep][length]
   static constexpr uint32_t kMaxChunk = 32;
-  hipGraphExec_t exec[2][kMaxChunk + 1] = {};
-  uint64_t exec_key = 0;
+  hipGraphExec_t exec[kNumKinds][2][kMaxChunk + 1] = {};
+  uint64_t exec_key[kNumKinds] = {};
   hipStream_t own_stream = nullptr;
 
   std::vector<double> last_errs;
   SolveProfile last_profile;
 
-  void drop_graphs() {
-    for (auto& row : exec)
-      for (auto& e : row)
-        if (e) {
-          (void)hipGraphExecDestroy(e);
-          e = nullptr;
-        }
+  void drop_graphs(int kind = -1) {
+    for (int k = 0; k < kNumKinds; ++k) {
+      if (kind >= 0 && k != kind) continue;
+      for (auto& row : exec[k])
+        for (auto& e : row)
+          if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+          }
+      exec_key[k] = 0;
+    }
   }
 
   ~Graph();
@@ -212,9 +277,11 @@ struct Graph {
 
 inline uint64_t bytes_of(const Graph& g) {
   uint64_t b = 0;
-  for (const DevBuf* d : {&g.perm, &g.iperm, &g.offs, &g.idx, &g.epos, &g.degf, &g.hub_units,
-                          &g.hub_part, &g.hub_ticket, &g.r[0], &g.r[1], &g.c[0], &g.c[1], &g.tel,
-                          &g.partials, &g.state, &g.hist})
+  for (const DevBuf* d : {&g.perm, &g.iperm, &g.offs, &g.idx, &g.epos, &g.degf, &g.fwd.hub_units,
+                          &g.fwd.hub_part, &g.fwd.hub_ticket, &g.fwd.rows, &g.t_offs, &g.t_idx, &g.t_pos,
+                          &g.adj.hub_units, &g.adj.hub_part, &g.adj.hub_ticket, &g.adj.rows, &g.w_in,
+                          &g.w_out, &g.s, &g.gsrc, &g.rsave, &g.coef_a, &g.coef_b, &g.r[0], &g.r[1], &g.c[0],
+                          &g.c[1], &g.tel, &g.partials, &g.state, &g.hist})
     b += d->bytes;
   return b;
 }
@@ -222,9 +289,25 @@ inline uint64_t bytes_of(const Graph& g) {
 // graph construction (graph.hip)
 Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
                     const uint32_t* degrees, int device, hipStream_t stream, bool inputs_on_device);
+// segments and hub chunks for rows given in non-increasing degree order;
+// `row_of` maps a position to a row (identity when empty)
+void build_plan(Graph& g, Plan& plan, const std::vector<uint32_t>& deg_by_pos,
+                const std::vector<uint32_t>& offs_h, const std::vector<uint32_t>& row_of, hipStream_t s);
+// transposed CSR + its plan (graph.hip); no-op once built
+void ensure_transpose(Graph& g, hipStream_t s);
+// validates g.tune, rebuilds the plans and the sweep grid (graph.hip)
+void replan(Graph& g, hipStream_t s);
 
-// plain solve (pagerank.hip)
+// solves (pagerank.hip, weighted.hip)
+// Runs the chunked fixed-point iteration; the final iterate is left in
+// g.r[res.iters & 1] (new labels).
+void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
+// one gradient-coefficient pass over the transpose (kind kCoef)
+void run_coef_pass(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, const float* lam, hipStream_t s);
+// out[v] = src[iperm[v]] * scale (new labels -> caller order)
+void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s);
+void validate_params(const nwgr_pr_params& p);
 
 struct DeviceGuard {
   int prev = 0;

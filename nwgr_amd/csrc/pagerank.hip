@@ -1,29 +1,31 @@
-// PageRank sweeps on gfx950 and the C ABI of libnwgr_pagerank.
+// PageRank sweeps on gfx950 and the C ABI of the plain solve.
 //
 // One sweep = one launch of k_sweep: a pull gather-SpMV over the in-CSR
 // (page_rank.hpp:127-136, per-edge gather :129-131) with the teleport,
 // dangling-mass and L1-residual terms fused into the row epilogue.  Jacobi
 // semantics as page_rank_v10 (experimental/algorithms/page_rank.hpp:450-505).
+// The same kernel, instantiated per solve kind, runs the weighted forward and
+// the adjoint (on the transposed CSR) of the differentiable form.
 //
 // Work distribution: workgroups take units from a queue (one RMW atomic per
-// unit); rows are pre-sorted by in-degree so each unit covers rows of one
-// degree band, processed by G lanes per row (G = 1..64), or a chunk of one hub
-// row processed by the whole workgroup.  Hub chunks are combined in chunk
-// order by the last chunk to finish (RMW atomics only, no cache flushes).
+// unit); rows are visited in non-increasing degree order so each unit covers
+// rows of one degree band, processed by G lanes per row (G = 1..64), or a
+// chunk of one hub row processed by the whole workgroup.  Hub chunks are
+// combined in chunk order by the last chunk to finish (RMW atomics only, no
+// cache flushes).
 //
 // Cross-workgroup totals (L1 error, dangling mass) never meet inside a kernel:
 // every workgroup writes its partial sums, and every workgroup of the NEXT
 // sweep re-reduces them in the same fixed order, so all workgroups take the
-// same convergence decision without fences.  The sweep that finds the
-// previous sweep converged (or the cap reached) records the result and exits;
-// later launches see `done` and exit at once.  Sweeps are captured in chunks
-// of K launches in a hipGraph; the host reads a mapped flag after each chunk.
+// same convergence decision without fences.  The launch that finds the
+// previous sweep converged (or the cap reached) records the result and
+// exits; launches after it return at once.  Sweeps are captured in hipGraph
+// chunks; the host sizes the chunks from the observed error decay.
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <string>
 
 #include "nwgr_internal.hpp"
@@ -43,8 +45,11 @@ constexpr uint32_t kDanglingLeak = 0, kDanglingUniform = 1, kDanglingTeleport = 
 struct SweepArgs {
   const uint32_t* __restrict__ offs;
   const uint32_t* __restrict__ idx;
-  const float* __restrict__ degf;
+  const float* __restrict__ w;       // per-edge weights (weighted kinds)
+  const uint32_t* __restrict__ rows; // position -> row, or null
+  const float* __restrict__ div;     // degree or out-weight sum per row
   const float* __restrict__ tel;     // permuted teleport or null (uniform)
+  const float* __restrict__ gsrc;    // adjoint source term
   const float* __restrict__ r_in;
   const float* __restrict__ c_in;
   float* __restrict__ r_out;
@@ -60,6 +65,8 @@ struct SweepArgs {
   uint32_t* hub_ticket;
   SegTable segs;
   uint32_t p;          // parity of this sweep
+  uint32_t n;          // vertices; also the zero sentinel index of the gathers
+  uint32_t lds_k;      // entries of the gathered vector staged in LDS (<= n + 1)
   uint32_t max_iters;
   uint32_t dangling;
   float d_hi, d_lo;    // damping as an unevaluated float sum d_hi + d_lo
@@ -67,19 +74,26 @@ struct SweepArgs {
   float one_minus_d;   // 1-d as float (explicit teleport)
   float un;            // 1/n as float
   double tol;
+  // gradient coefficients (kCoef): a = d r / s, b = a * (P lambda)
+  const float* __restrict__ rsave;
+  float* __restrict__ coef_a;
+  float* __restrict__ coef_b;
 };
 
 struct InitArgs {
   const uint32_t* __restrict__ perm;
-  const float* __restrict__ degf;
+  const float* __restrict__ div;
   const float* __restrict__ init;      // caller order or null
   const float* __restrict__ tel_src;   // caller order or null
+  const float* __restrict__ gsrc;      // adjoint: lambda_0 (new labels)
   float* __restrict__ tel;             // permuted copy (when tel_src)
   float* __restrict__ r0;
   float* __restrict__ c0;
   double* __restrict__ part0;
   State* st;
   uint32_t n;
+  uint32_t adjoint;
+  uint32_t dangling;
   float un;
 };
 
@@ -118,27 +132,174 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double (*s)[kBl
 }
 
 __device__ __forceinline__ uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
 
-// Row epilogue: teleport + dangling + damping, residual, next contribution.
-__device__ __forceinline__ void finish_row(const SweepArgs& a, uint32_t row, float z, float Df, double& err,
-                                           double& dang) {
-  const float rold = a.r_in[row];
-  float base, t;
-  if (a.tel) {
-    t = a.tel[row];
-    base = a.one_minus_d * t;
-  } else {
-    t = a.un;
-    base = a.base_u;
+// Contribution of vertex j: from the LDS copy of the hot prefix when j < K,
+// else from global memory.
+__device__ __forceinline__ float contrib(const float* __restrict__ lds, uint32_t K, const float* __restrict__ c,
+                                         uint32_t j) {
+  return j < K ? lds[j] : c[j];
+}
+
+// Sum over edges k = k0, k0 + step, ... < e of c[idx[k]] (times w[k]).
+// Edges are taken U at a time with no bounds branch inside a batch: the U
+// index loads are issued together, then the U gathers, so each lane keeps U
+// loads in flight.  Loads past `e` read padding (edge arrays carry kEdgePad
+// extra entries) and are redirected to the sentinel vertex n, whose entry
+// is 0.  The summation order is fixed: batch by batch, in k order.
+template <int U, bool W>
+__device__ __forceinline__ float gather_sum(const uint32_t* __restrict__ idx, const float* __restrict__ w,
+                                            const float* __restrict__ lds, uint32_t K, const float* __restrict__ c,
+                                            uint32_t k0, uint32_t e, uint32_t step, uint32_t sentinel) {
+  float z = 0.f;
+  for (uint32_t k = k0; k < e; k += step * U) {
+    uint32_t j[U];
+    float wv[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t kk = k + q * step;
+      const uint32_t jj = ld_stream(idx + kk);
+      j[q] = kk < e ? jj : sentinel;
+      if constexpr (W) wv[q] = ld_stream(w + kk);
+    }
+    float v[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) v[q] = contrib(lds, K, c, j[q]);
+#pragma unroll
+    for (int q = 0; q < U; ++q) z += W ? wv[q] * v[q] : v[q];
   }
-  const float u = a.dangling == kDanglingUniform ? a.un : (a.dangling == kDanglingTeleport ? t : 0.f);
-  const float s = fmaf(Df, u, z);
-  const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
-  a.r_out[row] = rn;
-  const float dg = a.degf[row];
-  a.c_out[row] = rn / dg;  // IEEE division; +inf/NaN for dg == 0 is never gathered
-  err += fabs(static_cast<double>(rn) - static_cast<double>(rold));
-  if (dg == 0.f) dang += static_cast<double>(rn);
+  return z;
+}
+
+// Per-row operands of the epilogue, loaded before the row's gathers so that
+// their latency overlaps the gathers'.
+struct RowIn {
+  float old;  // previous iterate (rank or lambda)
+  float div;  // degree / out-weight sum
+  float t;    // teleport entry (forward, adjoint with TELEPORT)
+  float g;    // adjoint source term, or the forward rank (coef)
+};
+
+template <uint32_t KIND>
+__device__ __forceinline__ RowIn load_row(const SweepArgs& a, uint32_t row) {
+  RowIn in;
+  in.div = a.div[row];
+  in.old = KIND == kCoef ? 0.f : a.r_in[row];
+  in.t = a.tel ? a.tel[row] : a.un;
+  if constexpr (KIND == kAdjoint) in.g = a.gsrc[row];
+  else if constexpr (KIND == kCoef) in.g = a.rsave[row];
+  else in.g = 0.f;
+  return in;
+}
+
+// Row epilogue.
+//  forward kinds: teleport + dangling + damping, residual, next contribution
+//  adjoint: lambda' = d (s > 0 ? z/s : Q) + g, residual, Q partial of lambda'
+//  coef: gradient coefficients from P lambda = z/s
+template <uint32_t KIND>
+__device__ __forceinline__ void finish_row(const SweepArgs& a, uint32_t row, float z, float Df, const RowIn& in,
+                                           double& err, double& dang) {
+  if constexpr (KIND == kPlain || KIND == kWeighted) {
+    const float base = a.tel ? a.one_minus_d * in.t : a.base_u;
+    const float u = a.dangling == kDanglingUniform ? a.un : (a.dangling == kDanglingTeleport ? in.t : 0.f);
+    const float s = fmaf(Df, u, z);
+    const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
+    a.r_out[row] = rn;
+    a.c_out[row] = rn / in.div;  // IEEE division; +inf/NaN for div == 0 is never gathered
+    err += fabs(static_cast<double>(rn) - static_cast<double>(in.old));
+    if (in.div == 0.f) dang += static_cast<double>(rn);
+  } else if constexpr (KIND == kAdjoint) {
+    const float x = in.div > 0.f ? z / in.div : Df;
+    const float ln = fmaf(a.d_hi, x, fmaf(a.d_lo, x, in.g));
+    a.r_out[row] = ln;
+    err += fabs(static_cast<double>(ln) - static_cast<double>(in.old));
+    if (a.dangling == kDanglingUniform)
+      dang += static_cast<double>(ln) * static_cast<double>(a.un);
+    else if (a.dangling == kDanglingTeleport)
+      dang += static_cast<double>(ln) * static_cast<double>(in.t);
+  } else {  // kCoef
+    float ca = 0.f, cb = 0.f;
+    if (in.div > 0.f) {
+      ca = (a.d_hi + a.d_lo) * in.g / in.div;
+      cb = ca * (z / in.div);
+    }
+    a.coef_a[row] = ca;
+    a.coef_b[row] = cb;
+  }
+}
+
+// Rows of one lane-group segment for one wavefront: G lanes per row, P rows
+// per lane group in flight together, U entries per lane per batch.  Row
+// bounds and epilogue operands of all P rows are loaded first, then the index
+// batches, then the gathers; rows longer than U*G entries continue in a loop.
+template <uint32_t KIND, int U, int P>
+__device__ __forceinline__ void process_rows(const SweepArgs& a, const float* __restrict__ lds, uint32_t K,
+                                             const Seg& sg, uint32_t ul, float Df, double& err, double& dang) {
+  constexpr bool kW = KIND != kPlain;
+  const uint32_t wl = threadIdx.x & (kWave - 1);
+  const uint32_t G = sg.G;
+  const uint32_t lg = __builtin_ctz(G);
+  const uint32_t lane = wl & (G - 1);
+  const uint32_t per_pass = kWave >> lg;
+  const uint32_t pos_end = min(sg.row_lo + (ul + 1) * sg.rows_per_unit, sg.row_hi);
+  for (uint32_t pos0 = sg.row_lo + ul * sg.rows_per_unit + (wl >> lg); pos0 < pos_end; pos0 += P * per_pass) {
+    uint32_t row[P], b[P], e[P];
+    bool valid[P];
+    RowIn in[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const uint32_t pos = pos0 + p * per_pass;
+      valid[p] = pos < pos_end;
+      const uint32_t pc = valid[p] ? pos : pos_end - 1;
+      row[p] = a.rows ? a.rows[pc] : pc;
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if constexpr (U > 0) {
+        b[p] = a.offs[row[p]];
+        e[p] = valid[p] ? a.offs[row[p] + 1] : b[p];
+      }
+      in[p] = load_row<KIND>(a, row[p]);
+    }
+    float z[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) z[p] = 0.f;
+    if constexpr (U > 0) {
+      uint32_t j[P][U];
+      float wv[P][U];
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          const uint32_t kk = b[p] + lane + q * G;
+          const uint32_t jj = ld_stream(a.idx + kk);
+          j[p][q] = kk < e[p] ? jj : a.n;
+          if constexpr (kW) wv[p][q] = ld_stream(a.w + kk);
+        }
+      float v[P][U];
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[p][q] = contrib(lds, K, a.c_in, j[p][q]);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int q = 0; q < U; ++q) z[p] += kW ? wv[p][q] * v[p][q] : v[p][q];
+      // the rest of rows longer than one batch
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        if (b[p] + U * G < e[p])
+          z[p] += gather_sum<U, kW>(a.idx, a.w, lds, K, a.c_in, b[p] + lane + U * G, e[p], G, a.n);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        for (uint32_t o = G >> 1; o > 0; o >>= 1) z[p] += __shfl_xor(z[p], o);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        if (valid[p]) finish_row<KIND>(a, row[p], z[p], Df, in[p], err, dang);
+    }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
@@ -147,12 +308,25 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
     const uint32_t src = a.perm[i];
-    const float r = a.init ? a.init[src] : a.un;
-    const float dg = a.degf[i];
-    a.r0[i] = r;
-    a.c0[i] = r / dg;
-    if (dg == 0.f) dang += static_cast<double>(r);
-    if (a.tel_src) a.tel[i] = a.tel_src[src];
+    float t = a.un;
+    if (a.tel_src) {
+      t = a.tel_src[src];
+      a.tel[i] = t;
+    }
+    if (a.adjoint) {
+      const float l = a.gsrc[i];
+      a.r0[i] = l;
+      if (a.dangling == kDanglingUniform)
+        dang += static_cast<double>(l) * static_cast<double>(a.un);
+      else if (a.dangling == kDanglingTeleport)
+        dang += static_cast<double>(l) * static_cast<double>(t);
+    } else {
+      const float r = a.init ? a.init[src] : a.un;
+      const float dg = a.div[i];
+      a.r0[i] = r;
+      a.c0[i] = r / dg;
+      if (dg == 0.f) dang += static_cast<double>(r);
+    }
   }
   block_sum2(dang, unused, s_red);
   if (threadIdx.x == 0) {
@@ -166,62 +340,97 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       a.st->done = 0;
       a.st->iters = 0;
       a.st->err = INFINITY;
+      a.st->dang = 0;
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
-  __shared__ double s_red[2][kBlock / 64];
-  __shared__ float s_hub[kBlock / 64];
-  __shared__ uint32_t s_unit[2];
+// One sweep.  One workgroup of kSweepThreads per CU: it stages the hot
+// prefix of the gathered vector in LDS, then every wavefront takes units from
+// the queue on its own until the queue is empty.
+template <uint32_t KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
+  constexpr bool kW = KIND != kPlain;
+  constexpr int kNW = kSweepThreads / kWave;
+  extern __shared__ float s_cache[];  // lds_k floats (dynamic)
+  __shared__ double s_red[2][kNW];
   __shared__ uint32_t s_done;
   const uint32_t tid = threadIdx.x;
-  // one decision per workgroup: `done` may be written by workgroup 0 of this
-  // very launch (only when every workgroup stops anyway)
-  if (tid == 0) s_done = a.st->done;
-  __syncthreads();
-  if (s_done) return;
+  const uint32_t wl = tid & (kWave - 1);
+  const uint32_t wv = tid / kWave;
+  float Df = 0.f;
+  uint32_t* queue;
+  if constexpr (KIND != kCoef) {
+    // one decision per workgroup: `done` may be written by workgroup 0 of
+    // this very launch (only when every workgroup stops anyway)
+    if (tid == 0) s_done = a.st->done;
+    __syncthreads();
+    if (s_done) return;
 
-  // totals of the previous sweep, identical in every workgroup
-  double E = 0, D = 0;
-  for (uint32_t b = tid; b < gridDim.x; b += kBlock) {
-    E += a.part_in[2 * b];
-    D += a.part_in[2 * b + 1];
-  }
-  block_sum2(E, D, s_red);
-  const uint32_t done_sweeps = a.st->ctr[a.p ^ 1];
-  if ((done_sweeps > 0 && E < a.tol) || done_sweeps >= a.max_iters) {
-    if (blockIdx.x == 0 && tid == 0) {
-      a.st->done = 1;
-      a.st->iters = done_sweeps;
-      a.st->err = E;
-      if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
-      a.hflag->iters = done_sweeps;
-      a.hflag->err = E;
-      a.hflag->done = 1;
+    // totals of the previous sweep, identical in every workgroup: a fixed
+    // per-lane order, a fixed butterfly, waves added in order
+    double E = 0, D = 0;
+    for (uint32_t b = tid; b < gridDim.x; b += kSweepThreads) {
+      E += a.part_in[2 * b];
+      D += a.part_in[2 * b + 1];
     }
-    return;
-  }
-  if (blockIdx.x == 0 && tid == 0) {
-    a.st->ctr[a.p] = done_sweeps + 1;
-    __hip_atomic_exchange(&a.st->work[a.p ^ 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
-    // progress for the host's chunk planning
-    a.hflag->last_err = E;
-    a.hflag->progress = done_sweeps;
+    E = wave_sum(E);
+    D = wave_sum(D);
+    if (wl == 0) {
+      s_red[0][wv] = E;
+      s_red[1][wv] = D;
+    }
+    __syncthreads();
+    E = 0;
+    D = 0;
+#pragma unroll
+    for (int i = 0; i < kNW; ++i) {
+      E += s_red[0][i];
+      D += s_red[1][i];
+    }
+    const uint32_t done_sweeps = a.st->ctr[a.p ^ 1];
+    if ((done_sweeps > 0 && E < a.tol) || done_sweeps >= a.max_iters) {
+      if (blockIdx.x == 0 && tid == 0) {
+        a.st->done = 1;
+        a.st->iters = done_sweeps;
+        a.st->err = E;
+        a.st->dang = D;
+        if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
+        a.hflag->iters = done_sweeps;
+        a.hflag->err = E;
+        a.hflag->done = 1;
+      }
+      return;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+      a.st->ctr[a.p] = done_sweeps + 1;
+      __hip_atomic_exchange(&a.st->work[a.p ^ 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
+      // progress for the host's chunk planning
+      a.hflag->last_err = E;
+      a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
+      a.hflag->progress = done_sweeps;
+    }
+    Df = static_cast<float>(D);
+    queue = &a.st->work[a.p];
+  } else {
+    queue = &a.st->work[0];  // zeroed by k_reset_queue before the pass
   }
 
-  const float Df = static_cast<float>(D);
-  double err = 0, dang = 0;
-  uint32_t* queue = &a.st->work[a.p];
-  const uint32_t total = a.segs.total_units;
-  if (tid == 0) s_unit[0] = __hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // stage the hot prefix [0, K) of the gathered vector (entry n is the zero
+  // sentinel, so K <= n + 1 covers it when the graph is small)
+  const uint32_t K = a.lds_k;
+  for (uint32_t i = tid; i < K; i += kSweepThreads) s_cache[i] = a.c_in[i];
   __syncthreads();
-  uint32_t sel = 0;
-  uint32_t cur = s_unit[0];
+
+  double err = 0, dang = 0;
+  const uint32_t total = a.segs.total_units;
+  uint32_t cur = 0, nxt = 0;
+  if (wl == 0) cur = __hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  cur = __builtin_amdgcn_readfirstlane(cur);
   while (cur < total) {
-    if (tid == 0)
-      s_unit[sel ^ 1] = __hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // take the next unit now; its latency overlaps this unit's work
+    if (wl == 0) nxt = __hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // segment of this unit (unrolled: constant indices keep the table in SGPRs)
     Seg sg = a.segs.s[0];
 #pragma unroll
@@ -229,67 +438,74 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
       if (i < static_cast<int>(a.segs.nseg) && cur >= a.segs.s[i].unit_lo) sg = a.segs.s[i];
     const uint32_t ul = cur - sg.unit_lo;
     if (sg.G == 0) {
-      // one chunk of a hub row, whole workgroup
+      // one chunk of a hub row, whole wavefront
       const HubUnit hu = a.hub_units[ul];
-      float z = 0.f;
-#pragma unroll 4
-      for (uint32_t k = hu.e_lo + tid; k < hu.e_hi; k += kBlock) z += a.c_in[ld_stream(a.idx + k)];
+      float z = gather_sum<8, kW>(a.idx, a.w, s_cache, K, a.c_in, hu.e_lo + wl, hu.e_hi, kWave, a.n);
       z = wave_sum(z);
-      if ((tid & 63) == 0) s_hub[tid >> 6] = z;
-      __syncthreads();
-      if (tid == 0) {
-        float zs = 0.f;
-#pragma unroll
-        for (int i = 0; i < kBlock / 64; ++i) zs += s_hub[i];
-        uint32_t old = __hip_atomic_exchange(&a.hub_part[ul], __float_as_uint(zs), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-        // the partial must be performed before the ticket is taken
-        asm volatile("s_waitcnt vmcnt(0)" : : "v"(old) : "memory");
-        const uint32_t t = __hip_atomic_fetch_add(&a.hub_ticket[hu.hub], 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        if (t == hu.nchunks - 1) {
-          float zz = 0.f;
-          for (uint32_t c = 0; c < hu.nchunks; ++c)
-            zz += __uint_as_float(__hip_atomic_fetch_or(&a.hub_part[hu.first + c], 0u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT));
-          __hip_atomic_exchange(&a.hub_ticket[hu.hub], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          finish_row(a, hu.row, zz, Df, err, dang);
+      if (wl == 0) {
+        if (hu.nchunks == 1) {
+          finish_row<KIND>(a, hu.row, z, Df, load_row<KIND>(a, hu.row), err, dang);
+        } else {
+          uint32_t old = __hip_atomic_exchange(&a.hub_part[ul], __float_as_uint(z), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+          // the partial must be performed before the ticket is taken
+          asm volatile("s_waitcnt vmcnt(0)" : : "v"(old) : "memory");
+          const uint32_t t = __hip_atomic_fetch_add(&a.hub_ticket[hu.hub], 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+          if (t == hu.nchunks - 1) {
+            float zz = 0.f;
+            for (uint32_t c = 0; c < hu.nchunks; ++c)
+              zz += __uint_as_float(__hip_atomic_fetch_or(&a.hub_part[hu.first + c], 0u, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT));
+            __hip_atomic_exchange(&a.hub_ticket[hu.hub], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            finish_row<KIND>(a, hu.row, zz, Df, load_row<KIND>(a, hu.row), err, dang);
+          }
         }
       }
+    } else if (sg.U == 0) {
+      process_rows<KIND, 0, 4>(a, s_cache, K, sg, ul, Df, err, dang);
+    } else if (sg.U == 8) {
+      process_rows<KIND, 8, 1>(a, s_cache, K, sg, ul, Df, err, dang);
+    } else if (sg.P == 2) {
+      process_rows<KIND, 2, 2>(a, s_cache, K, sg, ul, Df, err, dang);
     } else {
-      const uint32_t G = sg.G;
-      const uint32_t lg = __builtin_ctz(G);
-      const uint32_t lane = tid & (G - 1);
-      const uint32_t row_end = min(sg.row_lo + (ul + 1) * sg.rows_per_unit, sg.row_hi);
-      for (uint32_t row = sg.row_lo + ul * sg.rows_per_unit + (tid >> lg); row < row_end;
-           row += (kBlock >> lg)) {
-        const uint32_t b = a.offs[row], e = a.offs[row + 1];
-        float z = 0.f;
-#pragma unroll 4
-        for (uint32_t k = b + lane; k < e; k += G) z += a.c_in[ld_stream(a.idx + k)];
-        for (uint32_t o = G >> 1; o > 0; o >>= 1) z += __shfl_xor(z, o);
-        if (lane == 0) finish_row(a, row, z, Df, err, dang);
-      }
+      process_rows<KIND, 2, 4>(a, s_cache, K, sg, ul, Df, err, dang);
+    }
+    cur = __builtin_amdgcn_readfirstlane(nxt);  // lane 0 took it; every lane is active here
+  }
+  if constexpr (KIND != kCoef) {
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    __syncthreads();  // every wave is past its last read of s_red
+    if (wl == 0) {
+      s_red[0][wv] = err;
+      s_red[1][wv] = dang;
     }
     __syncthreads();
-    sel ^= 1;
-    cur = s_unit[sel];
-  }
-  block_sum2(err, dang, s_red);
-  if (tid == 0) {
-    a.part_out[2 * blockIdx.x] = err;
-    a.part_out[2 * blockIdx.x + 1] = dang;
+    if (tid == 0) {
+      double te = 0, td = 0;
+#pragma unroll
+      for (int i = 0; i < kNW; ++i) {
+        te += s_red[0][i];
+        td += s_red[1][i];
+      }
+      a.part_out[2 * blockIdx.x] = te;
+      a.part_out[2 * blockIdx.x + 1] = td;
+    }
   }
 }
 
-// out[v] = r_final[iperm[v]]; the final iterate is r[iters & 1].
+__global__ void k_reset_queue(State* st) { st->work[0] = 0; }
+
+// out[v] = r[iperm[v]] * scale.  With `st`, r is the final iterate
+// r_pair[st->iters & 1].
 __global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t* __restrict__ iperm,
-                                                      const float* __restrict__ r0,
-                                                      const float* __restrict__ r1, const State* st,
-                                                      uint32_t n, float* __restrict__ out) {
-  const float* r = (st->iters & 1) ? r1 : r0;
+                                                      const float* __restrict__ r0, const float* __restrict__ r1,
+                                                      const State* st, uint32_t n, float scale,
+                                                      float* __restrict__ out) {
+  const float* r = (st && (st->iters & 1)) ? r1 : r0;
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) out[v] = r[iperm[v]];
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) out[v] = r[iperm[v]] * scale;
 }
 
 uint64_t fnv_mix(uint64_t h, uint64_t v) {
@@ -297,16 +513,24 @@ uint64_t fnv_mix(uint64_t h, uint64_t v) {
   return h * 0x100000001b3ULL;
 }
 
-SweepArgs sweep_args(Graph& g, const nwgr_pr_params& p, uint32_t parity, bool has_tel) {
+SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uint32_t parity) {
   SweepArgs a{};
-  a.offs = g.offs.as<uint32_t>();
-  a.idx = g.idx.as<uint32_t>();
-  a.degf = g.degf.as<float>();
-  a.tel = has_tel ? g.tel.as<float>() : nullptr;
+  a.offs = sp.offs;
+  a.idx = sp.idx;
+  a.w = sp.w;
+  a.rows = sp.plan->rows.p ? sp.plan->rows.as<uint32_t>() : nullptr;
+  a.div = sp.div;
+  a.tel = sp.tel_src ? g.tel.as<float>() : nullptr;
+  a.gsrc = sp.gsrc;
   a.r_in = g.r[parity ^ 1].as<float>();
-  a.c_in = g.c[parity ^ 1].as<float>();
   a.r_out = g.r[parity].as<float>();
-  a.c_out = g.c[parity].as<float>();
+  if (sp.kind == kAdjoint) {
+    a.c_in = a.r_in;  // the adjoint gathers lambda itself
+    a.c_out = nullptr;
+  } else {
+    a.c_in = g.c[parity ^ 1].as<float>();
+    a.c_out = g.c[parity].as<float>();
+  }
   double* part = g.partials.as<double>();
   a.part_in = part + 2 * g.grid * (parity ^ 1);
   a.part_out = part + 2 * g.grid * parity;
@@ -314,11 +538,13 @@ SweepArgs sweep_args(Graph& g, const nwgr_pr_params& p, uint32_t parity, bool ha
   a.hflag = g.hflag_dev;
   a.hist = g.hist.as<double>();
   a.hist_cap = g.hist_cap;
-  a.hub_units = g.hub_units.as<HubUnit>();
-  a.hub_part = g.hub_part.as<uint32_t>();
-  a.hub_ticket = g.hub_ticket.as<uint32_t>();
-  a.segs = g.segs;
+  a.hub_units = sp.plan->hub_units.as<HubUnit>();
+  a.hub_part = sp.plan->hub_part.as<uint32_t>();
+  a.hub_ticket = sp.plan->hub_ticket.as<uint32_t>();
+  a.segs = sp.plan->segs;
   a.p = parity;
+  a.n = g.n;
+  a.lds_k = std::min<uint32_t>(g.tune.lds_cache, g.n + 1);
   a.max_iters = p.max_iters;
   a.dangling = p.dangling;
   const double d = p.damping;
@@ -328,31 +554,41 @@ SweepArgs sweep_args(Graph& g, const nwgr_pr_params& p, uint32_t parity, bool ha
   a.one_minus_d = static_cast<float>(1.0 - d);
   a.un = static_cast<float>(1.0 / g.n);
   a.tol = p.tol;
+  a.rsave = g.rsave.as<float>();
+  a.coef_a = g.coef_a.as<float>();
+  a.coef_b = g.coef_b.as<float>();
   return a;
 }
 
-void validate_params(const nwgr_pr_params& p) {
-  NWGR_REQUIRE(std::isfinite(p.damping) && p.damping >= 0.0 && p.damping <= 1.0,
-               "damping must be in [0, 1]");
-  NWGR_REQUIRE(!std::isnan(p.tol), "tol is NaN");
-  NWGR_REQUIRE(p.dangling <= kDanglingTeleport, "unknown dangling mode");
-  NWGR_REQUIRE(p.dangling != kDanglingTeleport || p.teleport != nullptr,
-               "dangling = TELEPORT needs a teleport vector");
+void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t s) {
+  // the LDS cache is dynamic shared memory; allow up to the full 144 KiB once
+  static const bool attrs = [] {
+    const int bytes = static_cast<int>(kLdsCache * sizeof(float));
+    for (const void* f : {reinterpret_cast<const void*>(&k_sweep<kPlain>),
+                          reinterpret_cast<const void*>(&k_sweep<kWeighted>),
+                          reinterpret_cast<const void*>(&k_sweep<kAdjoint>),
+                          reinterpret_cast<const void*>(&k_sweep<kCoef>)})
+      NWGR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return true;
+  }();
+  (void)attrs;
+  const size_t shm = sizeof(float) * a.lds_k;
+  switch (kind) {
+    case kPlain: k_sweep<kPlain><<<grid, kSweepThreads, shm, s>>>(a); break;
+    case kWeighted: k_sweep<kWeighted><<<grid, kSweepThreads, shm, s>>>(a); break;
+    case kAdjoint: k_sweep<kAdjoint><<<grid, kSweepThreads, shm, s>>>(a); break;
+    default: k_sweep<kCoef><<<grid, kSweepThreads, shm, s>>>(a); break;
+  }
 }
-
-}  // namespace
 
 // Captured chunk of `len` consecutive sweep launches whose first launch has
 // parity `parity` (sweep j has parity j & 1).
-hipGraphExec_t chunk_graph(Graph& g, const nwgr_pr_params& p, bool has_tel, uint32_t parity, uint32_t len,
+hipGraphExec_t chunk_graph(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uint32_t parity, uint32_t len,
                            hipStream_t s) {
-  hipGraphExec_t& e = g.exec[parity][len];
+  hipGraphExec_t& e = g.exec[sp.kind][parity][len];
   if (e) return e;
   NWGR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (uint32_t i = 0; i < len; ++i) {
-    SweepArgs a = sweep_args(g, p, parity ^ (i & 1), has_tel);
-    k_sweep<<<g.grid, kBlock, 0, s>>>(a);
-  }
+  for (uint32_t i = 0; i < len; ++i) launch_sweep(sp.kind, g.grid, sweep_args(g, sp, p, parity ^ (i & 1)), s);
   hipGraph_t graph;
   NWGR_HIP(hipStreamEndCapture(s, &graph));
   hipError_t err = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
@@ -361,21 +597,43 @@ hipGraphExec_t chunk_graph(Graph& g, const nwgr_pr_params& p, bool has_tel, uint
   return e;
 }
 
-void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
-  validate_params(p);
+struct Events {
+  std::vector<hipEvent_t> v;
+  ~Events() {
+    for (auto e : v) (void)hipEventDestroy(e);
+  }
+  hipEvent_t make() {
+    hipEvent_t e;
+    NWGR_HIP(hipEventCreate(&e));
+    v.push_back(e);
+    return e;
+  }
+};
+
+}  // namespace
+
+void validate_params(const nwgr_pr_params& p) {
+  NWGR_REQUIRE(std::isfinite(p.damping) && p.damping >= 0.0 && p.damping <= 1.0, "damping must be in [0, 1]");
+  NWGR_REQUIRE(!std::isnan(p.tol), "tol is NaN");
+  NWGR_REQUIRE(p.dangling <= kDanglingTeleport, "unknown dangling mode");
+  NWGR_REQUIRE(p.dangling != kDanglingTeleport || p.teleport != nullptr,
+               "dangling = TELEPORT needs a teleport vector");
+}
+
+void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s) {
   const uint32_t K = std::clamp<uint32_t>(p.chunk ? p.chunk : g.tune.chunk, 1, Graph::kMaxChunk);
   if (g.hist_cap < p.max_iters || !g.hist.p) {
     g.hist_cap = std::max<uint32_t>(p.max_iters, 1);
     g.hist.alloc(sizeof(double) * g.hist_cap);
     g.drop_graphs();
   }
-  const bool has_tel = p.teleport != nullptr;
+  const bool has_tel = sp.tel_src != nullptr;
   if (has_tel && !g.tel.p) {
     g.tel.alloc(sizeof(float) * g.n);
     g.drop_graphs();
   }
 
-  // the captured graphs bake in the parameters and the stream
+  // the captured graphs bake in the arguments and the stream
   uint64_t key = 0xcbf29ce484222325ULL, bits;
   double dd = p.damping, tt = p.tol;
   std::memcpy(&bits, &dd, 8);
@@ -386,38 +644,33 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   key = fnv_mix(key, p.dangling);
   key = fnv_mix(key, has_tel);
   key = fnv_mix(key, reinterpret_cast<uintptr_t>(s));
-  if (g.exec_key != key) {
-    g.drop_graphs();
-    g.exec_key = key;
+  for (const void* ptr : {static_cast<const void*>(sp.offs), static_cast<const void*>(sp.idx),
+                          static_cast<const void*>(sp.w), static_cast<const void*>(sp.div),
+                          static_cast<const void*>(sp.gsrc), static_cast<const void*>(sp.plan)})
+    key = fnv_mix(key, reinterpret_cast<uintptr_t>(ptr));
+  if (g.exec_key[sp.kind] != key) {
+    g.drop_graphs(static_cast<int>(sp.kind));
+    g.exec_key[sp.kind] = key;
   }
 
-  struct Events {
-    std::vector<hipEvent_t> v;
-    ~Events() {
-      for (auto e : v) (void)hipEventDestroy(e);
-    }
-    hipEvent_t make(bool timing = true) {
-      hipEvent_t e;
-      NWGR_HIP(hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming));
-      v.push_back(e);
-      return e;
-    }
-  } evs;
+  Events evs;
   hipEvent_t ev0 = evs.make(), ev1 = evs.make();
-
   std::memset(const_cast<HostFlag*>(g.hflag), 0, sizeof(HostFlag));
   NWGR_HIP(hipEventRecord(ev0, s));
   InitArgs ia{};
   ia.perm = g.perm.as<uint32_t>();
-  ia.degf = g.degf.as<float>();
-  ia.init = p.init;
-  ia.tel_src = p.teleport;
+  ia.div = sp.div;
+  ia.init = sp.init;
+  ia.tel_src = sp.tel_src;
+  ia.gsrc = sp.gsrc;
   ia.tel = has_tel ? g.tel.as<float>() : nullptr;
   ia.r0 = g.r[0].as<float>();
   ia.c0 = g.c[0].as<float>();
   ia.part0 = g.partials.as<double>();
   ia.st = g.state.as<State>();
   ia.n = g.n;
+  ia.adjoint = sp.kind == kAdjoint;
+  ia.dangling = p.dangling;
   ia.un = static_cast<float>(1.0 / g.n);
   k_init<<<g.grid, kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());
@@ -436,11 +689,11 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   const uint64_t limit = static_cast<uint64_t>(p.max_iters) + 1;  // last launch index needed
   uint64_t next = 1;                                                // next launch index
   SolveProfile prof;
-  uint32_t p1 = 0, p2 = 0;
-  double e1 = 0, e2 = 0;  // two (progress, error) observations
+  uint32_t pobs = 0;      // latest observed progress (sweeps completed)
+  double eobs = 0, rho = 0;  // its error and the per-sweep decay ratio
   auto launch = [&](uint32_t len) {
     Chunk c{evs.make(), evs.make(), len};
-    hipGraphExec_t e = chunk_graph(g, p, has_tel, static_cast<uint32_t>(next & 1), len, s);
+    hipGraphExec_t e = chunk_graph(g, sp, p, static_cast<uint32_t>(next & 1), len, s);
     NWGR_HIP(hipEventRecord(c.a, s));
     NWGR_HIP(hipGraphLaunch(e, s));
     NWGR_HIP(hipEventRecord(c.b, s));
@@ -449,26 +702,33 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
     prof.launches += len;
     prof.chunks += 1;
   };
+  const uint32_t K0 = std::min<uint32_t>(K, 4);  // before the decay rate is known
   auto plan = [&]() -> uint64_t {
     // launches still useful beyond `next - 1`
     if (next > limit) return 0;
-    uint64_t want = K;
-    if (p2 > p1 && e1 > 0 && e2 > 0 && e2 < e1 && std::isfinite(p.tol) && p.tol > 0) {
-      const double rho = std::pow(e2 / e1, 1.0 / (p2 - p1));
-      double more = e2 < p.tol ? 0.0 : std::ceil(std::log(p.tol / e2) / std::log(rho));
+    uint64_t want = K0;
+    if (pobs > 0 && eobs > 0 && rho > 0 && rho < 1 && std::isfinite(p.tol) && p.tol > 0) {
+      double more = eobs < p.tol ? 0.0 : std::ceil(std::log(p.tol / eobs) / std::log(rho));
       if (!std::isfinite(more) || more > 1e6) more = 1e6;
-      const uint64_t stop_launch = p2 + static_cast<uint64_t>(more) + 1;  // launch that detects it
+      const uint64_t stop_launch = pobs + static_cast<uint64_t>(more) + 1;  // launch that detects it
       want = stop_launch >= next ? stop_launch - next + 1 : 0;
+    } else if (pobs > 0) {
+      want = K;  // no usable decay estimate (stagnating or tol <= 0): full chunks
     }
     want = std::min<uint64_t>(want, K);
     want = std::min<uint64_t>(want, limit - next + 1);
     return want;
   };
-  launch(static_cast<uint32_t>(std::min<uint64_t>(K, limit)));
-  bool done = false;
-  while (!done) {
+  auto retire = [&](const Chunk& c) {
+    NWGR_HIP(hipEventSynchronize(c.b));
+    float cms = 0;
+    NWGR_HIP(hipEventElapsedTime(&cms, c.a, c.b));
+    prof.chunk_ms += cms;
+  };
+  launch(static_cast<uint32_t>(std::min<uint64_t>(K0, limit)));
+  for (;;) {
     if (pending.size() - head < 2) {
-      uint64_t len = plan();
+      const uint64_t len = plan();
       if (len > 0) {
         launch(static_cast<uint32_t>(len));
         continue;
@@ -480,34 +740,18 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
       launch(1);
       continue;
     }
-    NWGR_HIP(hipEventSynchronize(pending[head].b));
-    float cms = 0;
-    NWGR_HIP(hipEventElapsedTime(&cms, pending[head].a, pending[head].b));
-    prof.chunk_ms += cms;
-    ++head;
-    if (g.hflag->done) {
-      done = true;
-      break;
-    }
+    retire(pending[head++]);
+    if (g.hflag->done) break;
     const uint32_t pr = g.hflag->progress;
-    const double le = g.hflag->last_err;
-    if (pr > p2) {
-      p1 = p2;
-      e1 = e2;
-      p2 = pr;
-      e2 = le;
+    if (pr > pobs) {
+      pobs = pr;
+      eobs = g.hflag->last_err;
+      const double pe = g.hflag->prev_err;
+      rho = (pr >= 2 && pe > 0 && eobs < pe) ? eobs / pe : 0.0;
     }
   }
-  // drain chunks still in flight (their launches see `done` and return)
-  for (; head < pending.size(); ++head) {
-    NWGR_HIP(hipEventSynchronize(pending[head].b));
-    float cms = 0;
-    NWGR_HIP(hipEventElapsedTime(&cms, pending[head].a, pending[head].b));
-    prof.chunk_ms += cms;
-  }
-  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, d_out);
-  NWGR_HIP(hipGetLastError());
+  // chunks still in flight see `done` and return at once
+  for (; head < pending.size(); ++head) retire(pending[head]);
   NWGR_HIP(hipEventRecord(ev1, s));
   NWGR_HIP(hipEventSynchronize(ev1));
   float ms = 0;
@@ -527,6 +771,50 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
     for (uint32_t i = 0; i < g.last_errs.size(); ++i)
       std::printf("%-5u%-20.10f%-20.10f%-20.10f\n", i, g.last_errs[i], prof.chunk_ms / 1e3 / prof.launches, 0.0);
   }
+}
+
+void run_coef_pass(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, const float* lam, hipStream_t s) {
+  SweepArgs a = sweep_args(g, sp, p, 0);
+  a.c_in = lam;
+  a.r_in = lam;
+  k_reset_queue<<<1, 1, 0, s>>>(g.state.as<State>());
+  NWGR_HIP(hipGetLastError());
+  launch_sweep(kCoef, g.grid, a, s);
+  NWGR_HIP(hipGetLastError());
+}
+
+void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s) {
+  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      g.iperm.as<uint32_t>(), src, src, nullptr, g.n, scale, out);
+  NWGR_HIP(hipGetLastError());
+}
+
+void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
+  validate_params(p);
+  SolveSpec sp;
+  sp.kind = kPlain;
+  sp.plan = &g.fwd;
+  sp.offs = g.offs.as<uint32_t>();
+  sp.idx = g.idx.as<uint32_t>();
+  sp.div = g.degf.as<float>();
+  sp.init = p.init;
+  sp.tel_src = p.teleport;
+  solve(g, sp, p, res, s);
+  hipEvent_t e0, e1;
+  NWGR_HIP(hipEventCreate(&e0));
+  NWGR_HIP(hipEventCreate(&e1));
+  NWGR_HIP(hipEventRecord(e0, s));
+  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, 1.0f, d_out);
+  hipError_t le = hipGetLastError();
+  NWGR_HIP(hipEventRecord(e1, s));
+  NWGR_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  NWGR_HIP(le);
+  res.ms += ms;
 }
 
 }  // namespace nwgr
@@ -588,6 +876,42 @@ uint32_t nwgr_pr_graph_n(const nwgr_pr_graph* g) { return g ? g->n : 0; }
 uint64_t nwgr_pr_graph_m(const nwgr_pr_graph* g) { return g ? g->m : 0; }
 uint64_t nwgr_pr_graph_device_bytes(const nwgr_pr_graph* g) { return g ? nwgr::bytes_of(*g) : 0; }
 double nwgr_pr_graph_setup_ms(const nwgr_pr_graph* g) { return g ? g->setup_ms : 0.0; }
+
+int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && t, "null argument");
+    const nwgr::Tuning& u = g->tune;
+    t->lds_cache = u.lds_cache;
+    t->hub_threshold = u.hub_threshold;
+    t->hub_chunk = u.hub_chunk;
+    t->unit_edges = u.unit_edges;
+    t->zero_rows_per_unit = u.zero_rows_per_unit;
+    t->blocks_per_cu = u.blocks_per_cu;
+    t->chunk = u.chunk;
+  });
+}
+
+int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && t, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    const nwgr::Tuning old = g->tune;
+    g->tune.lds_cache = t->lds_cache;
+    g->tune.hub_threshold = t->hub_threshold;
+    g->tune.hub_chunk = t->hub_chunk;
+    g->tune.unit_edges = t->unit_edges;
+    g->tune.zero_rows_per_unit = t->zero_rows_per_unit;
+    g->tune.blocks_per_cu = t->blocks_per_cu;
+    g->tune.chunk = t->chunk;
+    try {
+      nwgr::replan(*g, g->own_stream);
+    } catch (...) {
+      g->tune = old;
+      nwgr::replan(*g, g->own_stream);
+      throw;
+    }
+  });
+}
 
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,
                            void* stream) {

@@ -106,6 +106,23 @@ class PageRankGraph:
     def setup_ms(self) -> float:
         return float(pagerank_lib().nwgr_pr_graph_setup_ms(self._h))
 
+    @property
+    def tuning(self) -> dict:
+        """Work decomposition of the sweeps (see nwgr_pr_tuning)."""
+        t = _lib.PrTuning()
+        check(pagerank_lib().nwgr_pr_graph_get_tuning(self._h, ctypes.byref(t)))
+        return {name: int(getattr(t, name)) for name, _ in _lib.PrTuning._fields_}
+
+    @tuning.setter
+    def tuning(self, values: dict) -> None:
+        cur = self.tuning
+        unknown = set(values) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(values)
+        t = _lib.PrTuning(**cur)
+        check(pagerank_lib().nwgr_pr_graph_set_tuning(self._h, ctypes.byref(t)))
+
     def last_profile(self) -> dict:
         """Sweep-kernel timing of the last solve: summed event time of the
         captured sweep chunks, number of sweep launches and chunks."""
@@ -147,6 +164,27 @@ class PageRankGraph:
         p.init = init_ptr
         res = PrResult()
         check(pagerank_lib().nwgr_pr_run_device_f32(self._h, ctypes.byref(p), out_ptr, ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def wpr_forward(self, w_ptr: int, out_ptr: int, damping=0.85, tol=1e-6, max_iters=100, dangling="leak",
+                    teleport_ptr=None, chunk=0, stream=None) -> RunResult:
+        """Weighted forward on device arrays (raw pointers, caller order)."""
+        p = _params(damping, tol, max_iters, dangling, chunk, False)
+        p.teleport = teleport_ptr
+        res = PrResult()
+        check(pagerank_lib().nwgr_wpr_forward_device_f32(self._h, ctypes.byref(p), w_ptr, out_ptr,
+                                                         ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def wpr_adjoint(self, w_ptr: int, r_ptr: int, grad_r_ptr: int, grad_w_ptr: int, grad_t_ptr: int, damping=0.85,
+                    tol=1e-6, max_iters=1000, dangling="leak", teleport_ptr=None, chunk=0,
+                    stream=None) -> RunResult:
+        """Adjoint on device arrays; `tol` is an absolute L1 threshold on lambda."""
+        p = _params(damping, tol, max_iters, dangling, chunk, False)
+        p.teleport = teleport_ptr
+        res = PrResult()
+        check(pagerank_lib().nwgr_wpr_adjoint_device_f32(self._h, ctypes.byref(p), w_ptr, r_ptr, grad_r_ptr,
+                                                         grad_w_ptr, grad_t_ptr, ctypes.byref(res), stream))
         return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
 

@@ -71,6 +71,11 @@ def _lib(fast: bool = False):
         getattr(L, fn).argtypes = [u32, _u32p, _u32p, _u32p, real, real, u32, i32, realp, _vp]
     L.orc_wpr_forward_f64.restype = i32
     L.orc_wpr_forward_f64.argtypes = [u32, _u32p, _u32p, _f64p, _f64p, f64, f64, u32, i32, _f64p, _vp]
+    L.orc_wpr_forward_f64_par.restype = i32
+    L.orc_wpr_forward_f64_par.argtypes = [u32, _u32p, _u32p, _f64p, _f64p, f64, f64, u32, i32, _vp, _f64p, i32]
+    L.orc_wpr_adjoint_f64_par.restype = i32
+    L.orc_wpr_adjoint_f64_par.argtypes = [u32, _u32p, _u32p, _f64p, _f64p, _f64p, _f64p, f64, f64, u32, i32,
+                                          _f64p, _f64p, _f64p, i32]
     L.orc_wpr_adjoint_f64.restype = i32
     L.orc_wpr_adjoint_f64.argtypes = [u32, _u32p, _u32p, _f64p, _f64p, _f64p, _f64p, f64, f64, u32, i32,
                                       _f64p, _f64p, _f64p]
@@ -144,17 +149,25 @@ def jacobi_par(offsets, indices, deg, damping=0.85, tol=1e-4, max_iters=20, nthr
     return r, it
 
 
-def wpr_forward(offsets, indices, w, t, damping=0.85, tol=1e-10, max_iters=1000, dangling=LEAK):
-    """FP64 weighted PageRank; returns (r, sweeps)."""
+def wpr_forward(offsets, indices, w, t, damping=0.85, tol=1e-10, max_iters=1000, dangling=LEAK,
+                nthreads=1, init=None):
+    """FP64 weighted PageRank; returns (r, sweeps).  nthreads > 1 uses the
+    OpenMP variant (same per-row sums); init warm-starts it."""
     offsets, indices = _u32(offsets), _u32(indices)
     n = len(offsets) - 1
     r = np.empty(n, np.float64)
-    it = _lib().orc_wpr_forward_f64(n, offsets, indices, _f64(w), _f64(t), damping, tol, max_iters,
-                                    dangling, r, None)
+    if nthreads > 1 or init is not None:
+        ini = None if init is None else _f64(init)
+        it = _lib().orc_wpr_forward_f64_par(n, offsets, indices, _f64(w), _f64(t), damping, tol, max_iters,
+                                            dangling, _ptr(ini), r, max(nthreads, 1))
+    else:
+        it = _lib().orc_wpr_forward_f64(n, offsets, indices, _f64(w), _f64(t), damping, tol, max_iters,
+                                        dangling, r, None)
     return r, it
 
 
-def wpr_adjoint(offsets, indices, w, t, r, g, damping=0.85, tol=1e-12, max_iters=2000, dangling=LEAK):
+def wpr_adjoint(offsets, indices, w, t, r, g, damping=0.85, tol=1e-12, max_iters=2000, dangling=LEAK,
+                nthreads=1):
     """FP64 adjoint; returns (lambda, dL/dw, dL/dt, sweeps)."""
     offsets, indices = _u32(offsets), _u32(indices)
     n = len(offsets) - 1
@@ -162,8 +175,13 @@ def wpr_adjoint(offsets, indices, w, t, r, g, damping=0.85, tol=1e-12, max_iters
     lam = np.empty(n, np.float64)
     dw = np.empty(max(m, 1), np.float64)
     dt = np.empty(n, np.float64)
-    it = _lib().orc_wpr_adjoint_f64(n, offsets, indices, _f64(w), _f64(t), _f64(r), _f64(g), damping,
-                                    tol, max_iters, dangling, lam, dw, dt)
+    fn = _lib().orc_wpr_adjoint_f64_par if nthreads > 1 else None
+    if fn is not None:
+        it = fn(n, offsets, indices, _f64(w), _f64(t), _f64(r), _f64(g), damping, tol, max_iters, dangling,
+                lam, dw, dt, nthreads)
+    else:
+        it = _lib().orc_wpr_adjoint_f64(n, offsets, indices, _f64(w), _f64(t), _f64(r), _f64(g), damping,
+                                        tol, max_iters, dangling, lam, dw, dt)
     return lam, dw[:m], dt, it
 
 

