@@ -116,27 +116,37 @@ def main():
     log(f"generated R-MAT s{args.scale} ef{args.edge_factor}: n={g.n} m={g.m} in {time.perf_counter() - t0:.1f}s")
     pg = PageRankGraph(g, deg)
     log(f"device graph: {pg.device_bytes / 2**20:.0f} MiB, setup {pg.setup_ms:.1f} ms")
+    import torch
     kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform", chunk=args.chunk)
-    out = np.empty(g.n, np.float32)
+    out_d = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
     for i in range(args.warmup):
-        _, res = pg.run(out=out, **kw)
-        log(f"warmup {i}: sweeps={res.iters} err={res.error:.3e} {res.ms:.3f} ms")
+        res = pg.run_device(out_d.data_ptr(), **kw)
+        log(f"warmup {i}: sweeps={res.iters} err={res.error:.3e} {res.ms:.3f} ms, engine {pg.last_engine()}")
     sweeps = 0
     sweep_ms = 0.0
     launches = 0
     dev_ms = 0.0
     iters = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, res = pg.run(out=out, **kw)
+        res = pg.run_device(out_d.data_ptr(), **kw)
         prof = pg.last_profile()
         sweeps += res.iters
         iters.append(res.iters)
         sweep_ms += prof["sweep_ms"]
         launches += prof["launches"]
         dev_ms += res.ms
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    mass = float(out.sum(dtype=np.float64))
+    engine = pg.last_engine()
+    mass = float(out_d.double().sum())
+    # the drop-in's host-vector form (ranks copied back over PCIe), one solve
+    out = np.empty(g.n, np.float32)
+    t1 = time.perf_counter()
+    pg.run(out=out, **kw)
+    host_ms = (time.perf_counter() - t1) * 1e3
     log(f"timed {args.steps} solves: {wall * 1e3:.2f} ms wall, {dev_ms:.2f} ms device, sweeps/solve={iters[0]}, "
         f"sum(r)={mass:.7f}")
     if abs(mass - 1.0) > 1e-4:
@@ -148,7 +158,8 @@ def main():
     traffic = load_traffic(args.scale, args.edge_factor)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_sweep", "algorithmic_bytes_per_sweep": b_iter,
+            "kernel": "k_tile+k_combine" if engine == "tiled" else "k_sweep", "engine": engine,
+            "algorithmic_bytes_per_sweep": b_iter,
             "avg_launch_us": round(sweep_ms / launches * 1e3, 2), "launches": launches, "sweeps": sweeps}
     value = m * sweeps / wall / 1e9
 
@@ -168,7 +179,8 @@ def main():
                                f"{args.seed}, dedup, no self-loops), damping 0.85, dangling uniform, tol "
                                f"{args.tol:g} L1, max {args.max_iters} sweeps, FP32",
                    "n": n, "m": m, "sweeps_per_solve": iters[0], "device_ms_per_solve": round(dev_ms / args.steps, 4),
-                   "setup_ms": round(pg.setup_ms, 1)},
+                   "setup_ms": round(pg.setup_ms, 1), "engine": engine,
+                   "host_output_ms_per_solve": round(host_ms, 3)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

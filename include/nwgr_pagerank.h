@@ -163,7 +163,17 @@ typedef struct nwgr_pr_tuning {
   uint32_t zero_rows_per_unit; /* rows per work unit among rows without entries */
   uint32_t blocks_per_cu;      /* sweep workgroups (of 1024 threads) per compute unit */
   uint32_t chunk;              /* sweeps per captured hipGraph launch */
+  uint32_t engine;             /* NWGR_ENGINE_BINNED, NWGR_ENGINE_TILED or NWGR_ENGINE_AUTO */
+  uint32_t tile_bits;          /* tiled engine: source tile width 2^tile_bits, 13 .. 15 */
+  uint32_t tile_fill_cost;     /* tiled engine: chunks of 512 entries one tile fill is
+                                  charged when the chunks are cut into workgroup ranges */
 } nwgr_pr_tuning;
+
+/* Sweep engines.  BINNED: pull gathers over degree-binned rows, the hottest
+ * contributions staged in LDS.  TILED: source tiles staged in LDS, per-tile
+ * partial sums added by a second pass.  AUTO (default): TILED for a CSR whose
+ * entries average at least two per (row, source tile) run, else BINNED. */
+enum { NWGR_ENGINE_BINNED = 0, NWGR_ENGINE_TILED = 1, NWGR_ENGINE_AUTO = 2 };
 
 int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t);
 /* Validates and applies `t`: re-plans the work units of the handle (cheap,
@@ -180,6 +190,10 @@ uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap)
  * past it) and chunks.  Backs the per-sweep time of the trace table
  * (page_rank.hpp:123,142).  Any output pointer may be NULL. */
 int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* launches, uint32_t* chunks);
+
+/* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED or NWGR_ENGINE_TILED
+ * (-1 before the first solve or for a null handle). */
+int nwgr_pr_last_engine(const nwgr_pr_graph* g);
 
 /* Thread-local message of the last failure on this thread ("" if none). */
 const char* nwgr_last_error(void);

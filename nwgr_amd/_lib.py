@@ -62,7 +62,13 @@ class PrTuning(ctypes.Structure):
         ("zero_rows_per_unit", ctypes.c_uint32),
         ("blocks_per_cu", ctypes.c_uint32),
         ("chunk", ctypes.c_uint32),
+        ("engine", ctypes.c_uint32),
+        ("tile_bits", ctypes.c_uint32),
+        ("tile_fill_cost", ctypes.c_uint32),
     ]
+
+
+ENGINE_BINNED, ENGINE_TILED, ENGINE_AUTO = 0, 1, 2
 
 
 _pr = None
@@ -100,6 +106,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_last_errors.argtypes = [vp, vp, u32]
     L.nwgr_pr_last_profile.restype = i32
     L.nwgr_pr_last_profile.argtypes = [vp, P(f64), P(u32), P(u32)]
+    L.nwgr_pr_last_engine.restype = i32
+    L.nwgr_pr_last_engine.argtypes = [vp]
     L.nwgr_wpr_forward_device_f32.restype = i32
     L.nwgr_wpr_forward_device_f32.argtypes = [vp, P(PrParams), vp, vp, P(PrResult), vp]
     L.nwgr_wpr_adjoint_device_f32.restype = i32

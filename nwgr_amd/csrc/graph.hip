@@ -317,13 +317,26 @@ void replan(Graph& g, hipStream_t stream) {
   NWGR_REQUIRE(t.hub_chunk >= kWave, "hub_chunk must be >= 64");
   NWGR_REQUIRE(t.unit_edges >= 1 && t.zero_rows_per_unit >= 1, "unit sizes must be positive");
   NWGR_REQUIRE(t.chunk >= 1 && t.chunk <= Graph::kMaxChunk, "chunk must be in [1, 32]");
+  NWGR_REQUIRE(t.engine <= kEngineAuto, "engine must be 0 (degree-binned), 1 (tiled) or 2 (auto)");
+  NWGR_REQUIRE(t.tile_bits >= 13 && t.tile_bits <= kMaxTileBits, "tile_bits must be in [13, 15]");
+  NWGR_REQUIRE(t.tile_fill_cost <= 65536, "tile_fill_cost must be <= 65536");
   build_plan(g, g.fwd, g.in_deg_h, g.offs_h, {}, stream);
   if (g.has_t) build_plan(g, g.adj, g.t_deg_h, g.t_offs_h, g.t_rows_h, stream);
-  const uint32_t grid = std::max<uint32_t>(
+  // tiled layouts follow the tile width; their task ranges follow the fill cost
+  for (Tiled* tl : {&g.tfwd, &g.tadj}) {
+    if (!tl->built) continue;
+    if (tl->bits != t.tile_bits)
+      *tl = Tiled{};  // rebuilt on next use
+    else if (tl->nblocks != tile_blocks(g) || tl->fill_cost != t.tile_fill_cost)
+      plan_tasks(g, *tl, stream);
+  }
+  g.grid = std::max<uint32_t>(
       1, std::min<uint32_t>(g.num_cu * t.blocks_per_cu, std::max<uint32_t>(g.fwd.segs.total_units, 64)));
-  if (grid != g.grid || !g.partials.p) {
-    g.grid = grid;
-    g.partials.alloc(sizeof(double) * 2 * 2 * g.grid);
+  g.grid_c = std::max<uint32_t>(1, g.num_cu * 8);
+  const uint32_t stride = std::max(g.grid, g.grid_c);
+  if (stride != g.part_stride || !g.partials.p) {
+    g.part_stride = stride;
+    g.partials.alloc(sizeof(double) * 2 * 2 * stride);
   }
   g.drop_graphs();
 }
@@ -484,10 +497,11 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   replan(*g, stream);  // plans and sweep grid for the default tuning
 
   // --- iteration buffers
-  // one extra entry per vector: index n is the zero sentinel of the gathers
+  // index n is the zero sentinel of the gathers; the zero tail up to whole
+  // maximal tiles lets the tiled engine copy tiles without bounds checks
   for (int b = 0; b < 2; ++b) {
-    g->r[b].alloc(sizeof(float) * (static_cast<size_t>(n) + 1));
-    g->c[b].alloc(sizeof(float) * (static_cast<size_t>(n) + 1));
+    g->r[b].alloc(sizeof(float) * iter_len(n));
+    g->c[b].alloc(sizeof(float) * iter_len(n));
     NWGR_HIP(hipMemsetAsync(g->r[b].p, 0, g->r[b].bytes, stream));
     NWGR_HIP(hipMemsetAsync(g->c[b].p, 0, g->c[b].bytes, stream));
   }

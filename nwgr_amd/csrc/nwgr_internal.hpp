@@ -140,6 +140,7 @@ struct State {
   uint32_t iters;     // sweeps performed (valid once done)
   double err;         // error of the last sweep (valid once done)
   double dang;        // dangling mass (or adjoint Q) of the final iterate (valid once done)
+  double dsum;        // tiled engine: dangling mass (or Q) of the sweep's input iterate
 };
 
 // Host-mapped mirror the solver publishes into: progress every sweep, the
@@ -161,6 +162,11 @@ struct SolveProfile {
   uint32_t chunks = 0;
 };
 
+// Engine choice.  Auto takes the tiled engine for a CSR whose entries form
+// on average at least kTiledMinRun entries per (row, source tile) run.
+constexpr uint32_t kEngineBinned = 0, kEngineTiled = 1, kEngineAuto = 2;
+constexpr double kTiledMinRun = 2.0;
+
 struct Tuning {
   uint32_t lds_cache = kLdsCache;  // hot vertices staged in LDS per sweep workgroup
   uint32_t hub_threshold = 4096;   // rows with in-degree > this are hubs
@@ -169,6 +175,9 @@ struct Tuning {
   uint32_t zero_rows_per_unit = 2048;
   uint32_t blocks_per_cu = 1;
   uint32_t chunk = 16;             // sweeps per captured graph
+  uint32_t engine = kEngineAuto;   // kEngineBinned, kEngineTiled or kEngineAuto
+  uint32_t tile_bits = 15;         // tiled engine: log2 of the source tile width (13 .. 15)
+  uint32_t tile_fill_cost = 64;    // tiled engine: chunk-equivalents charged per tile fill
 };
 
 // Work decomposition of one CSR orientation: segments over row positions and
@@ -181,6 +190,56 @@ struct Plan {
   DevBuf hub_ticket; // u32[] per hub row
   DevBuf rows;       // u32[n] position -> row, or empty (identity)
   uint32_t n_hub_units = 0, n_hubs = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Tiled layout of a CSR (the "tiled" engine).
+//
+// Source vertices are cut into tiles of T = 2^bits consecutive ids (the
+// degree relabelling puts the hottest sources in the first tiles).  Edges are
+// stored tile-major -- ordered by (tile, row, source) -- as 16-bit local ids
+// whose top bit marks the first edge of a sub-pair: a run of edges of one row
+// inside one tile, additionally cut at every chunk boundary.  A chunk is
+// kChunkEdges consecutive entries (8 per lane of a wavefront); every tile's
+// run is padded to whole chunks, the padding forming one dummy sub-pair.
+//
+// A sweep has two passes.  The tile pass gives every workgroup a fixed,
+// contiguous range of chunks (cut at tile boundaries into tasks); per task it
+// copies the tile's contributions into LDS, gathers from there, and reduces
+// every sub-pair to one partial sum stored at the sub-pair's ordinal, so the
+// partials of a chunk are written contiguously.  The combine pass adds each
+// row's partials in tile order (`rslot` lists them row by row) and applies the
+// row epilogue.
+constexpr uint32_t kMaxTileBits = 15;          // local ids use 15 bits of a u16
+constexpr uint32_t kMaxTile = 1u << kMaxTileBits;
+constexpr uint32_t kChunkEdges = 512;
+constexpr uint32_t kWaveRowMin = 32;           // rows with more partials are combined by a wavefront
+
+struct TiledTask {
+  uint32_t tile;
+  uint32_t chunk_lo, chunk_hi;  // chunks [chunk_lo, chunk_hi) of this tile
+};
+
+struct Tiled {
+  bool built = false;
+  uint32_t bits = 0;             // tile width 2^bits the layout was built with
+  uint32_t ntiles = 0, nchunks = 0;
+  uint32_t nsub = 0;             // sub-pairs including the dummy ones
+  uint32_t nblocks = 0;          // workgroups of the tile pass the tasks were planned for
+  uint32_t fill_cost = 0;        // tile fill cost the tasks were planned with
+  uint32_t ntasks = 0;
+  uint32_t nwave_rows = 0;
+  DevBuf eloc;       // u16[nchunks * kChunkEdges]
+  DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
+  DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)
+  DevBuf chunk_sp0;  // u32[nchunks] ordinal of the chunk's first sub-pair
+  DevBuf pair_ptr;   // u32[n + 1] each row's range in rslot
+  DevBuf rslot;      // u32[nsub] sub-pair ordinals, row by row, tiles ascending
+  DevBuf partial;    // f32[nsub] one partial sum per sub-pair ordinal
+  DevBuf tasks;      // TiledTask[ntasks]
+  DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
+  DevBuf wave_rows;  // u32[nwave_rows] rows with more than kWaveRowMin partials
+  std::vector<uint32_t> tile_chunk0;  // host: first chunk of each tile (ntiles + 1)
 };
 
 // Kinds of fixed-point solve sharing the sweep machinery.
@@ -203,6 +262,8 @@ struct SolveSpec {
   const float* gsrc = nullptr;     // adjoint source term, new labels
   const float* init = nullptr;     // forward initial ranks in caller order, or null
   const float* tel_src = nullptr;  // teleport in caller order (permuted at init), or null
+  const Tiled* tiled = nullptr;    // tiled engine layout, or null for the degree-binned engine
+  const float* tw = nullptr;       // weights in the tiled order (weighted kinds, tiled engine)
 };
 
 struct Graph {
@@ -232,6 +293,11 @@ struct Graph {
   DevBuf t_pos;     // u32[m]  out-CSR entry -> in-CSR (new order) position
   Plan adj;         // plan of the out-CSR (row list by out-degree)
   std::vector<uint32_t> t_deg_h, t_rows_h, t_offs_h;  // host copies for re-planning
+  Tiled tfwd, tadj;  // tiled layouts of the in-CSR and of the transposed CSR (built on first use)
+  int auto_tiled[2] = {-1, -1};  // engine auto choice per CSR (fwd, adj): -1 not measured yet
+  uint32_t auto_bits = 0;        // tile width the auto choices were measured for
+  uint32_t grid_c = 0;  // workgroups of the tiled engine's combine pass
+  uint32_t part_stride = 0;  // partial pairs per parity in `partials` (max of both producers)
 
   // weighted / adjoint work arrays
   DevBuf w_in, w_out;  // f32[m] weights in in-CSR (new) and out-CSR order
@@ -258,6 +324,7 @@ struct Graph {
 
   std::vector<double> last_errs;
   SolveProfile last_profile;
+  int last_engine = -1;
 
   void drop_graphs(int kind = -1) {
     for (int k = 0; k < kNumKinds; ++k) {
@@ -297,6 +364,22 @@ void build_plan(Graph& g, Plan& plan, const std::vector<uint32_t>& deg_by_pos,
 void ensure_transpose(Graph& g, hipStream_t s);
 // validates g.tune, rebuilds the plans and the sweep grid (graph.hip)
 void replan(Graph& g, hipStream_t s);
+// tiled layout of a CSR over rows [0, n) (tiled.hip); no-op once built
+void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s);
+// cut the chunks of `t` into per-workgroup task ranges (tiled.hip)
+void plan_tasks(Graph& g, Tiled& t, hipStream_t s);
+// workgroups of the tile pass for the current tile width
+uint32_t tile_blocks(const Graph& g);
+// the tiled layout the engine choice selects for the in-CSR (adj = false) or
+// the transposed CSR (adj = true), built on first use; null = degree-binned
+Tiled* select_tiled(Graph& g, bool adj, hipStream_t s);
+// entries (>= 1) of `weights` (CSR order, padded with zeros) gathered into the
+// tiled order of `t` (t.w)
+void tiled_weights(Graph& g, Tiled& t, const float* weights, hipStream_t s);
+// elements of an iteration vector (r, c): n + 1 rounded up to whole maximal tiles
+inline size_t iter_len(uint32_t n) {
+  return ((static_cast<size_t>(n) + 1 + kMaxTile - 1) / kMaxTile) * kMaxTile;
+}
 
 // solves (pagerank.hip, weighted.hip)
 // Runs the chunked fixed-point iteration; the final iterate is left in

@@ -495,6 +495,313 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled engine.  Per sweep: k_tile reduces every sub-pair (one row's entries
+// inside one source tile) to a partial sum from an LDS copy of the tile's
+// contributions; k_combine adds each row's partials (tiles ascending) and
+// applies the row epilogue.
+
+struct TileArgs {
+  const uint16_t* __restrict__ eloc;
+  const float* __restrict__ w;  // weights in tiled order (weighted kinds)
+  const uint32_t* __restrict__ chunk_sp0;
+  const TiledTask* __restrict__ tasks;
+  const uint32_t* __restrict__ wg_task;
+  const float* __restrict__ c_in;  // padded to whole tiles
+  float* __restrict__ partial;
+  const double* __restrict__ part_in;
+  State* st;
+  HostFlag* hflag;
+  double* hist;
+  uint32_t hist_cap;
+  uint32_t nparts;  // partial pairs written by the previous combine pass (or init)
+  uint32_t bits;    // tile width 2^bits
+  uint32_t p;
+  uint32_t max_iters;
+  double tol;
+};
+
+struct CombineArgs {
+  SweepArgs s;  // epilogue operands (r_in, r_out, c_out, div, tel, ...)
+  const uint32_t* __restrict__ pair_ptr;
+  const uint32_t* __restrict__ rslot;
+  const float* __restrict__ partial;
+  const uint32_t* __restrict__ wave_rows;
+  uint32_t nwave_rows;
+};
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The loads of one chunk for one lane: 8 local ids (and 8 weights) plus the
+// chunk's first sub-pair ordinal.  Issued ahead of use (see k_tile).
+struct ChunkIn {
+  u32x4 e;
+  f32x4 w0, w1;
+  uint32_t sp0;
+};
+
+template <bool W>
+__device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uint32_t wl) {
+  ChunkIn in;
+  const uint64_t base = static_cast<uint64_t>(c) * kChunkEdges + wl * 8;
+  in.e = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base));
+  if constexpr (W) {
+    in.w0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base));
+    in.w1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 4));
+  }
+  in.sp0 = a.chunk_sp0[c];
+  return in;
+}
+
+// One chunk (kChunkEdges entries, 8 per lane): gather from LDS, segmented
+// reduction over the sub-pairs, one partial per sub-pair end, stored at the
+// sub-pair's ordinal (consecutive ordinals across the wavefront).
+template <bool W>
+__device__ __forceinline__ void tile_chunk(const TileArgs& a, const float* __restrict__ s_tile, uint32_t mask,
+                                           const ChunkIn& in, uint32_t wl) {
+  const uint32_t words[4] = {in.e.x, in.e.y, in.e.z, in.e.w};
+  float s[8];
+  uint32_t f = 0;  // bit i: entry i starts a sub-pair
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t e = (words[i >> 1] >> ((i & 1) * 16)) & 0xffffu;
+    f |= (e >> 15) << i;
+    s[i] = s_tile[e & mask];
+  }
+  if constexpr (W) {
+    s[0] *= in.w0.x; s[1] *= in.w0.y; s[2] *= in.w0.z; s[3] *= in.w0.w;
+    s[4] *= in.w1.x; s[5] *= in.w1.y; s[6] *= in.w1.z; s[7] *= in.w1.w;
+  }
+  // inclusive segmented scan inside the lane
+#pragma unroll
+  for (int i = 1; i < 8; ++i)
+    if (!((f >> i) & 1u)) s[i] = s[i - 1] + s[i];
+  // inclusive segmented scan of the lane tails across the wavefront
+  float tv = s[7];
+  uint32_t tf = f != 0;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const float ov = __shfl_up(tv, d);
+    const uint32_t of = __shfl_up(tf, d);
+    if (wl >= static_cast<uint32_t>(d)) {
+      if (!tf) tv = ov + tv;
+      tf |= of;
+    }
+  }
+  float carry = __shfl_up(tv, 1);
+  if (wl == 0) carry = 0.f;
+  // entries before the lane's first sub-pair start continue the open sub-pair
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if ((f & ((2u << i) - 1u)) == 0) s[i] = carry + s[i];
+  // sub-pair ordinals: starts before this lane within the chunk
+  uint32_t run = __popc(f);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = __shfl_up(run, d);
+    if (wl >= static_cast<uint32_t>(d)) run += o;
+  }
+  run -= __popc(f);  // exclusive
+  const uint32_t first_next = __shfl_down(f & 1u, 1);
+  const uint32_t next_lane_starts = wl == kWave - 1 ? 1u : first_next;  // chunks end sub-pairs
+  float* __restrict__ out = a.partial + in.sp0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    run += (f >> i) & 1u;
+    const bool end = i < 7 ? ((f >> (i + 1)) & 1u) : next_lane_starts;
+    if (end) out[run - 1] = s[i];
+  }
+}
+
+// Tile copy into LDS: all J loads of a thread are issued before its LDS
+// stores.  The source vector is padded to whole tiles, so no bounds checks.
+template <int J>
+__device__ __forceinline__ void fill_tile(const float4* __restrict__ src, float4* __restrict__ dst, uint32_t tid) {
+  float4 v[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) v[j] = src[tid + j * kSweepThreads];
+#pragma unroll
+  for (int j = 0; j < J; ++j) dst[tid + j * kSweepThreads] = v[j];
+}
+
+template <uint32_t KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
+  constexpr bool kW = KIND != kPlain;
+  constexpr int kNW = kSweepThreads / kWave;
+  extern __shared__ float s_tile[];  // 2^bits floats (dynamic)
+  __shared__ double s_red[2][kNW];
+  __shared__ uint32_t s_flag;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wl = tid & (kWave - 1);
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+  if constexpr (KIND != kCoef) {
+    if (tid == 0) s_flag = a.st->done;
+    __syncthreads();
+    if (s_flag) return;
+    // totals of the previous sweep, identical in every workgroup
+    double E = 0, D = 0;
+    for (uint32_t b = tid; b < a.nparts; b += kSweepThreads) {
+      E += a.part_in[2 * b];
+      D += a.part_in[2 * b + 1];
+    }
+    E = wave_sum(E);
+    D = wave_sum(D);
+    if (wl == 0) {
+      s_red[0][wv] = E;
+      s_red[1][wv] = D;
+    }
+    __syncthreads();
+    E = 0;
+    D = 0;
+#pragma unroll
+    for (int i = 0; i < kNW; ++i) {
+      E += s_red[0][i];
+      D += s_red[1][i];
+    }
+    const uint32_t done_sweeps = a.st->ctr[a.p ^ 1];
+    if ((done_sweeps > 0 && E < a.tol) || done_sweeps >= a.max_iters) {
+      if (blockIdx.x == 0 && tid == 0) {
+        a.st->done = 1;
+        a.st->iters = done_sweeps;
+        a.st->err = E;
+        a.st->dang = D;
+        if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
+        a.hflag->iters = done_sweeps;
+        a.hflag->err = E;
+        a.hflag->done = 1;
+      }
+      return;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+      a.st->ctr[a.p] = done_sweeps + 1;
+      a.st->dsum = D;  // read by this sweep's combine pass
+      if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
+      a.hflag->last_err = E;
+      a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
+      a.hflag->progress = done_sweeps;
+    }
+  }
+
+  const uint32_t mask = (1u << a.bits) - 1;
+  const uint32_t nq = (mask + 1) / 4;  // float4 per tile
+  const uint32_t t_lo = a.wg_task[blockIdx.x], t_hi = a.wg_task[blockIdx.x + 1];
+  for (uint32_t ti = t_lo; ti < t_hi; ++ti) {
+    const TiledTask task = a.tasks[ti];
+    if (ti > t_lo) __syncthreads();  // every wave is done with the previous tile
+    {
+      const float4* src = reinterpret_cast<const float4*>(a.c_in + (static_cast<size_t>(task.tile) << a.bits));
+      float4* dst = reinterpret_cast<float4*>(s_tile);
+      if (nq == kMaxTile / 4)
+        fill_tile<kMaxTile / 4 / kSweepThreads>(src, dst, tid);
+      else if (nq == kMaxTile / 8)
+        fill_tile<kMaxTile / 8 / kSweepThreads>(src, dst, tid);
+      else
+        fill_tile<kMaxTile / 16 / kSweepThreads>(src, dst, tid);
+    }
+    __syncthreads();
+    // wave wv takes chunks lo + wv, lo + wv + kNW, ...; the loads of the next
+    // two chunks are in flight while one is reduced
+    const uint32_t hi = task.chunk_hi;
+    uint32_t c = task.chunk_lo + wv;
+    ChunkIn q0{}, q1{}, q2{};
+    if (c < hi) q0 = load_chunk<kW>(a, c, wl);
+    if (c + kNW < hi) q1 = load_chunk<kW>(a, c + kNW, wl);
+    for (; c < hi; c += 3 * kNW) {
+      if (c + 2 * kNW < hi) q2 = load_chunk<kW>(a, c + 2 * kNW, wl);
+      tile_chunk<kW>(a, s_tile, mask, q0, wl);
+      if (c + kNW >= hi) break;
+      if (c + 3 * kNW < hi) q0 = load_chunk<kW>(a, c + 3 * kNW, wl);
+      tile_chunk<kW>(a, s_tile, mask, q1, wl);
+      if (c + 2 * kNW >= hi) break;
+      if (c + 4 * kNW < hi) q1 = load_chunk<kW>(a, c + 4 * kNW, wl);
+      tile_chunk<kW>(a, s_tile, mask, q2, wl);
+    }
+  }
+}
+
+// Partials staged in LDS per workgroup step of kBlock rows.
+constexpr uint32_t kCombineStage = 4096;
+
+template <uint32_t KIND>
+__global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
+  __shared__ double s_red[2][kBlock / 64];
+  __shared__ float s_buf[kCombineStage];
+  __shared__ uint32_t s_flag;
+  const uint32_t tid = threadIdx.x;
+  float Df = 0.f;
+  if constexpr (KIND != kCoef) {
+    if (tid == 0) s_flag = a.s.st->done;
+    __syncthreads();
+    if (s_flag) return;
+    Df = static_cast<float>(a.s.st->dsum);
+  }
+  double err = 0, dang = 0;
+  const uint32_t wl = tid & (kWave - 1);
+  const uint32_t* __restrict__ pp = a.pair_ptr;
+  // rows with many partials: one wavefront each (lane-strided, then a butterfly)
+  const uint32_t nwaves = gridDim.x * (kBlock / kWave);
+  for (uint32_t i = blockIdx.x * (kBlock / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
+    const uint32_t row = a.wave_rows[i];
+    const uint32_t b = pp[row], e = pp[row + 1];
+    const RowIn in = load_row<KIND>(a.s, row);
+    float z = 0.f;
+    for (uint32_t k = b + wl; k < e; k += kWave) z += a.partial[a.rslot[k]];
+    z = wave_sum(z);
+    if (wl == 0) finish_row<KIND>(a.s, row, z, Df, in, err, dang);
+  }
+  // the other rows: a thread each, kBlock consecutive rows per step; the
+  // step's partials are gathered into LDS by the whole workgroup first
+  const uint32_t n = a.s.n;
+  for (uint32_t r0 = blockIdx.x * kBlock; r0 < n; r0 += gridDim.x * kBlock) {
+    const uint32_t row = r0 + tid;
+    const bool valid = row < n;
+    const uint32_t base = pp[r0], end = pp[min(r0 + kBlock, n)];
+    uint32_t b = 0, e = 0;
+    RowIn in{};
+    if (valid) {
+      b = pp[row];
+      e = pp[row + 1];
+      in = load_row<KIND>(a.s, row);
+    }
+    const bool staged = end - base <= kCombineStage;
+    if (staged) {
+      for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kBlock) {
+        uint32_t sl[4];
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t k = k0 + j * kBlock;
+          sl[j] = k < end ? a.rslot[k] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = k0 + j * kBlock < end ? a.partial[sl[j]] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (k0 + j * kBlock < end) s_buf[k0 + j * kBlock - base] = v[j];
+      }
+    }
+    __syncthreads();
+    if (valid && e - b <= kWaveRowMin) {
+      float z = 0.f;
+      if (staged) {
+        for (uint32_t k = b; k < e; ++k) z += s_buf[k - base];
+      } else {
+        for (uint32_t k = b; k < e; ++k) z += a.partial[a.rslot[k]];
+      }
+      finish_row<KIND>(a.s, row, z, Df, in, err, dang);
+    }
+    __syncthreads();
+  }
+  if constexpr (KIND != kCoef) {
+    block_sum2(err, dang, s_red);
+    if (tid == 0) {
+      a.s.part_out[2 * blockIdx.x] = err;
+      a.s.part_out[2 * blockIdx.x + 1] = dang;
+    }
+  }
+}
+
 __global__ void k_reset_queue(State* st) { st->work[0] = 0; }
 
 // out[v] = r[iperm[v]] * scale.  With `st`, r is the final iterate
@@ -532,8 +839,8 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
     a.c_out = g.c[parity].as<float>();
   }
   double* part = g.partials.as<double>();
-  a.part_in = part + 2 * g.grid * (parity ^ 1);
-  a.part_out = part + 2 * g.grid * parity;
+  a.part_in = part + 2 * g.part_stride * (parity ^ 1);
+  a.part_out = part + 2 * g.part_stride * parity;
   a.st = g.state.as<State>();
   a.hflag = g.hflag_dev;
   a.hist = g.hist.as<double>();
@@ -581,6 +888,67 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
   }
 }
 
+// One tiled sweep: k_tile over the workgroup task ranges, then k_combine.
+void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
+  static const bool attrs = [] {
+    const int bytes = static_cast<int>(kMaxTile * sizeof(float));
+    for (const void* f : {reinterpret_cast<const void*>(&k_tile<kPlain>),
+                          reinterpret_cast<const void*>(&k_tile<kWeighted>),
+                          reinterpret_cast<const void*>(&k_tile<kAdjoint>),
+                          reinterpret_cast<const void*>(&k_tile<kCoef>)})
+      NWGR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return true;
+  }();
+  (void)attrs;
+  const Tiled& t = *sp.tiled;
+  TileArgs ta{};
+  ta.eloc = t.eloc.as<uint16_t>();
+  ta.w = sp.tw;
+  ta.chunk_sp0 = t.chunk_sp0.as<uint32_t>();
+  ta.tasks = t.tasks.as<TiledTask>();
+  ta.wg_task = t.wg_task.as<uint32_t>();
+  ta.c_in = sa.c_in;
+  ta.partial = t.partial.as<float>();
+  ta.part_in = sa.part_in;
+  ta.st = sa.st;
+  ta.hflag = sa.hflag;
+  ta.hist = sa.hist;
+  ta.hist_cap = sa.hist_cap;
+  ta.nparts = g.grid_c;
+  ta.bits = t.bits;
+  ta.p = sa.p;
+  ta.max_iters = sa.max_iters;
+  ta.tol = sa.tol;
+  CombineArgs ca{};
+  ca.s = sa;
+  ca.pair_ptr = t.pair_ptr.as<uint32_t>();
+  ca.rslot = t.rslot.as<uint32_t>();
+  ca.partial = t.partial.as<float>();
+  ca.wave_rows = t.wave_rows.as<uint32_t>();
+  ca.nwave_rows = t.nwave_rows;
+  const size_t shm = sizeof(float) << t.bits;
+  const uint32_t nb = t.nblocks;
+  switch (sp.kind) {
+    case kPlain:
+      k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
+      k_combine<kPlain><<<g.grid_c, kBlock, 0, s>>>(ca);
+      break;
+    case kWeighted:
+      k_tile<kWeighted><<<nb, kSweepThreads, shm, s>>>(ta);
+      k_combine<kWeighted><<<g.grid_c, kBlock, 0, s>>>(ca);
+      break;
+    case kAdjoint:
+      k_tile<kAdjoint><<<nb, kSweepThreads, shm, s>>>(ta);
+      k_combine<kAdjoint><<<g.grid_c, kBlock, 0, s>>>(ca);
+      break;
+    default:
+      k_tile<kCoef><<<nb, kSweepThreads, shm, s>>>(ta);
+      k_combine<kCoef><<<g.grid_c, kBlock, 0, s>>>(ca);
+      break;
+  }
+  NWGR_HIP(hipGetLastError());
+}
+
 // Captured chunk of `len` consecutive sweep launches whose first launch has
 // parity `parity` (sweep j has parity j & 1).
 hipGraphExec_t chunk_graph(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uint32_t parity, uint32_t len,
@@ -588,7 +956,13 @@ hipGraphExec_t chunk_graph(Graph& g, const SolveSpec& sp, const nwgr_pr_params& 
   hipGraphExec_t& e = g.exec[sp.kind][parity][len];
   if (e) return e;
   NWGR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (uint32_t i = 0; i < len; ++i) launch_sweep(sp.kind, g.grid, sweep_args(g, sp, p, parity ^ (i & 1)), s);
+  for (uint32_t i = 0; i < len; ++i) {
+    const SweepArgs a = sweep_args(g, sp, p, parity ^ (i & 1));
+    if (sp.tiled)
+      launch_tiled(g, sp, a, s);
+    else
+      launch_sweep(sp.kind, g.grid, a, s);
+  }
   hipGraph_t graph;
   NWGR_HIP(hipStreamEndCapture(s, &graph));
   hipError_t err = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
@@ -646,7 +1020,8 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   key = fnv_mix(key, reinterpret_cast<uintptr_t>(s));
   for (const void* ptr : {static_cast<const void*>(sp.offs), static_cast<const void*>(sp.idx),
                           static_cast<const void*>(sp.w), static_cast<const void*>(sp.div),
-                          static_cast<const void*>(sp.gsrc), static_cast<const void*>(sp.plan)})
+                          static_cast<const void*>(sp.gsrc), static_cast<const void*>(sp.plan),
+                          static_cast<const void*>(sp.tiled), static_cast<const void*>(sp.tw)})
     key = fnv_mix(key, reinterpret_cast<uintptr_t>(ptr));
   if (g.exec_key[sp.kind] != key) {
     g.drop_graphs(static_cast<int>(sp.kind));
@@ -672,7 +1047,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.adjoint = sp.kind == kAdjoint;
   ia.dangling = p.dangling;
   ia.un = static_cast<float>(1.0 / g.n);
-  k_init<<<g.grid, kBlock, 0, s>>>(ia);
+  k_init<<<sp.tiled ? g.grid_c : g.grid, kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());
 
   // Chunks of captured sweeps, at most two in flight.  Launch j performs
@@ -761,6 +1136,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   res.converged = (res.iters > 0 && res.error < p.tol) ? 1u : 0u;
   res.ms = ms;
   g.last_profile = prof;
+  g.last_engine = sp.tiled ? static_cast<int>(kEngineTiled) : static_cast<int>(kEngineBinned);
   g.last_errs.assign(std::min(res.iters, g.hist_cap), 0.0);
   if (!g.last_errs.empty())
     NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
@@ -779,7 +1155,13 @@ void run_coef_pass(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, const
   a.r_in = lam;
   k_reset_queue<<<1, 1, 0, s>>>(g.state.as<State>());
   NWGR_HIP(hipGetLastError());
-  launch_sweep(kCoef, g.grid, a, s);
+  if (sp.tiled) {
+    SolveSpec sc = sp;
+    sc.kind = kCoef;
+    launch_tiled(g, sc, a, s);
+  } else {
+    launch_sweep(kCoef, g.grid, a, s);
+  }
   NWGR_HIP(hipGetLastError());
 }
 
@@ -799,6 +1181,7 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   sp.div = g.degf.as<float>();
   sp.init = p.init;
   sp.tel_src = p.teleport;
+  sp.tiled = select_tiled(g, false, s);
   solve(g, sp, p, res, s);
   hipEvent_t e0, e1;
   NWGR_HIP(hipEventCreate(&e0));
@@ -873,6 +1256,7 @@ void nwgr_pr_graph_destroy(nwgr_pr_graph* g) {
 }
 
 uint32_t nwgr_pr_graph_n(const nwgr_pr_graph* g) { return g ? g->n : 0; }
+int nwgr_pr_last_engine(const nwgr_pr_graph* g) { return g ? g->last_engine : -1; }
 uint64_t nwgr_pr_graph_m(const nwgr_pr_graph* g) { return g ? g->m : 0; }
 uint64_t nwgr_pr_graph_device_bytes(const nwgr_pr_graph* g) { return g ? nwgr::bytes_of(*g) : 0; }
 double nwgr_pr_graph_setup_ms(const nwgr_pr_graph* g) { return g ? g->setup_ms : 0.0; }
@@ -888,6 +1272,9 @@ int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t) {
     t->zero_rows_per_unit = u.zero_rows_per_unit;
     t->blocks_per_cu = u.blocks_per_cu;
     t->chunk = u.chunk;
+    t->engine = u.engine;
+    t->tile_bits = u.tile_bits;
+    t->tile_fill_cost = u.tile_fill_cost;
   });
 }
 
@@ -903,6 +1290,9 @@ int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
     g->tune.zero_rows_per_unit = t->zero_rows_per_unit;
     g->tune.blocks_per_cu = t->blocks_per_cu;
     g->tune.chunk = t->chunk;
+    g->tune.engine = t->engine;
+    g->tune.tile_bits = t->tile_bits;
+    g->tune.tile_fill_cost = t->tile_fill_cost;
     try {
       nwgr::replan(*g, g->own_stream);
     } catch (...) {

@@ -1,0 +1,410 @@
+// Construction of the tiled layout of a CSR (see `Tiled` in nwgr_internal.hpp).
+//
+// Input: a CSR over rows [0, n) whose entries are sorted within each row (the
+// relabelled in-CSR, or its transpose).  Everything is built on the device
+// with radix sorts and scans; the host only plans the tile padding, the
+// workgroup task ranges and the list of rows the combine pass gives a
+// wavefront each.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <numeric>
+
+#include "nwgr_internal.hpp"
+
+namespace nwgr {
+
+namespace {
+
+constexpr int kT = 256;
+
+inline uint32_t grid_for(uint64_t work) {
+  uint64_t g = (work + kT - 1) / kT;
+  return static_cast<uint32_t>(std::clamp<uint64_t>(g, 1, 65535u * 4));
+}
+
+uint32_t bits_for(uint64_t v) {
+  uint32_t b = 1;
+  while (b < 32 && (uint64_t{1} << b) <= v) ++b;
+  return b;
+}
+
+#define GRID_STRIDE(i, count)                                                                       \
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < (count);     \
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+
+// Row of every CSR entry (binary search in offs), tile of every entry, and
+// entries per tile.
+__global__ void k_rows_and_tiles(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ idx, uint32_t n,
+                                 uint64_t m, uint32_t bits, uint32_t* erow, uint32_t* tkey, uint32_t* tval,
+                                 uint32_t* tile_cnt) {
+  GRID_STRIDE(k, m) {
+    uint32_t lo = 0, hi = n;  // offs[lo] <= k < offs[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (offs[mid] <= k) lo = mid; else hi = mid;
+    }
+    erow[k] = lo;
+    const uint32_t t = idx[k] >> bits;
+    tkey[k] = t;
+    tval[k] = static_cast<uint32_t>(k);
+    atomicAdd(&tile_cnt[t], 1u);
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
+  GRID_STRIDE(i, count) p[i] = v;
+}
+
+// Place entry k' of the tile-sorted order at its padded position and mark the
+// first entry of every sub-pair.
+__global__ void k_tile_place(const uint32_t* __restrict__ tkey, const uint32_t* __restrict__ tperm,
+                             const uint32_t* __restrict__ idx, const uint32_t* __restrict__ erow,
+                             const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_chunk0,
+                             uint64_t m, uint32_t mask, uint16_t* eloc, uint32_t* epos, uint32_t* flagv) {
+  GRID_STRIDE(k, m) {
+    const uint32_t t = tkey[k];
+    const uint32_t i = static_cast<uint32_t>(k) - tile_start[t];
+    const uint64_t P = static_cast<uint64_t>(tile_chunk0[t]) * kChunkEdges + i;
+    const uint32_t e = tperm[k];
+    const uint32_t row = erow[e];
+    const bool first = i == 0 || (P % kChunkEdges) == 0 || erow[tperm[k - 1]] != row;
+    eloc[P] = static_cast<uint16_t>((idx[e] & mask) | (first ? 0x8000u : 0u));
+    epos[P] = e;
+    flagv[P] = first ? 1u : 0u;
+  }
+}
+
+// The padding after a tile's entries forms one dummy sub-pair.
+__global__ void k_pad_flags(const uint32_t* __restrict__ tile_cnt, const uint32_t* __restrict__ tile_chunk0,
+                            uint32_t ntiles, uint16_t* eloc, uint32_t* flagv) {
+  GRID_STRIDE(t, ntiles) {
+    const uint32_t cnt = tile_cnt[t];
+    if (cnt % kChunkEdges) {
+      const uint64_t P = static_cast<uint64_t>(tile_chunk0[t]) * kChunkEdges + cnt;
+      eloc[P] = 0x8000u;
+      flagv[P] = 1u;
+    }
+  }
+}
+
+// Row of every sub-pair (n for the dummy ones) and sub-pairs per row.
+__global__ void k_subpairs(const uint32_t* __restrict__ flagv, const uint32_t* __restrict__ ord,
+                           const uint32_t* __restrict__ epos, const uint32_t* __restrict__ erow, uint64_t mp,
+                           uint64_t m, uint32_t n, uint32_t* sprow, uint32_t* rowcnt) {
+  GRID_STRIDE(P, mp) {
+    if (flagv[P]) {
+      const uint32_t e = epos[P];
+      const uint32_t row = e < m ? erow[e] : n;
+      sprow[ord[P]] = row;
+      if (row < n) atomicAdd(&rowcnt[row], 1u);
+    }
+  }
+}
+
+__global__ void k_chunk_sp0(const uint32_t* __restrict__ ord, uint32_t nchunks, uint32_t* chunk_sp0) {
+  GRID_STRIDE(c, nchunks) chunk_sp0[c] = ord[c * static_cast<uint64_t>(kChunkEdges)];
+}
+
+__global__ void k_iota(uint32_t* v, uint64_t count) {
+  GRID_STRIDE(i, count) v[i] = static_cast<uint32_t>(i);
+}
+
+// (row, tile) runs of a CSR with rows sorted by source: one wavefront per row,
+// one RMW atomic per wavefront.
+__global__ __launch_bounds__(kT) void k_count_runs(const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ idx, uint32_t n, uint32_t bits,
+                                                   unsigned long long* runs) {
+  const uint32_t wl = threadIdx.x & 63;
+  const uint32_t nw = gridDim.x * (kT / 64);
+  uint32_t cnt = 0;
+  for (uint32_t row = blockIdx.x * (kT / 64) + threadIdx.x / 64; row < n; row += nw) {
+    const uint32_t b = offs[row], e = offs[row + 1];
+    for (uint32_t k = b + wl; k < e; k += 64) cnt += (k == b || (idx[k] >> bits) != (idx[k - 1] >> bits)) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (wl == 0 && cnt) atomicAdd(runs, static_cast<unsigned long long>(cnt));
+}
+
+__global__ void k_gather_w(const float* __restrict__ src, const uint32_t* __restrict__ pos, uint64_t count,
+                           float* __restrict__ dst) {
+  GRID_STRIDE(i, count) dst[i] = src[pos[i]];
+}
+
+template <class Fn>
+void cub_call(Fn&& fn, hipStream_t s) {
+  size_t bytes = 0;
+  NWGR_HIP(fn(nullptr, bytes));
+  DevBuf t;
+  t.alloc(bytes);
+  NWGR_HIP(fn(t.p, bytes));
+  (void)s;
+}
+
+}  // namespace
+
+uint32_t tile_blocks(const Graph& g) {
+  // one 1024-thread workgroup per CU holds a 128 KiB tile; narrower tiles
+  // leave room for two
+  return g.num_cu * (g.tune.tile_bits >= 15 ? 1u : 2u);
+}
+
+// Cut the chunk sequence (tile-major) into one contiguous range per
+// workgroup, each split at tile boundaries into tasks.  A range costs its
+// chunks plus `fill_cost` per task (each task copies its tile into LDS).  The
+// greedy cut at a cost target is bisected to the smallest target that needs
+// at most G ranges.
+namespace {
+struct Cut {
+  std::vector<TiledTask> tasks;
+  std::vector<uint32_t> first;  // first task of each range
+};
+
+Cut greedy_cut(const Tiled& t, double F, double target) {
+  Cut r;
+  double acc = 0;
+  bool open = false;
+  for (uint32_t tile = 0; tile < t.ntiles; ++tile) {
+    uint32_t c = t.tile_chunk0[tile];
+    const uint32_t hi = t.tile_chunk0[tile + 1];
+    while (c < hi) {
+      if (!open || acc + F + 1 > target) {
+        r.first.push_back(static_cast<uint32_t>(r.tasks.size()));
+        acc = 0;
+        open = true;
+      }
+      acc += F;
+      const double room = std::max(1.0, std::floor(target - acc));
+      const uint32_t take = static_cast<uint32_t>(std::min<double>(room, hi - c));
+      r.tasks.push_back({tile, c, c + take});
+      acc += take;
+      c += take;
+      if (acc >= target) open = false;
+    }
+  }
+  return r;
+}
+}  // namespace
+
+void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
+  const uint32_t G = tile_blocks(g);
+  const double F = g.tune.tile_fill_cost;
+  uint32_t busy = 0;
+  for (uint32_t tile = 0; tile < t.ntiles; ++tile) busy += t.tile_chunk0[tile + 1] > t.tile_chunk0[tile];
+  // one range holding everything always fits
+  double hi = static_cast<double>(t.nchunks) + F * busy + F + 1;
+  double lo = std::max(1.0, (static_cast<double>(t.nchunks) + F * busy) / G);
+  Cut best = greedy_cut(t, F, hi);
+  for (int it = 0; it < 48 && hi - lo > 0.5; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    Cut c = greedy_cut(t, F, mid);
+    if (c.first.size() <= G) {
+      hi = mid;
+      best = std::move(c);
+    } else {
+      lo = mid;
+    }
+  }
+  std::vector<uint32_t> wg(G + 1, static_cast<uint32_t>(best.tasks.size()));
+  for (size_t i = 0; i < best.first.size() && i < G; ++i) wg[i] = best.first[i];
+  const std::vector<TiledTask>& tasks = best.tasks;
+  t.ntasks = static_cast<uint32_t>(tasks.size());
+  t.nblocks = G;
+  t.fill_cost = g.tune.tile_fill_cost;
+  t.tasks.alloc(sizeof(TiledTask) * std::max<size_t>(tasks.size(), 1));
+  t.wg_task.alloc(sizeof(uint32_t) * (G + 1));
+  if (!tasks.empty())
+    NWGR_HIP(hipMemcpyAsync(t.tasks.p, tasks.data(), sizeof(TiledTask) * tasks.size(), hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipMemcpyAsync(t.wg_task.p, wg.data(), sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+}
+
+void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s) {
+  if (t.built) return;
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  const uint32_t bits = g.tune.tile_bits;
+  t.bits = bits;
+  t.ntiles = static_cast<uint32_t>((static_cast<uint64_t>(n) + (1u << bits) - 1) >> bits);
+  const uint32_t ntiles = t.ntiles;
+  DevBuf erow, tkey, tval, tile_cnt, tk_sorted, tperm;
+  erow.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  tkey.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  tval.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  tile_cnt.alloc(sizeof(uint32_t) * ntiles);
+  NWGR_HIP(hipMemsetAsync(tile_cnt.p, 0, tile_cnt.bytes, s));
+  if (m) {
+    k_rows_and_tiles<<<grid_for(m), kT, 0, s>>>(offs, idx, n, m, bits, erow.as<uint32_t>(), tkey.as<uint32_t>(),
+                                                tval.as<uint32_t>(), tile_cnt.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  std::vector<uint32_t> cnt(ntiles);
+  NWGR_HIP(hipMemcpyAsync(cnt.data(), tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> start(ntiles + 1, 0);
+  t.tile_chunk0.assign(ntiles + 1, 0);
+  for (uint32_t i = 0; i < ntiles; ++i) {
+    start[i + 1] = start[i] + cnt[i];
+    t.tile_chunk0[i + 1] = t.tile_chunk0[i] + (cnt[i] + kChunkEdges - 1) / kChunkEdges;
+  }
+  t.nchunks = t.tile_chunk0[ntiles];
+  const uint64_t mp = static_cast<uint64_t>(t.nchunks) * kChunkEdges;
+  DevBuf d_start, d_chunk0;
+  d_start.alloc(sizeof(uint32_t) * (ntiles + 1));
+  d_chunk0.alloc(sizeof(uint32_t) * (ntiles + 1));
+  NWGR_HIP(hipMemcpyAsync(d_start.p, start.data(), sizeof(uint32_t) * (ntiles + 1), hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipMemcpyAsync(d_chunk0.p, t.tile_chunk0.data(), sizeof(uint32_t) * (ntiles + 1), hipMemcpyHostToDevice, s));
+
+  // stable sort of the entries by tile keeps (row, source) order inside a tile
+  tk_sorted.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  tperm.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  if (m) {
+    const int end_bit = static_cast<int>(bits_for(ntiles));
+    cub_call([&](void* tmp, size_t& bytes) {
+      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, tkey.as<uint32_t>(), tk_sorted.as<uint32_t>(),
+                                                tval.as<uint32_t>(), tperm.as<uint32_t>(), static_cast<int>(m), 0,
+                                                end_bit, s);
+    }, s);
+  }
+  tkey.reset();
+  tval.reset();
+
+  t.eloc.alloc(sizeof(uint16_t) * std::max<uint64_t>(mp, 1));
+  t.epos.alloc(sizeof(uint32_t) * std::max<uint64_t>(mp, 1));
+  DevBuf flagv, ord;
+  flagv.alloc(sizeof(uint32_t) * std::max<uint64_t>(mp, 1));
+  ord.alloc(sizeof(uint32_t) * std::max<uint64_t>(mp, 1));
+  NWGR_HIP(hipMemsetAsync(t.eloc.p, 0, t.eloc.bytes, s));
+  NWGR_HIP(hipMemsetAsync(flagv.p, 0, flagv.bytes, s));
+  if (mp) {
+    k_fill_u32<<<grid_for(mp), kT, 0, s>>>(t.epos.as<uint32_t>(), mp, static_cast<uint32_t>(m));
+    NWGR_HIP(hipGetLastError());
+  }
+  if (m) {
+    k_tile_place<<<grid_for(m), kT, 0, s>>>(tk_sorted.as<uint32_t>(), tperm.as<uint32_t>(), idx, erow.as<uint32_t>(),
+                                            d_start.as<uint32_t>(), d_chunk0.as<uint32_t>(), m, (1u << bits) - 1,
+                                            t.eloc.as<uint16_t>(), t.epos.as<uint32_t>(), flagv.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  k_pad_flags<<<grid_for(ntiles), kT, 0, s>>>(tile_cnt.as<uint32_t>(), d_chunk0.as<uint32_t>(), ntiles,
+                                              t.eloc.as<uint16_t>(), flagv.as<uint32_t>());
+  NWGR_HIP(hipGetLastError());
+  tk_sorted.reset();
+  tperm.reset();
+
+  // sub-pair ordinals
+  if (mp) {
+    cub_call([&](void* tmp, size_t& bytes) {
+      return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flagv.as<uint32_t>(), ord.as<uint32_t>(),
+                                              static_cast<int>(mp), s);
+    }, s);
+  }
+  uint32_t last[2] = {0, 0};
+  if (mp) {
+    NWGR_HIP(hipMemcpyAsync(&last[0], ord.as<uint32_t>() + mp - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipMemcpyAsync(&last[1], flagv.as<uint32_t>() + mp - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+  }
+  t.nsub = last[0] + last[1];
+  const uint32_t nsub = t.nsub;
+  t.chunk_sp0.alloc(sizeof(uint32_t) * std::max<uint32_t>(t.nchunks, 1));
+  if (t.nchunks) {
+    k_chunk_sp0<<<grid_for(t.nchunks), kT, 0, s>>>(ord.as<uint32_t>(), t.nchunks, t.chunk_sp0.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  DevBuf sprow, rowcnt;
+  sprow.alloc(sizeof(uint32_t) * std::max<uint32_t>(nsub, 1));
+  rowcnt.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  NWGR_HIP(hipMemsetAsync(rowcnt.p, 0, rowcnt.bytes, s));
+  if (mp) {
+    k_subpairs<<<grid_for(mp), kT, 0, s>>>(flagv.as<uint32_t>(), ord.as<uint32_t>(), t.epos.as<uint32_t>(),
+                                           erow.as<uint32_t>(), mp, m, n, sprow.as<uint32_t>(),
+                                           rowcnt.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  flagv.reset();
+  ord.reset();
+  erow.reset();
+  // each row's sub-pair ordinals in the stable row order (tiles ascending)
+  t.pair_ptr.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  cub_call([&](void* tmp, size_t& bytes) {
+    return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, rowcnt.as<uint32_t>(), t.pair_ptr.as<uint32_t>(),
+                                            static_cast<int>(n) + 1, s);
+  }, s);
+  t.rslot.alloc(sizeof(uint32_t) * std::max<uint32_t>(nsub, 1));
+  if (nsub) {
+    DevBuf ids, rows_sorted;
+    ids.alloc(sizeof(uint32_t) * nsub);
+    rows_sorted.alloc(sizeof(uint32_t) * nsub);
+    k_iota<<<grid_for(nsub), kT, 0, s>>>(ids.as<uint32_t>(), nsub);
+    NWGR_HIP(hipGetLastError());
+    const int end_bit = static_cast<int>(bits_for(n));
+    cub_call([&](void* tmp, size_t& bytes) {
+      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, sprow.as<uint32_t>(), rows_sorted.as<uint32_t>(),
+                                                ids.as<uint32_t>(), t.rslot.as<uint32_t>(), static_cast<int>(nsub),
+                                                0, end_bit, s);
+    }, s);
+  }
+  t.partial.alloc(sizeof(float) * std::max<uint32_t>(nsub, 1));
+  NWGR_HIP(hipMemsetAsync(t.partial.p, 0, t.partial.bytes, s));
+  // rows whose partials a wavefront adds
+  std::vector<uint32_t> pp(static_cast<size_t>(n) + 1);
+  NWGR_HIP(hipMemcpyAsync(pp.data(), t.pair_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> wave_rows;
+  for (uint32_t r = 0; r < n; ++r)
+    if (pp[r + 1] - pp[r] > kWaveRowMin) wave_rows.push_back(r);
+  t.nwave_rows = static_cast<uint32_t>(wave_rows.size());
+  t.wave_rows.alloc(sizeof(uint32_t) * std::max<size_t>(wave_rows.size(), 1));
+  if (!wave_rows.empty())
+    NWGR_HIP(hipMemcpyAsync(t.wave_rows.p, wave_rows.data(), sizeof(uint32_t) * wave_rows.size(),
+                            hipMemcpyHostToDevice, s));
+  plan_tasks(g, t, s);
+  t.built = true;
+}
+
+void tiled_weights(Graph& g, Tiled& t, const float* weights, hipStream_t s) {
+  const uint64_t mp = static_cast<uint64_t>(t.nchunks) * kChunkEdges;
+  if (!t.w.p || t.w.bytes < sizeof(float) * std::max<uint64_t>(mp, 1)) t.w.alloc(sizeof(float) * std::max<uint64_t>(mp, 1));
+  if (mp) {
+    k_gather_w<<<grid_for(mp), kT, 0, s>>>(weights, t.epos.as<uint32_t>(), mp, t.w.as<float>());
+    NWGR_HIP(hipGetLastError());
+  }
+  (void)g;
+}
+
+Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
+  if (g.tune.engine == kEngineBinned) return nullptr;
+  const uint32_t* offs = adj ? g.t_offs.as<uint32_t>() : g.offs.as<uint32_t>();
+  const uint32_t* idx = adj ? g.t_idx.as<uint32_t>() : g.idx.as<uint32_t>();
+  Tiled& t = adj ? g.tadj : g.tfwd;
+  if (g.tune.engine == kEngineAuto) {
+    if (g.auto_bits != g.tune.tile_bits) {
+      g.auto_tiled[0] = g.auto_tiled[1] = -1;
+      g.auto_bits = g.tune.tile_bits;
+    }
+    int& choice = g.auto_tiled[adj ? 1 : 0];
+    if (choice < 0) {
+      unsigned long long runs = 0;
+      if (g.m) {
+        DevBuf d;
+        d.alloc(sizeof(unsigned long long));
+        NWGR_HIP(hipMemsetAsync(d.p, 0, sizeof(unsigned long long), s));
+        const uint32_t blocks = std::clamp<uint32_t>((g.n + kT / 64 - 1) / (kT / 64), 1, 8192);
+        k_count_runs<<<blocks, kT, 0, s>>>(offs, idx, g.n, g.tune.tile_bits, d.as<unsigned long long>());
+        NWGR_HIP(hipGetLastError());
+        NWGR_HIP(hipMemcpyAsync(&runs, d.p, sizeof(runs), hipMemcpyDeviceToHost, s));
+        NWGR_HIP(hipStreamSynchronize(s));
+      }
+      choice = (runs == 0 || static_cast<double>(g.m) >= kTiledMinRun * static_cast<double>(runs)) ? 1 : 0;
+    }
+    if (!choice) return nullptr;
+  }
+  build_tiled(g, t, offs, idx, s);
+  if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
+  return &t;
+}
+
+}  // namespace nwgr

@@ -149,6 +149,16 @@ SolveSpec forward_spec(Graph& g, const nwgr_pr_params& p) {
   return sp;
 }
 
+// Engine choice for one orientation; the tiled engine reads the weights in
+// its own order.
+void attach_tiled(Graph& g, SolveSpec& sp, bool adj, const float* weights, hipStream_t s) {
+  Tiled* t = select_tiled(g, adj, s);
+  if (!t) return;
+  tiled_weights(g, *t, weights, s);
+  sp.tiled = t;
+  sp.tw = t->w.as<float>();
+}
+
 SolveSpec adjoint_spec(Graph& g, const nwgr_pr_params& p) {
   SolveSpec sp;
   sp.kind = kAdjoint;
@@ -169,7 +179,8 @@ void run_weighted_forward(Graph& g, const nwgr_pr_params& p, const float* d_w, f
   validate_params(p);
   NWGR_REQUIRE(d_w != nullptr || g.m == 0, "weights are null");
   prepare_weights(g, d_w, s);
-  const SolveSpec sp = forward_spec(g, p);
+  SolveSpec sp = forward_spec(g, p);
+  attach_tiled(g, sp, false, g.w_in.as<float>(), s);
   solve(g, sp, p, res, s);
   const float* fin = g.r[res.iters & 1].as<float>();
   unpermute(g, fin, d_out, 1.0f, s);
@@ -193,7 +204,8 @@ void run_weighted_adjoint(Graph& g, const nwgr_pr_params& p, const float* d_w, c
   NWGR_HIP(hipGetLastError());
   k_gather<<<grid_for(n), kT, 0, s>>>(d_r, g.perm.as<uint32_t>(), n, g.rsave.as<float>());
   NWGR_HIP(hipGetLastError());
-  const SolveSpec sa = adjoint_spec(g, p);
+  SolveSpec sa = adjoint_spec(g, p);
+  attach_tiled(g, sa, true, g.w_out.as<float>(), s);
   solve(g, sa, p, res, s);
   const float* lam = g.r[res.iters & 1].as<float>();
   // gradient coefficients from the converged lambda, then per-edge gradients

@@ -131,6 +131,11 @@ class PageRankGraph:
                                                   ctypes.byref(chunks)))
         return {"sweep_ms": ms.value, "launches": launches.value, "chunks": chunks.value}
 
+    def last_engine(self) -> str | None:
+        """Engine of the last solve: "binned", "tiled", or None before any."""
+        e = pagerank_lib().nwgr_pr_last_engine(self._h)
+        return {0: "binned", 1: "tiled"}.get(e)
+
     def last_errors(self) -> np.ndarray:
         buf = np.zeros(4096, np.float64)
         k = pagerank_lib().nwgr_pr_last_errors(self._h, buf.ctypes.data, len(buf))
