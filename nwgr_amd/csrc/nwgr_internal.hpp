@@ -177,7 +177,7 @@ struct Tuning {
   uint32_t chunk = 16;             // sweeps per captured graph
   uint32_t engine = kEngineAuto;   // kEngineBinned, kEngineTiled or kEngineAuto
   uint32_t tile_bits = 15;         // tiled engine: log2 of the source tile width (13 .. 15)
-  uint32_t tile_fill_cost = 64;    // tiled engine: chunk-equivalents charged per tile fill
+  uint32_t tile_fill_cost = 40;    // tiled engine: chunk-equivalents charged per tile fill
 };
 
 // Work decomposition of one CSR orientation: segments over row positions and
@@ -198,21 +198,27 @@ struct Plan {
 // Source vertices are cut into tiles of T = 2^bits consecutive ids (the
 // degree relabelling puts the hottest sources in the first tiles).  Edges are
 // stored tile-major -- ordered by (tile, row, source) -- as 16-bit local ids
-// whose top bit marks the first edge of a sub-pair: a run of edges of one row
-// inside one tile, additionally cut at every chunk boundary.  A chunk is
-// kChunkEdges consecutive entries (8 per lane of a wavefront); every tile's
-// run is padded to whole chunks, the padding forming one dummy sub-pair.
+// whose top bit marks the first edge of a fragment: a run of edges of one row
+// inside one tile, additionally cut at every group boundary (kGroupEdges
+// consecutive entries: the share of 16 lanes, one DPP row).  A chunk is
+// kChunkEdges consecutive entries (one wavefront's share, kLaneEdges per
+// lane); every tile's run is padded to whole chunks, the padding forming
+// dummy fragments.
 //
 // A sweep has two passes.  The tile pass gives every workgroup a fixed,
 // contiguous range of chunks (cut at tile boundaries into tasks); per task it
-// copies the tile's contributions into LDS, gathers from there, and reduces
-// every sub-pair to one partial sum stored at the sub-pair's ordinal, so the
-// partials of a chunk are written contiguously.  The combine pass adds each
-// row's partials in tile order (`rslot` lists them row by row) and applies the
-// row epilogue.
+// copies the tile's contributions into LDS and gathers from there.  Each lane
+// reduces its entries; runs crossing lanes are joined by a segmented scan
+// over the 16 lanes of a DPP row (no LDS traffic), and every fragment's sum
+// is stored at the fragment's ordinal (`lane_sp0`: ordinal of the fragment
+// holding each lane's first entry), so a chunk's partials are written
+// contiguously.  The combine pass adds each row's partials in tile order
+// (`rslot` lists them row by row) and applies the row epilogue.
 constexpr uint32_t kMaxTileBits = 15;          // local ids use 15 bits of a u16
 constexpr uint32_t kMaxTile = 1u << kMaxTileBits;
-constexpr uint32_t kChunkEdges = 512;
+constexpr uint32_t kLaneEdges = 16;
+constexpr uint32_t kGroupEdges = 16 * kLaneEdges;
+constexpr uint32_t kChunkEdges = 64 * kLaneEdges;
 constexpr uint32_t kWaveRowMin = 32;           // rows with more partials are combined by a wavefront
 
 struct TiledTask {
@@ -224,7 +230,7 @@ struct Tiled {
   bool built = false;
   uint32_t bits = 0;             // tile width 2^bits the layout was built with
   uint32_t ntiles = 0, nchunks = 0;
-  uint32_t nsub = 0;             // sub-pairs including the dummy ones
+  uint32_t nsub = 0;             // fragments including the dummy ones
   uint32_t nblocks = 0;          // workgroups of the tile pass the tasks were planned for
   uint32_t fill_cost = 0;        // tile fill cost the tasks were planned with
   uint32_t ntasks = 0;
@@ -232,10 +238,10 @@ struct Tiled {
   DevBuf eloc;       // u16[nchunks * kChunkEdges]
   DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
   DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)
-  DevBuf chunk_sp0;  // u32[nchunks] ordinal of the chunk's first sub-pair
+  DevBuf lane_sp0;   // u32[nchunks * 64] ordinal of the fragment holding each lane's first entry
   DevBuf pair_ptr;   // u32[n + 1] each row's range in rslot
-  DevBuf rslot;      // u32[nsub] sub-pair ordinals, row by row, tiles ascending
-  DevBuf partial;    // f32[nsub] one partial sum per sub-pair ordinal
+  DevBuf rslot;      // u32[nsub] fragment ordinals, row by row, tiles ascending
+  DevBuf partial;    // f32[nsub] one partial sum per fragment ordinal
   DevBuf tasks;      // TiledTask[ntasks]
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
   DevBuf wave_rows;  // u32[nwave_rows] rows with more than kWaveRowMin partials

@@ -496,15 +496,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Tiled engine.  Per sweep: k_tile reduces every sub-pair (one row's entries
-// inside one source tile) to a partial sum from an LDS copy of the tile's
-// contributions; k_combine adds each row's partials (tiles ascending) and
-// applies the row epilogue.
+// Tiled engine.  Per sweep: k_tile reduces every fragment (a run of one row's
+// entries inside one source tile and one lane's share of a chunk) to a
+// partial sum from an LDS copy of the tile's contributions; k_combine adds
+// each row's partials (tiles ascending) and applies the row epilogue.
 
 struct TileArgs {
   const uint16_t* __restrict__ eloc;
   const float* __restrict__ w;  // weights in tiled order (weighted kinds)
-  const uint32_t* __restrict__ chunk_sp0;
+  const uint32_t* __restrict__ lane_sp0;
   const TiledTask* __restrict__ tasks;
   const uint32_t* __restrict__ wg_task;
   const float* __restrict__ c_in;  // padded to whole tiles
@@ -533,84 +533,165 @@ struct CombineArgs {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// The loads of one chunk for one lane: 8 local ids (and 8 weights) plus the
-// chunk's first sub-pair ordinal.  Issued ahead of use (see k_tile).
+// The loads of one chunk for one lane: its kLaneEdges local ids (and
+// weights) plus the ordinal of its first fragment.  Issued ahead of use (see
+// k_tile).
 struct ChunkIn {
-  u32x4 e;
-  f32x4 w0, w1;
+  u32x4 e0, e1;
+  f32x4 w0, w1, w2, w3;
   uint32_t sp0;
 };
 
 template <bool W>
 __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uint32_t wl) {
   ChunkIn in;
-  const uint64_t base = static_cast<uint64_t>(c) * kChunkEdges + wl * 8;
-  in.e = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base));
+  const uint64_t lane = static_cast<uint64_t>(c) * kWave + wl;
+  const uint64_t base = lane * kLaneEdges;
+  in.e0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base));
+  in.e1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base + 8));
   if constexpr (W) {
     in.w0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base));
     in.w1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 4));
+    in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 8));
+    in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 12));
   }
-  in.sp0 = a.chunk_sp0[c];
+  in.sp0 = __builtin_nontemporal_load(a.lane_sp0 + lane);
   return in;
 }
 
-// One chunk (kChunkEdges entries, 8 per lane): gather from LDS, segmented
-// reduction over the sub-pairs, one partial per sub-pair end, stored at the
-// sub-pair's ordinal (consecutive ordinals across the wavefront).
+// Per-wavefront LDS window through which a chunk's partials are written.
+constexpr uint32_t kStageW = 192;
+
+// Orders one wavefront's LDS accesses around it (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP moves inside rows of 16 lanes; lanes whose source falls outside the
+// row read 0.
+constexpr int kDppRowShr = 0x110, kDppRowShl = 0x100;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xf, 0xf, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// One lane's share of a chunk.  Gather from LDS; the sum of the fragment
+// still open at the lane's end goes through an inclusive segmented scan
+// over the lane's DPP row (groups are cut at row boundaries), which gives
+// every lane the carry of the fragment entering it; then each fragment that
+// ends in this lane is summed in entry order onto its carry and stored at
+// its ordinal.
 template <bool W>
-__device__ __forceinline__ void tile_chunk(const TileArgs& a, const float* __restrict__ s_tile, uint32_t mask,
-                                           const ChunkIn& in, uint32_t wl) {
-  const uint32_t words[4] = {in.e.x, in.e.y, in.e.z, in.e.w};
-  float s[8];
-  uint32_t f = 0;  // bit i: entry i starts a sub-pair
+__device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __restrict__ s_tile, float* stage,
+                                          uint32_t mask, const ChunkIn& in, uint32_t wl) {
+  static_assert(kLaneEdges == 16 && kGroupEdges == 16 * kLaneEdges, "tile_lane: 16 entries, 16-lane rows");
+  const uint32_t words[8] = {in.e0.x, in.e0.y, in.e0.z, in.e0.w, in.e1.x, in.e1.y, in.e1.z, in.e1.w};
+  const float wt[16] = {in.w0.x, in.w0.y, in.w0.z, in.w0.w, in.w1.x, in.w1.y, in.w1.z, in.w1.w,
+                        in.w2.x, in.w2.y, in.w2.z, in.w2.w, in.w3.x, in.w3.y, in.w3.z, in.w3.w};
+  float v[16];
+  uint32_t f = 0;  // bit i: entry i starts a fragment
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < 16; ++i) {
     const uint32_t e = (words[i >> 1] >> ((i & 1) * 16)) & 0xffffu;
     f |= (e >> 15) << i;
-    s[i] = s_tile[e & mask];
+    v[i] = s_tile[e & mask];
   }
   if constexpr (W) {
-    s[0] *= in.w0.x; s[1] *= in.w0.y; s[2] *= in.w0.z; s[3] *= in.w0.w;
-    s[4] *= in.w1.x; s[5] *= in.w1.y; s[6] *= in.w1.z; s[7] *= in.w1.w;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] *= wt[i];
   }
-  // inclusive segmented scan inside the lane
+  // sum of the fragment open at the lane's end, from its start in this lane
+  float tv = 0.f;
 #pragma unroll
-  for (int i = 1; i < 8; ++i)
-    if (!((f >> i) & 1u)) s[i] = s[i - 1] + s[i];
-  // inclusive segmented scan of the lane tails across the wavefront
-  float tv = s[7];
-  uint32_t tf = f != 0;
+  for (int i = 0; i < 16; ++i) tv = (((f >> i) & 1u) ? 0.f : tv) + v[i];
+  uint32_t tf = f != 0u;
+  // inclusive segmented scan over the 16 lanes of the row
+#define NWGR_SCAN_STEP(D)                                 \
+  {                                                       \
+    const float ov = dpp_f32<kDppRowShr | (D)>(tv);       \
+    const uint32_t of = dpp_u32<kDppRowShr | (D)>(tf);    \
+    tv = tf ? tv : tv + ov;                               \
+    tf |= of;                                             \
+  }
+  NWGR_SCAN_STEP(1)
+  NWGR_SCAN_STEP(2)
+  NWGR_SCAN_STEP(4)
+  NWGR_SCAN_STEP(8)
+#undef NWGR_SCAN_STEP
+  const float carry = dpp_f32<kDppRowShr | 1>(tv);  // 0 in the row's first lane
+  const uint32_t next0 = dpp_u32<kDppRowShl | 1>(f & 1u);
+  const uint32_t next_starts = (wl & 15u) == 15u ? 1u : next0;  // groups end at row ends
+  // fragment sums at their last entries
+  float ev[16];
+  uint32_t endm = 0;
+  float acc = carry;
 #pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const float ov = __shfl_up(tv, d);
-    const uint32_t of = __shfl_up(tf, d);
-    if (wl >= static_cast<uint32_t>(d)) {
-      if (!tf) tv = ov + tv;
-      tf |= of;
+  for (int i = 0; i < 16; ++i) {
+    const bool st = (f >> i) & 1u;
+    acc = (st ? 0.f : acc) + v[i];
+    ev[i] = acc;
+    const uint32_t end = i < 15 ? (f >> (i + 1)) & 1u : (next_starts != 0u ? 1u : 0u);
+    endm |= end << i;
+  }
+  // The chunk's fragments have consecutive ordinals [base, base + cnt); they
+  // pass through the wavefront's LDS window kStageW at a time and leave in
+  // coalesced stores.  LDS operations of one wavefront complete in order.
+  const uint32_t base = __builtin_amdgcn_readfirstlane(in.sp0);  // lane 0, entry 0 starts a fragment
+  const uint32_t last = in.sp0 + static_cast<uint32_t>(__popc(f & 0xfffeu));  // ordinal at entry 15
+  const uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last), kWave - 1)) + 1u - base;
+  const uint32_t rel0 = in.sp0 - base;
+  for (uint32_t w0 = 0; w0 < cnt; w0 += kStageW) {
+    uint32_t q = rel0 - w0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i > 0) q += (f >> i) & 1u;
+      if (((endm >> i) & 1u) && q < kStageW) stage[q] = ev[i];
     }
+    wave_sync();
+    const uint32_t nw = min(kStageW, cnt - w0);
+#pragma unroll
+    for (uint32_t j = 0; j < kStageW; j += kWave)
+      if (j + wl < nw) a.partial[base + w0 + j + wl] = stage[j + wl];
+    wave_sync();
   }
-  float carry = __shfl_up(tv, 1);
-  if (wl == 0) carry = 0.f;
-  // entries before the lane's first sub-pair start continue the open sub-pair
+}
+
+// The chunks of one task taken by one wavefront (lo + wv, lo + wv + 16, ...),
+// NB per batch, double-buffered: the loads of the next batch are issued --
+// unconditionally, with clamped chunk ids -- before the current batch is
+// reduced, so the waits for them are exact.
+template <bool W, int NB>
+__device__ __forceinline__ void run_chunks(const TileArgs& a, const float* __restrict__ s_tile, float* stage,
+                                           uint32_t mask, uint32_t first, uint32_t hi, uint32_t wl) {
+  constexpr uint32_t S = kSweepThreads / kWave;
+  if (first >= hi) return;
+  const uint32_t last = hi - 1;
+  ChunkIn A[NB], B[NB];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
-    if ((f & ((2u << i) - 1u)) == 0) s[i] = carry + s[i];
-  // sub-pair ordinals: starts before this lane within the chunk
-  uint32_t run = __popc(f);
+  for (int j = 0; j < NB; ++j) A[j] = load_chunk<W>(a, min(first + j * S, last), wl);
+  for (uint32_t c = first;;) {
 #pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint32_t o = __shfl_up(run, d);
-    if (wl >= static_cast<uint32_t>(d)) run += o;
-  }
-  run -= __popc(f);  // exclusive
-  const uint32_t first_next = __shfl_down(f & 1u, 1);
-  const uint32_t next_lane_starts = wl == kWave - 1 ? 1u : first_next;  // chunks end sub-pairs
-  float* __restrict__ out = a.partial + in.sp0;
+    for (int j = 0; j < NB; ++j) B[j] = load_chunk<W>(a, min(c + (NB + j) * S, last), wl);
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    run += (f >> i) & 1u;
-    const bool end = i < 7 ? ((f >> (i + 1)) & 1u) : next_lane_starts;
-    if (end) out[run - 1] = s[i];
+    for (int j = 0; j < NB; ++j)
+      if (c + j * S < hi) tile_lane<W>(a, s_tile, stage, mask, A[j], wl);
+    c += NB * S;
+    if (c >= hi) break;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) A[j] = load_chunk<W>(a, min(c + (NB + j) * S, last), wl);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (c + j * S < hi) tile_lane<W>(a, s_tile, stage, mask, B[j], wl);
+    c += NB * S;
+    if (c >= hi) break;
   }
 }
 
@@ -629,7 +710,7 @@ template <uint32_t KIND>
 __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   constexpr bool kW = KIND != kPlain;
   constexpr int kNW = kSweepThreads / kWave;
-  extern __shared__ float s_tile[];  // 2^bits floats (dynamic)
+  extern __shared__ float s_tile[];  // 2^bits floats, then kStageW per wavefront (dynamic)
   __shared__ double s_red[2][kNW];
   __shared__ uint32_t s_flag;
   const uint32_t tid = threadIdx.x;
@@ -700,23 +781,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
         fill_tile<kMaxTile / 16 / kSweepThreads>(src, dst, tid);
     }
     __syncthreads();
-    // wave wv takes chunks lo + wv, lo + wv + kNW, ...; the loads of the next
-    // two chunks are in flight while one is reduced
-    const uint32_t hi = task.chunk_hi;
-    uint32_t c = task.chunk_lo + wv;
-    ChunkIn q0{}, q1{}, q2{};
-    if (c < hi) q0 = load_chunk<kW>(a, c, wl);
-    if (c + kNW < hi) q1 = load_chunk<kW>(a, c + kNW, wl);
-    for (; c < hi; c += 3 * kNW) {
-      if (c + 2 * kNW < hi) q2 = load_chunk<kW>(a, c + 2 * kNW, wl);
-      tile_chunk<kW>(a, s_tile, mask, q0, wl);
-      if (c + kNW >= hi) break;
-      if (c + 3 * kNW < hi) q0 = load_chunk<kW>(a, c + 3 * kNW, wl);
-      tile_chunk<kW>(a, s_tile, mask, q1, wl);
-      if (c + 2 * kNW >= hi) break;
-      if (c + 4 * kNW < hi) q1 = load_chunk<kW>(a, c + 4 * kNW, wl);
-      tile_chunk<kW>(a, s_tile, mask, q2, wl);
-    }
+    run_chunks<kW, kW ? 1 : 2>(a, s_tile, s_tile + mask + 1 + wv * kStageW, mask, task.chunk_lo + wv, task.chunk_hi,
+                               wl);
   }
 }
 
@@ -746,7 +812,17 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
     const uint32_t b = pp[row], e = pp[row + 1];
     const RowIn in = load_row<KIND>(a.s, row);
     float z = 0.f;
-    for (uint32_t k = b + wl; k < e; k += kWave) z += a.partial[a.rslot[k]];
+    // four slots per lane in flight per step
+    for (uint32_t k0 = b + wl; k0 < e; k0 += 4 * kWave) {
+      uint32_t sl[4];
+      float v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sl[j] = k0 + j * kWave < e ? a.rslot[k0 + j * kWave] : 0u;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = k0 + j * kWave < e ? a.partial[sl[j]] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) z += v[j];
+    }
     z = wave_sum(z);
     if (wl == 0) finish_row<KIND>(a.s, row, z, Df, in, err, dang);
   }
@@ -891,7 +967,7 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
 // One tiled sweep: k_tile over the workgroup task ranges, then k_combine.
 void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
   static const bool attrs = [] {
-    const int bytes = static_cast<int>(kMaxTile * sizeof(float));
+    const int bytes = static_cast<int>((kMaxTile + kSweepThreads / kWave * kStageW) * sizeof(float));
     for (const void* f : {reinterpret_cast<const void*>(&k_tile<kPlain>),
                           reinterpret_cast<const void*>(&k_tile<kWeighted>),
                           reinterpret_cast<const void*>(&k_tile<kAdjoint>),
@@ -904,7 +980,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   TileArgs ta{};
   ta.eloc = t.eloc.as<uint16_t>();
   ta.w = sp.tw;
-  ta.chunk_sp0 = t.chunk_sp0.as<uint32_t>();
+  ta.lane_sp0 = t.lane_sp0.as<uint32_t>();
   ta.tasks = t.tasks.as<TiledTask>();
   ta.wg_task = t.wg_task.as<uint32_t>();
   ta.c_in = sa.c_in;
@@ -926,7 +1002,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ca.partial = t.partial.as<float>();
   ca.wave_rows = t.wave_rows.as<uint32_t>();
   ca.nwave_rows = t.nwave_rows;
-  const size_t shm = sizeof(float) << t.bits;
+  const size_t shm = sizeof(float) * ((size_t{1} << t.bits) + kSweepThreads / kWave * kStageW);
   const uint32_t nb = t.nblocks;
   switch (sp.kind) {
     case kPlain:

@@ -58,7 +58,7 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
 }
 
 // Place entry k' of the tile-sorted order at its padded position and mark the
-// first entry of every sub-pair.
+// first entry of every fragment.
 __global__ void k_tile_place(const uint32_t* __restrict__ tkey, const uint32_t* __restrict__ tperm,
                              const uint32_t* __restrict__ idx, const uint32_t* __restrict__ erow,
                              const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_chunk0,
@@ -69,27 +69,29 @@ __global__ void k_tile_place(const uint32_t* __restrict__ tkey, const uint32_t* 
     const uint64_t P = static_cast<uint64_t>(tile_chunk0[t]) * kChunkEdges + i;
     const uint32_t e = tperm[k];
     const uint32_t row = erow[e];
-    const bool first = i == 0 || (P % kChunkEdges) == 0 || erow[tperm[k - 1]] != row;
+    const bool first = i == 0 || (P % kGroupEdges) == 0 || erow[tperm[k - 1]] != row;
     eloc[P] = static_cast<uint16_t>((idx[e] & mask) | (first ? 0x8000u : 0u));
     epos[P] = e;
     flagv[P] = first ? 1u : 0u;
   }
 }
 
-// The padding after a tile's entries forms one dummy sub-pair.
+// The padding after a tile's entries forms dummy fragments (cut at group
+// boundaries like the others).
 __global__ void k_pad_flags(const uint32_t* __restrict__ tile_cnt, const uint32_t* __restrict__ tile_chunk0,
                             uint32_t ntiles, uint16_t* eloc, uint32_t* flagv) {
   GRID_STRIDE(t, ntiles) {
-    const uint32_t cnt = tile_cnt[t];
-    if (cnt % kChunkEdges) {
-      const uint64_t P = static_cast<uint64_t>(tile_chunk0[t]) * kChunkEdges + cnt;
-      eloc[P] = 0x8000u;
-      flagv[P] = 1u;
-    }
+    const uint64_t lo = static_cast<uint64_t>(tile_chunk0[t]) * kChunkEdges + tile_cnt[t];
+    const uint64_t hi = static_cast<uint64_t>(tile_chunk0[t + 1]) * kChunkEdges;
+    for (uint64_t P = lo; P < hi; ++P)
+      if (P == lo || P % kGroupEdges == 0) {
+        eloc[P] = 0x8000u;
+        flagv[P] = 1u;
+      }
   }
 }
 
-// Row of every sub-pair (n for the dummy ones) and sub-pairs per row.
+// Row of every fragment (n for the dummy ones) and fragments per row.
 __global__ void k_subpairs(const uint32_t* __restrict__ flagv, const uint32_t* __restrict__ ord,
                            const uint32_t* __restrict__ epos, const uint32_t* __restrict__ erow, uint64_t mp,
                            uint64_t m, uint32_t n, uint32_t* sprow, uint32_t* rowcnt) {
@@ -103,8 +105,11 @@ __global__ void k_subpairs(const uint32_t* __restrict__ flagv, const uint32_t* _
   }
 }
 
-__global__ void k_chunk_sp0(const uint32_t* __restrict__ ord, uint32_t nchunks, uint32_t* chunk_sp0) {
-  GRID_STRIDE(c, nchunks) chunk_sp0[c] = ord[c * static_cast<uint64_t>(kChunkEdges)];
+// Ordinal of the fragment holding a lane's first entry: the exclusive count
+// of starts before it, minus one when that entry continues a fragment.
+__global__ void k_lane_sp0(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ flagv, uint64_t nlanes,
+                           uint32_t* lane_sp0) {
+  GRID_STRIDE(l, nlanes) lane_sp0[l] = ord[l * kLaneEdges] + flagv[l * kLaneEdges] - 1u;
 }
 
 __global__ void k_iota(uint32_t* v, uint64_t count) {
@@ -152,17 +157,21 @@ uint32_t tile_blocks(const Graph& g) {
 }
 
 // Cut the chunk sequence (tile-major) into one contiguous range per
-// workgroup, each split at tile boundaries into tasks.  A range costs its
-// chunks plus `fill_cost` per task (each task copies its tile into LDS).  The
-// greedy cut at a cost target is bisected to the smallest target that needs
-// at most G ranges.
+// workgroup, each split at tile boundaries into tasks.  A chunk costs
+// 1 + fragments / kFragsPerChunkCost (each fragment adds a partial to reduce,
+// stage and store) and every task `fill_cost` more (it copies its tile into
+// LDS and refills the load pipeline).  The greedy cut at a cost target is
+// bisected to the smallest target that needs at most G ranges.
 namespace {
+constexpr double kFragsPerChunkCost = 800.0;
+
 struct Cut {
   std::vector<TiledTask> tasks;
   std::vector<uint32_t> first;  // first task of each range
 };
 
-Cut greedy_cut(const Tiled& t, double F, double target) {
+// pc[c]: cost of chunks [0, c)
+Cut greedy_cut(const Tiled& t, const std::vector<double>& pc, double F, double target) {
   Cut r;
   double acc = 0;
   bool open = false;
@@ -170,36 +179,62 @@ Cut greedy_cut(const Tiled& t, double F, double target) {
     uint32_t c = t.tile_chunk0[tile];
     const uint32_t hi = t.tile_chunk0[tile + 1];
     while (c < hi) {
-      if (!open || acc + F + 1 > target) {
+      if (!open || acc + F + (pc[c + 1] - pc[c]) > target) {
         r.first.push_back(static_cast<uint32_t>(r.tasks.size()));
         acc = 0;
         open = true;
       }
       acc += F;
-      const double room = std::max(1.0, std::floor(target - acc));
-      const uint32_t take = static_cast<uint32_t>(std::min<double>(room, hi - c));
-      r.tasks.push_back({tile, c, c + take});
-      acc += take;
-      c += take;
+      // the most chunks from c that fit the remaining budget, at least one
+      const double limit = pc[c] + (target - acc);
+      uint32_t e = static_cast<uint32_t>(std::upper_bound(pc.begin() + c + 1, pc.begin() + hi + 1, limit) -
+                                         pc.begin()) - 1;
+      e = std::max(e, c + 1);
+      r.tasks.push_back({tile, c, e});
+      acc += pc[e] - pc[c];
+      c = e;
       if (acc >= target) open = false;
     }
   }
   return r;
+}
+
+__global__ void k_chunk_frags(const uint32_t* __restrict__ lane_sp0, uint32_t nchunks, uint32_t nsub,
+                              uint32_t* __restrict__ frags) {
+  GRID_STRIDE(c, nchunks) {
+    const uint32_t a = lane_sp0[c * 64];
+    const uint32_t b = c + 1 < nchunks ? lane_sp0[(c + 1) * 64] : nsub;
+    frags[c] = b - a;
+  }
 }
 }  // namespace
 
 void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
   const uint32_t G = tile_blocks(g);
   const double F = g.tune.tile_fill_cost;
+  // chunk costs from the fragments of every chunk (a chunk starts a group,
+  // so its first lane starts a fragment)
+  std::vector<double> pc(static_cast<size_t>(t.nchunks) + 1, 0.0);
+  if (t.nchunks) {
+    std::vector<uint32_t> fr(t.nchunks);
+    DevBuf d;
+    d.alloc(sizeof(uint32_t) * t.nchunks);
+    k_chunk_frags<<<grid_for(t.nchunks), kT, 0, s>>>(t.lane_sp0.as<uint32_t>(), t.nchunks, t.nsub,
+                                                      d.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    NWGR_HIP(hipMemcpyAsync(fr.data(), d.p, sizeof(uint32_t) * t.nchunks, hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+    for (uint32_t c = 0; c < t.nchunks; ++c) pc[c + 1] = pc[c] + 1.0 + fr[c] / kFragsPerChunkCost;
+  }
   uint32_t busy = 0;
   for (uint32_t tile = 0; tile < t.ntiles; ++tile) busy += t.tile_chunk0[tile + 1] > t.tile_chunk0[tile];
   // one range holding everything always fits
-  double hi = static_cast<double>(t.nchunks) + F * busy + F + 1;
-  double lo = std::max(1.0, (static_cast<double>(t.nchunks) + F * busy) / G);
-  Cut best = greedy_cut(t, F, hi);
-  for (int it = 0; it < 48 && hi - lo > 0.5; ++it) {
+  double hi = pc.back() + F * busy + F + 2 * (1.0 + kChunkEdges / kFragsPerChunkCost);
+  double lo = std::max(1.0, (pc.back() + F * busy) / G);
+  Cut best = greedy_cut(t, pc, F, hi);
+  for (int it = 0; it < 48 && hi - lo > 0.25; ++it) {
     const double mid = 0.5 * (lo + hi);
-    Cut c = greedy_cut(t, F, mid);
+    Cut c = greedy_cut(t, pc, F, mid);
     if (c.first.size() <= G) {
       hi = mid;
       best = std::move(c);
@@ -294,7 +329,7 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   tk_sorted.reset();
   tperm.reset();
 
-  // sub-pair ordinals
+  // fragment ordinals
   if (mp) {
     cub_call([&](void* tmp, size_t& bytes) {
       return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flagv.as<uint32_t>(), ord.as<uint32_t>(),
@@ -309,9 +344,11 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   }
   t.nsub = last[0] + last[1];
   const uint32_t nsub = t.nsub;
-  t.chunk_sp0.alloc(sizeof(uint32_t) * std::max<uint32_t>(t.nchunks, 1));
-  if (t.nchunks) {
-    k_chunk_sp0<<<grid_for(t.nchunks), kT, 0, s>>>(ord.as<uint32_t>(), t.nchunks, t.chunk_sp0.as<uint32_t>());
+  const uint64_t nlanes = mp / kLaneEdges;
+  t.lane_sp0.alloc(sizeof(uint32_t) * std::max<uint64_t>(nlanes, 1));
+  if (nlanes) {
+    k_lane_sp0<<<grid_for(nlanes), kT, 0, s>>>(ord.as<uint32_t>(), flagv.as<uint32_t>(), nlanes,
+                                               t.lane_sp0.as<uint32_t>());
     NWGR_HIP(hipGetLastError());
   }
   DevBuf sprow, rowcnt;
@@ -327,7 +364,7 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   flagv.reset();
   ord.reset();
   erow.reset();
-  // each row's sub-pair ordinals in the stable row order (tiles ascending)
+  // each row's fragment ordinals in the stable row order (tiles ascending)
   t.pair_ptr.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
   cub_call([&](void* tmp, size_t& bytes) {
     return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, rowcnt.as<uint32_t>(), t.pair_ptr.as<uint32_t>(),
