@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
+    ap.add_argument("--workload", choices=["pagerank", "ppr", "wpr"], default="pagerank",
+                    help="pagerank: the headline solve; ppr: 256 personalized columns on R-MAT s20; "
+                         "wpr: weighted forward + adjoint on R-MAT s20")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) and exit: the known-size "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -98,10 +101,58 @@ def cpu_baseline(g, deg, target_s, threads):
             "seconds_per_sweep": dt / it}
 
 
+def bench_ppr(args):
+    """256 personalized columns (one-hot seeds) on R-MAT s20 ef16 in one
+    batched solve; algorithmic bytes per sweep 4M + 4kM + 8kN + 8N."""
+    import torch
+    import nwgr_amd
+    from nwgr_amd import PageRankGraph
+    scale = 20 if args.scale == 22 else args.scale
+    k = 256
+    g = nwgr_amd.rmat(scale, args.edge_factor, seed=args.seed)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    seeds = np.random.default_rng(args.seed).integers(0, g.n, k).astype(np.int32)
+    sd = torch.from_numpy(seeds).to("cuda:0")
+    R = torch.empty((g.n, k), dtype=torch.float32, device="cuda:0")
+    kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform")
+    for _ in range(args.warmup):
+        pg.ppr_batch_device(sd.data_ptr(), k, R.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweeps, dev_ms = 0, 0.0
+    for _ in range(args.steps):
+        res = pg.ppr_batch_device(sd.data_ptr(), k, R.data_ptr(), **kw)
+        sweeps += res.iters
+        dev_ms += res.ms
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n, m = g.n, g.m
+    b_iter = 4 * m + 4 * k * m + 8 * k * n + 8 * n
+    achieved = b_iter * sweeps / (dev_ms / 1e3) / 1e9
+    line = {
+        "metric": "batched personalized PageRank, edge-columns per second", "value": round(m * k * sweeps / wall / 1e9, 3),
+        "unit": "G edge-columns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"ppr rmat s{scale} ef{args.edge_factor}, {k} one-hot seeds, dangling uniform, "
+                               f"tol {args.tol:g} per column", "n": n, "m": m, "k": k,
+                   "sweeps_per_solve": res.iters, "device_ms_per_solve": round(dev_ms / args.steps, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_ppr_sweep+k_ppr_hub+k_ppr_reduce", "algorithmic_bytes_per_sweep": b_iter},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "ppr":
+        bench_ppr(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "

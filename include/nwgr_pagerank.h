@@ -191,6 +191,25 @@ uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap)
  * (page_rank.hpp:123,142).  Any output pointer may be NULL. */
 int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* launches, uint32_t* chunks);
 
+/* Batched personalized PageRank: k (1 .. 256) columns solved together, column
+ * j with the one-hot teleport at vertex seeds[j] (caller ids; duplicates
+ * allowed).  Per column: the iteration of the plain solve with teleport =
+ * e_seed -- init 1/n, params->dangling LEAK, UNIFORM or TELEPORT (to the
+ * seed); params->teleport, init and chunk are not used.  All columns sweep
+ * together until every column's L1 change is below params->tol (or
+ * max_iters); result->error is the largest column change of the last sweep.
+ * R_out is n x k row-major (R_out[v * k + j]) in caller vertex order.  The
+ * _device form takes device pointers and a hipStream_t (NULL = the handle's
+ * stream).  No reference counterpart: the reference solves one teleport
+ * vector per call (page_rank.hpp:89-148). */
+int nwgr_ppr_batch_f32(nwgr_pr_graph* g, const nwgr_pr_params* params, uint32_t k, const uint32_t* seeds,
+                       float* R_out, nwgr_pr_result* result);
+int nwgr_ppr_batch_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* params, uint32_t k, const uint32_t* d_seeds,
+                              float* d_R_out, nwgr_pr_result* result, void* stream);
+/* Per-column L1 change of the last sweep of the last batched solve (up to
+ * `cap` entries); returns the number written. */
+uint32_t nwgr_ppr_last_column_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap);
+
 /* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED or NWGR_ENGINE_TILED
  * (-1 before the first solve or for a null handle). */
 int nwgr_pr_last_engine(const nwgr_pr_graph* g);

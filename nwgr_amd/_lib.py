@@ -108,6 +108,12 @@ def _declare_pagerank(L):
     L.nwgr_pr_last_profile.argtypes = [vp, P(f64), P(u32), P(u32)]
     L.nwgr_pr_last_engine.restype = i32
     L.nwgr_pr_last_engine.argtypes = [vp]
+    L.nwgr_ppr_batch_f32.restype = i32
+    L.nwgr_ppr_batch_f32.argtypes = [vp, P(PrParams), u32, vp, vp, P(PrResult)]
+    L.nwgr_ppr_batch_device_f32.restype = i32
+    L.nwgr_ppr_batch_device_f32.argtypes = [vp, P(PrParams), u32, vp, vp, P(PrResult), vp]
+    L.nwgr_ppr_last_column_errors.restype = u32
+    L.nwgr_ppr_last_column_errors.argtypes = [vp, vp, u32]
     L.nwgr_wpr_forward_device_f32.restype = i32
     L.nwgr_wpr_forward_device_f32.argtypes = [vp, P(PrParams), vp, vp, P(PrResult), vp]
     L.nwgr_wpr_adjoint_device_f32.restype = i32

@@ -272,6 +272,12 @@ struct SolveSpec {
   const float* tw = nullptr;       // weights in the tiled order (weighted kinds, tiled engine)
 };
 
+// Device state of the batched personalized solve (ppr.hip).
+struct PprState;
+struct PprDeleter {
+  void operator()(PprState* p) const;
+};
+
 struct Graph {
   int device = 0;
   uint32_t n = 0;
@@ -331,6 +337,10 @@ struct Graph {
   std::vector<double> last_errs;
   SolveProfile last_profile;
   int last_engine = -1;
+
+  // batched personalized solve (built on first use)
+  std::unique_ptr<PprState, PprDeleter> ppr;
+  uint32_t ppr_cols = 0;
 
   void drop_graphs(int kind = -1) {
     for (int k = 0; k < kNumKinds; ++k) {
@@ -394,6 +404,10 @@ void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_res
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
 // one gradient-coefficient pass over the transpose (kind kCoef)
 void run_coef_pass(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, const float* lam, hipStream_t s);
+// k personalized columns with one-hot teleports at d_seeds (caller ids);
+// d_out is n x k row-major in caller order (ppr.hip)
+void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_seeds, float* d_out,
+             nwgr_pr_result& res, hipStream_t s);
 // out[v] = src[iperm[v]] * scale (new labels -> caller order)
 void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s);
 void validate_params(const nwgr_pr_params& p);

@@ -193,6 +193,37 @@ class PageRankGraph:
         return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
 
+    def ppr_batch(self, seeds, damping=0.85, tol=1e-6, max_iters=100, dangling="leak", out=None):
+        """Personalized PageRank for each seed vertex (one-hot teleports), all
+        columns solved together; returns (R float32[n, k], RunResult)."""
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        k = int(sd.size)
+        if out is None:
+            out = np.empty((self.n, k), np.float32)
+        elif out.dtype != np.float32 or out.shape != (self.n, k) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float32 array of shape (n, k)")
+        p = _params(damping, tol, max_iters, dangling, 0, False)
+        res = PrResult()
+        check(pagerank_lib().nwgr_ppr_batch_f32(self._h, ctypes.byref(p), k, sd.ctypes.data, out.ctypes.data,
+                                                ctypes.byref(res)))
+        return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def ppr_batch_device(self, seeds_ptr: int, k: int, out_ptr: int, damping=0.85, tol=1e-6, max_iters=100,
+                         dangling="leak", stream=None) -> RunResult:
+        """Batched personalized solve on device arrays: seeds u32[k], out f32[n, k]."""
+        p = _params(damping, tol, max_iters, dangling, 0, False)
+        res = PrResult()
+        check(pagerank_lib().nwgr_ppr_batch_device_f32(self._h, ctypes.byref(p), k, seeds_ptr, out_ptr,
+                                                       ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def ppr_column_errors(self) -> np.ndarray:
+        """Per-column L1 change of the last sweep of the last batched solve."""
+        buf = np.zeros(256, np.float64)
+        k = pagerank_lib().nwgr_ppr_last_column_errors(self._h, buf.ctypes.data, len(buf))
+        return buf[:k].copy()
+
+
 def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, max_iters=20,
               num_threads=None, *, device: int = 0) -> None:
     """Drop-in for nw::graph::page_rank (page_rank.hpp:89-91).
