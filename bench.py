@@ -55,8 +55,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
-    ap.add_argument("--workload", choices=["pagerank", "ppr", "wpr"], default="pagerank",
+    ap.add_argument("--workload", choices=["pagerank", "ppr", "multi", "wpr"], default="pagerank",
                     help="pagerank: the headline solve; ppr: 256 personalized columns on R-MAT s20; "
+                         "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
                          "wpr: weighted forward + adjoint on R-MAT s20")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) and exit: the known-size "
@@ -146,6 +147,53 @@ def bench_ppr(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_multi(args):
+    """2,048 R-MAT graphs, n log-uniform in [1k, 64k], edge factor 8, every
+    graph solved to its own tolerance in one persistent launch."""
+    import torch
+    import nwgr_amd
+    from nwgr_amd import MultiPageRank
+    rng = np.random.default_rng(args.seed)
+    ns = np.exp(rng.uniform(np.log(1024), np.log(65536), 2048)).astype(np.uint32)
+    g, vbase = nwgr_amd.rmat_many(ns, 8, seed=args.seed)
+    deg = nwgr_amd.build_degrees(g)
+    mp = MultiPageRank(g, vbase, deg)
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform")
+    for _ in range(args.warmup):
+        mp.run_device(out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms, edge_sweeps = 0.0, 0
+    offs = g.offsets.astype(np.int64)
+    m_g = offs[vbase[1:].astype(np.int64)] - offs[vbase[:-1].astype(np.int64)]
+    for _ in range(args.steps):
+        res = mp.run_device(out.data_ptr(), **kw)
+        dev_ms += res.ms
+        iters, _ = mp.graph_results()
+        edge_sweeps += int(np.sum(m_g * iters.astype(np.int64)))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n_g = np.diff(vbase.astype(np.int64))
+    b_sweeps = int(np.sum((8 * m_g + 16 * n_g) * iters.astype(np.int64)))  # last run's sweeps per graph
+    achieved = b_sweeps / (dev_ms / args.steps / 1e3) / 1e9
+    line = {
+        "metric": "many-graph PageRank GTEPS (sum over graphs of edges x sweeps / s)",
+        "value": round(edge_sweeps / wall / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2048 rmat graphs, n log-uniform in [1k, 64k] (sum {g.n}), ef 8 (m {g.m}), "
+                               f"dangling uniform, tol {args.tol:g} per graph, one launch",
+                   "sweeps_min": int(iters.min()), "sweeps_max": int(iters.max()),
+                   "device_ms_per_solve": round(dev_ms / args.steps, 3), "tuning": mp.tuning},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_multi",
+                     "algorithmic_bytes": "sum over graphs of (8 m_g + 16 n_g) x sweeps_g"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -153,6 +201,9 @@ def main():
         return
     if args.workload == "ppr":
         bench_ppr(args)
+        return
+    if args.workload == "multi":
+        bench_multi(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "

@@ -210,6 +210,62 @@ int nwgr_ppr_batch_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* params, ui
  * `cap` entries); returns the number written. */
 uint32_t nwgr_ppr_last_column_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap);
 
+/* ---- in-CSR construction on the device ----------------------------------
+ * From the edge list d_src[i] -> d_dst[i] (device arrays, ids < n): row v of
+ * the result lists the sources of the edges into v in ascending order (the
+ * reference's adjacency<1>, index_adjacency<1> adjacency.hpp:90-97 ->
+ * fill_directed<1> build.hpp:209-265), with duplicates dropped under
+ * NWGR_BUILD_DEDUP (uniq, build.hpp:425-435) and u == v dropped under
+ * NWGR_BUILD_DROP_SELF_LOOPS (remove_self_loops, build.hpp:437-443).
+ * d_offsets: n + 1 entries; d_indices: room for m entries; d_out_degree
+ * (NULL = not wanted): n entries, the out-degrees of the result
+ * (build_degrees, bench/common.hpp:109-120).  *m_out receives the entries
+ * kept.  Ids >= n give NWGR_EINVAL.  m < 2^31. */
+enum { NWGR_BUILD_DEDUP = 1, NWGR_BUILD_DROP_SELF_LOOPS = 2 };
+int nwgr_csr_build_device(const uint32_t* d_src, const uint32_t* d_dst, uint64_t m, uint32_t n, uint32_t flags,
+                          uint32_t* d_offsets, uint32_t* d_indices, uint32_t* d_out_degree, uint64_t* m_out,
+                          void* stream);
+/* The same build followed by nwgr_pr_graph_create_device on its result, with
+ * the out-degrees as divisors. */
+int nwgr_pr_graph_create_from_edges_device(const uint32_t* d_src, const uint32_t* d_dst, uint64_t m, uint32_t n,
+                                           uint32_t flags, int device, void* stream, nwgr_pr_graph** out);
+
+/* ---- many small graphs in one launch ------------------------------------
+ * `count` independent graphs stored block-diagonally in one in-CSR over n
+ * vertices: graph g owns vertices [vbase[g], vbase[g+1]) (vbase has count + 1
+ * entries, vbase[0] = 0, vbase[count] = n, at most 65536 vertices per graph)
+ * and every entry of its rows is one of its own vertices (else NWGR_EINVAL).
+ * `degrees` (NULL = out-degrees) are the divisors, as in nwgr_pr_graph_create.
+ * A run solves every graph on its own -- the plain iteration with n_g in
+ * place of n: init 1/n_g (or params->init), teleport (1-d)/n_g (or the
+ * caller's n-vector params->teleport, one distribution per graph), dangling
+ * LEAK / UNIFORM / TELEPORT, stop on the graph's own L1 change < tol or
+ * max_iters -- all in one persistent launch.  result->iters is the largest
+ * sweep count, result->error the largest final change, result->converged is
+ * set when every graph converged.  No reference counterpart: the reference
+ * solves one graph per call (page_rank.hpp:89-148). */
+typedef struct nwgr_multi_graph nwgr_multi_graph;
+int nwgr_multi_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                      const uint32_t* vbase, uint32_t count, const uint32_t* degrees, int device,
+                      nwgr_multi_graph** out);
+void nwgr_multi_destroy(nwgr_multi_graph* mg);
+/* Host r_out (n floats) and host params->teleport / init. */
+int nwgr_multi_run_f32(nwgr_multi_graph* mg, const nwgr_pr_params* params, float* r_out, nwgr_pr_result* result);
+/* Device r_out, teleport and init; stream NULL = the handle's stream. */
+int nwgr_multi_run_device_f32(nwgr_multi_graph* mg, const nwgr_pr_params* params, float* d_r_out,
+                              nwgr_pr_result* result, void* stream);
+/* Sweeps and final L1 change of every graph of the last run (count entries
+ * each; either pointer may be NULL). */
+int nwgr_multi_graph_results(const nwgr_multi_graph* mg, uint32_t* iters, double* errors);
+/* Decomposition: graphs with at most lds_max_n vertices (<= 36864) gather
+ * from an LDS copy of their contributions; rows with more than heavy_min
+ * entries are reduced by a wavefront each. */
+int nwgr_multi_get_tuning(const nwgr_multi_graph* mg, uint32_t* lds_max_n, uint32_t* heavy_min);
+int nwgr_multi_set_tuning(nwgr_multi_graph* mg, uint32_t lds_max_n, uint32_t heavy_min);
+uint32_t nwgr_multi_n(const nwgr_multi_graph* mg);
+uint32_t nwgr_multi_count(const nwgr_multi_graph* mg);
+double nwgr_multi_setup_ms(const nwgr_multi_graph* mg);
+
 /* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED or NWGR_ENGINE_TILED
  * (-1 before the first solve or for a null handle). */
 int nwgr_pr_last_engine(const nwgr_pr_graph* g);

@@ -9,7 +9,8 @@ the torch autograd bridge.
 from .graph import (CSRGraph, build_adjacency, build_degrees, erdos_renyi, rmat, rmat_many,  # noqa: F401
                     transpose)
 from .page_rank import PageRankGraph, RunResult, device_count, page_rank  # noqa: F401
+from .multi import MultiPageRank  # noqa: F401
 from ._lib import NwgrError  # noqa: F401
 
 __all__ = ["CSRGraph", "build_adjacency", "build_degrees", "rmat", "erdos_renyi", "rmat_many", "transpose",
-           "PageRankGraph", "RunResult", "page_rank", "device_count", "NwgrError"]
+           "PageRankGraph", "MultiPageRank", "RunResult", "page_rank", "device_count", "NwgrError"]

@@ -114,6 +114,30 @@ def _declare_pagerank(L):
     L.nwgr_ppr_batch_device_f32.argtypes = [vp, P(PrParams), u32, vp, vp, P(PrResult), vp]
     L.nwgr_ppr_last_column_errors.restype = u32
     L.nwgr_ppr_last_column_errors.argtypes = [vp, vp, u32]
+    L.nwgr_csr_build_device.restype = i32
+    L.nwgr_csr_build_device.argtypes = [vp, vp, ctypes.c_uint64, u32, u32, vp, vp, vp, P(ctypes.c_uint64), vp]
+    L.nwgr_pr_graph_create_from_edges_device.restype = i32
+    L.nwgr_pr_graph_create_from_edges_device.argtypes = [vp, vp, ctypes.c_uint64, u32, u32, i32, vp, P(vp)]
+    L.nwgr_multi_create.restype = i32
+    L.nwgr_multi_create.argtypes = [vp, vp, u32, ctypes.c_uint64, vp, u32, vp, i32, P(vp)]
+    L.nwgr_multi_destroy.restype = None
+    L.nwgr_multi_destroy.argtypes = [vp]
+    L.nwgr_multi_run_f32.restype = i32
+    L.nwgr_multi_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
+    L.nwgr_multi_run_device_f32.restype = i32
+    L.nwgr_multi_run_device_f32.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
+    L.nwgr_multi_graph_results.restype = i32
+    L.nwgr_multi_graph_results.argtypes = [vp, vp, vp]
+    L.nwgr_multi_get_tuning.restype = i32
+    L.nwgr_multi_get_tuning.argtypes = [vp, P(u32), P(u32)]
+    L.nwgr_multi_set_tuning.restype = i32
+    L.nwgr_multi_set_tuning.argtypes = [vp, u32, u32]
+    L.nwgr_multi_n.restype = u32
+    L.nwgr_multi_n.argtypes = [vp]
+    L.nwgr_multi_count.restype = u32
+    L.nwgr_multi_count.argtypes = [vp]
+    L.nwgr_multi_setup_ms.restype = ctypes.c_double
+    L.nwgr_multi_setup_ms.argtypes = [vp]
     L.nwgr_wpr_forward_device_f32.restype = i32
     L.nwgr_wpr_forward_device_f32.argtypes = [vp, P(PrParams), vp, vp, P(PrResult), vp]
     L.nwgr_wpr_adjoint_device_f32.restype = i32

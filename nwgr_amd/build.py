@@ -33,7 +33,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["graph.hip", "pagerank.hip", "weighted.hip", "tiled.hip", "ppr.hip"]
+HIP_SOURCES = ["graph.hip", "pagerank.hip", "weighted.hip", "tiled.hip", "ppr.hip", "multi.hip", "csr.hip"]
 
 PAGERANK_LIB = os.path.join(PKG, "libnwgr_pagerank.so")
 GRAPHGEN_LIB = os.path.join(PKG, "libnwgr_graphgen.so")

@@ -1,0 +1,538 @@
+// Many independent small graphs solved in one persistent launch.
+//
+// The graphs are stored block-diagonally in one in-CSR: graph g owns the
+// vertices [vbase[g], vbase[g+1]) and only edges among them (at most 65536
+// vertices per graph, so entries are kept as 16-bit local ids).  Each
+// workgroup of k_multi takes a graph from a queue (largest first), runs its
+// whole Jacobi iteration (page_rank_v10, experimental/algorithms/
+// page_rank.hpp:450-505, per graph: init 1/n_g, teleport (1-d)/n_g or the
+// caller's vector, dangling LEAK / UNIFORM / TELEPORT, stop on the graph's
+// own L1 change) with workgroup barriers between sweeps, then takes the next
+// graph.  The contributions of a graph with at most `lds_max_n` vertices are
+// mirrored in LDS for the gathers; rows with more than `heavy_min` entries are
+// reduced by a wavefront each, the others by a thread each.  Cross-workgroup
+// communication is the queue counter alone (one RMW atomic per graph).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "nwgr_internal.hpp"
+
+namespace nwgr {
+
+namespace {
+
+constexpr int kMT = 1024;                 // threads per workgroup
+constexpr uint32_t kMultiMaxN = 65536;    // vertices per graph (16-bit local ids)
+constexpr uint32_t kMultiLdsMax = 36864;  // contributions mirrored in LDS at most (144 KiB)
+constexpr uint32_t kDLeak = 0, kDUniform = 1, kDTeleport = 2;
+
+struct MultiArgs {
+  const uint32_t* __restrict__ offs;   // n + 1 (global)
+  const uint16_t* __restrict__ lidx;   // m + padding, local ids
+  const uint32_t* __restrict__ vbase;  // count + 1
+  const float* __restrict__ degf;      // n: divisor of the contribution
+  const uint32_t* __restrict__ order;  // processing order of the graphs
+  const uint32_t* __restrict__ heavy;  // rows (global ids) with more than heavy_min entries, by graph
+  const uint32_t* __restrict__ hbase;  // count + 1
+  const float* __restrict__ tel;       // n or null
+  const float* __restrict__ init;      // n or null
+  float* r[2];
+  float* c[2];
+  float* __restrict__ out;             // n
+  uint32_t* __restrict__ iters_out;    // count
+  double* __restrict__ err_out;        // count
+  uint32_t* queue;
+  uint32_t count, lds_max_n, heavy_min, max_iters, dangling;
+  double tol;
+  float d_hi, d_lo, one_minus_d;
+};
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ float wsumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// (a, b) summed over the workgroup in a fixed order; every thread gets them.
+__device__ __forceinline__ void bsum2(double& a, double& b, double (*s)[kMT / 64]) {
+  a = wsum(a);
+  b = wsum(b);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    s[0][w] = a;
+    s[1][w] = b;
+  }
+  __syncthreads();
+  double ta = 0, tb = 0;
+#pragma unroll
+  for (int i = 0; i < kMT / 64; ++i) {
+    ta += s[0][i];
+    tb += s[1][i];
+  }
+  a = ta;
+  b = tb;
+}
+
+template <bool LDS>
+__device__ __forceinline__ float gather_thread(const MultiArgs& a, const float* __restrict__ cl,
+                                               const float* __restrict__ cg, uint32_t b, uint32_t e) {
+  float z = 0.f;
+  uint32_t k = b;
+  for (; k + 4 <= e; k += 4) {
+    const uint32_t j0 = a.lidx[k], j1 = a.lidx[k + 1], j2 = a.lidx[k + 2], j3 = a.lidx[k + 3];
+    float v0, v1, v2, v3;
+    if constexpr (LDS) {
+      v0 = cl[j0]; v1 = cl[j1]; v2 = cl[j2]; v3 = cl[j3];
+    } else {
+      v0 = cg[j0]; v1 = cg[j1]; v2 = cg[j2]; v3 = cg[j3];
+    }
+    z += v0;
+    z += v1;
+    z += v2;
+    z += v3;
+  }
+  for (; k < e; ++k) z += LDS ? cl[a.lidx[k]] : cg[a.lidx[k]];
+  return z;
+}
+
+template <bool LDS>
+__device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __restrict__ cl,
+                                             const float* __restrict__ cg, uint32_t b, uint32_t e, uint32_t wl) {
+  float z = 0.f;
+  for (uint32_t k = b + wl; k < e; k += 64) z += LDS ? cl[a.lidx[k]] : cg[a.lidx[k]];
+  return wsumf(z);
+}
+
+// One graph, start to finish, by the whole workgroup.
+template <bool LDS>
+__device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*s_red)[kMT / 64]) {
+  const uint32_t tid = threadIdx.x, wl = tid & 63, wv = tid >> 6;
+  const uint32_t v0 = a.vbase[g], ng = a.vbase[g + 1] - v0;
+  const float un = 1.0f / static_cast<float>(ng);
+  const float base_u = static_cast<float>((1.0 - (static_cast<double>(a.d_hi) + a.d_lo)) / ng);
+  // start: r = init or 1/n_g, contributions, dangling mass
+  double D = 0, unused = 0;
+  for (uint32_t v = tid; v < ng; v += kMT) {
+    const float r = a.init ? a.init[v0 + v] : un;
+    const float dg = a.degf[v0 + v];
+    a.r[0][v0 + v] = r;
+    const float c = r / dg;
+    a.c[0][v0 + v] = c;
+    if constexpr (LDS) cl[v] = c;
+    if (dg == 0.f) D += static_cast<double>(r);
+  }
+  bsum2(D, unused, s_red);  // barrier: LDS and the start vectors are complete
+  uint32_t p = 0, it = 0;
+  double E = 0;
+  const uint32_t h0 = a.hbase[g], h1 = a.hbase[g + 1];
+  while (it < a.max_iters) {
+    const float* __restrict__ cg = a.c[p] + v0;
+    const float* __restrict__ rin = a.r[p] + v0;
+    float* __restrict__ rout = a.r[p ^ 1] + v0;
+    float* __restrict__ cout = a.c[p ^ 1] + v0;
+    const float Df = static_cast<float>(D);
+    double err = 0, dang = 0;
+    auto finish = [&](uint32_t v, float z) {
+      const float t = a.tel ? a.tel[v0 + v] : un;
+      const float base = a.tel ? a.one_minus_d * t : base_u;
+      const float u = a.dangling == kDUniform ? un : (a.dangling == kDTeleport ? t : 0.f);
+      const float s = fmaf(Df, u, z);
+      const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
+      const float dg = a.degf[v0 + v];
+      rout[v] = rn;
+      cout[v] = rn / dg;
+      err += fabs(static_cast<double>(rn) - static_cast<double>(rin[v]));
+      if (dg == 0.f) dang += static_cast<double>(rn);
+    };
+    for (uint32_t v = tid; v < ng; v += kMT) {
+      const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
+      if (e - b > a.heavy_min) continue;
+      finish(v, gather_thread<LDS>(a, cl, cg, b, e));
+    }
+    for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
+      const uint32_t v = a.heavy[h] - v0;
+      const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
+      const float z = gather_wave<LDS>(a, cl, cg, b, e, wl);
+      if (wl == 0) finish(v, z);
+    }
+    bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
+    if constexpr (LDS) {
+      for (uint32_t v = tid; v < ng; v += kMT) cl[v] = cout[v];
+      __syncthreads();
+    }
+    p ^= 1;
+    ++it;
+    E = err;
+    D = dang;
+    if (E < a.tol) break;
+  }
+  const float* fin = a.r[p] + v0;
+  for (uint32_t v = tid; v < ng; v += kMT) a.out[v0 + v] = fin[v];
+  if (tid == 0) {
+    a.iters_out[g] = it;
+    a.err_out[g] = E;
+  }
+}
+
+__global__ __launch_bounds__(kMT) void k_multi(MultiArgs a) {
+  extern __shared__ float cl[];
+  __shared__ double s_red[2][kMT / 64];
+  __shared__ uint32_t s_g;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const uint32_t q = __hip_atomic_fetch_add(a.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_g = q < a.count ? a.order[q] : 0xffffffffu;
+    }
+    __syncthreads();
+    const uint32_t g = s_g;
+    __syncthreads();
+    if (g == 0xffffffffu) return;
+    const uint32_t ng = a.vbase[g + 1] - a.vbase[g];
+    if (ng <= a.lds_max_n)
+      solve_graph<true>(a, g, cl, s_red);
+    else
+      solve_graph<false>(a, g, cl, s_red);
+    __syncthreads();
+  }
+}
+
+// graph id of every vertex
+__global__ void k_vertex_graph(const uint32_t* __restrict__ vbase, uint32_t count, uint32_t* __restrict__ gid) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < count; g += gridDim.x * blockDim.x)
+    for (uint32_t v = vbase[g]; v < vbase[g + 1]; ++v) gid[v] = g;
+}
+
+// local ids, and a flag for an entry whose source lies outside its row's graph
+__global__ void k_localize(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ idx,
+                           const uint32_t* __restrict__ vbase, const uint32_t* __restrict__ gid, uint32_t n,
+                           uint16_t* __restrict__ lidx, uint32_t* __restrict__ bad) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t g = gid[v];
+    const uint32_t lo = vbase[g], hi = vbase[g + 1];
+    for (uint32_t k = offs[v]; k < offs[v + 1]; ++k) {
+      const uint32_t u = idx[k];
+      if (u < lo || u >= hi) {
+        atomicOr(bad, 1u);
+        lidx[k] = 0;
+      } else {
+        lidx[k] = static_cast<uint16_t>(u - lo);
+      }
+    }
+  }
+}
+
+__global__ void k_degf_plain(const uint32_t* __restrict__ deg, uint32_t n, float* __restrict__ out) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    out[v] = static_cast<float>(deg[v]);
+}
+
+__global__ void k_out_degree_u32(const uint32_t* __restrict__ idx, uint64_t m, uint32_t* __restrict__ deg) {
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < m;
+       k += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    atomicAdd(&deg[idx[k]], 1u);
+}
+
+inline uint32_t nblk(uint64_t work, int t = 256) {
+  return static_cast<uint32_t>(std::clamp<uint64_t>((work + t - 1) / t, 1, 65535u * 4));
+}
+
+}  // namespace
+
+struct MultiGraph {
+  int device = 0;
+  uint32_t n = 0, count = 0;
+  uint64_t m = 0;
+  uint32_t num_cu = 0;
+  uint32_t lds_max_n = kMultiLdsMax;
+  uint32_t heavy_min = 64;
+  DevBuf offs, lidx, vbase, degf, order, heavy, hbase, r[2], c[2], iters, errs, queue;
+  std::vector<uint32_t> offs_h, vbase_h, iters_h;
+  std::vector<double> errs_h;
+  hipStream_t stream = nullptr;
+  double setup_ms = 0;
+
+  ~MultiGraph() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void multi_plan_heavy(MultiGraph& mg) {
+  std::vector<uint32_t> heavy, hbase(mg.count + 1, 0);
+  for (uint32_t g = 0; g < mg.count; ++g) {
+    hbase[g] = static_cast<uint32_t>(heavy.size());
+    for (uint32_t v = mg.vbase_h[g]; v < mg.vbase_h[g + 1]; ++v)
+      if (mg.offs_h[v + 1] - mg.offs_h[v] > mg.heavy_min) heavy.push_back(v);
+  }
+  hbase[mg.count] = static_cast<uint32_t>(heavy.size());
+  mg.heavy.alloc(sizeof(uint32_t) * std::max<size_t>(heavy.size(), 1));
+  mg.hbase.alloc(sizeof(uint32_t) * (mg.count + 1));
+  if (!heavy.empty())
+    NWGR_HIP(hipMemcpyAsync(mg.heavy.p, heavy.data(), sizeof(uint32_t) * heavy.size(), hipMemcpyHostToDevice,
+                            mg.stream));
+  NWGR_HIP(hipMemcpyAsync(mg.hbase.p, hbase.data(), sizeof(uint32_t) * (mg.count + 1), hipMemcpyHostToDevice,
+                          mg.stream));
+  NWGR_HIP(hipStreamSynchronize(mg.stream));
+}
+
+MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                         const uint32_t* vbase, uint32_t count, const uint32_t* degrees, int device) {
+  NWGR_REQUIRE(offsets && vbase, "null argument");
+  NWGR_REQUIRE(m == 0 || indices, "indices is null");
+  NWGR_REQUIRE(count >= 1, "count must be >= 1");
+  NWGR_REQUIRE(m < 0xffffffffULL, "m must fit the uint32 offsets");
+  NWGR_REQUIRE(offsets[0] == 0 && offsets[n] == m, "offsets must start at 0 and end at m");
+  NWGR_REQUIRE(vbase[0] == 0 && vbase[count] == n, "vbase must start at 0 and end at n");
+  for (uint32_t g = 0; g < count; ++g) {
+    NWGR_REQUIRE(vbase[g + 1] > vbase[g], "every graph needs at least one vertex");
+    NWGR_REQUIRE(vbase[g + 1] - vbase[g] <= kMultiMaxN, "a graph has more than 65536 vertices");
+  }
+  for (uint32_t v = 0; v < n; ++v) NWGR_REQUIRE(offsets[v + 1] >= offsets[v], "offsets must be non-decreasing");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(NWGR_EDEVICE, "no HIP device available");
+  NWGR_REQUIRE(device >= 0 && device < ndev, "device ordinal out of range");
+  DeviceGuard dg(device);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto mg = std::make_unique<MultiGraph>();
+  mg->device = device;
+  mg->n = n;
+  mg->m = m;
+  mg->count = count;
+  hipDeviceProp_t prop;
+  NWGR_HIP(hipGetDeviceProperties(&prop, device));
+  mg->num_cu = static_cast<uint32_t>(prop.multiProcessorCount);
+  NWGR_HIP(hipStreamCreateWithFlags(&mg->stream, hipStreamNonBlocking));
+  hipStream_t s = mg->stream;
+  mg->offs_h.assign(offsets, offsets + n + 1);
+  mg->vbase_h.assign(vbase, vbase + count + 1);
+  mg->offs.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  mg->vbase.alloc(sizeof(uint32_t) * (count + 1));
+  NWGR_HIP(hipMemcpyAsync(mg->offs.p, offsets, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipMemcpyAsync(mg->vbase.p, vbase, sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice, s));
+  DevBuf d_idx, gid, bad;
+  d_idx.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  if (m) NWGR_HIP(hipMemcpyAsync(d_idx.p, indices, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+  mg->lidx.alloc(sizeof(uint16_t) * (m + 64));
+  NWGR_HIP(hipMemsetAsync(mg->lidx.p, 0, mg->lidx.bytes, s));
+  gid.alloc(sizeof(uint32_t) * n);
+  bad.alloc(sizeof(uint32_t));
+  NWGR_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+  k_vertex_graph<<<nblk(count), 256, 0, s>>>(mg->vbase.as<uint32_t>(), count, gid.as<uint32_t>());
+  k_localize<<<nblk(n), 256, 0, s>>>(mg->offs.as<uint32_t>(), d_idx.as<uint32_t>(), mg->vbase.as<uint32_t>(),
+                                     gid.as<uint32_t>(), n, mg->lidx.as<uint16_t>(), bad.as<uint32_t>());
+  NWGR_HIP(hipGetLastError());
+  // divisors: the caller's degrees, or out-degrees
+  mg->degf.alloc(sizeof(float) * n);
+  DevBuf deg;
+  deg.alloc(sizeof(uint32_t) * n);
+  if (degrees) {
+    NWGR_HIP(hipMemcpyAsync(deg.p, degrees, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+  } else {
+    NWGR_HIP(hipMemsetAsync(deg.p, 0, deg.bytes, s));
+    if (m) k_out_degree_u32<<<nblk(m), 256, 0, s>>>(d_idx.as<uint32_t>(), m, deg.as<uint32_t>());
+  }
+  k_degf_plain<<<nblk(n), 256, 0, s>>>(deg.as<uint32_t>(), n, mg->degf.as<float>());
+  NWGR_HIP(hipGetLastError());
+  uint32_t badh = 0;
+  NWGR_HIP(hipMemcpyAsync(&badh, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  NWGR_REQUIRE(badh == 0, "an edge joins two different graphs");
+  // largest graphs first (by entries, then vertices)
+  std::vector<uint32_t> order(count);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const uint64_t mx = offsets[vbase[x + 1]] - offsets[vbase[x]], my = offsets[vbase[y + 1]] - offsets[vbase[y]];
+    return mx != my ? mx > my : (vbase[x + 1] - vbase[x]) > (vbase[y + 1] - vbase[y]);
+  });
+  mg->order.alloc(sizeof(uint32_t) * count);
+  NWGR_HIP(hipMemcpyAsync(mg->order.p, order.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+  for (int b = 0; b < 2; ++b) {
+    mg->r[b].alloc(sizeof(float) * n);
+    mg->c[b].alloc(sizeof(float) * n);
+  }
+  mg->iters.alloc(sizeof(uint32_t) * count);
+  mg->errs.alloc(sizeof(double) * count);
+  mg->queue.alloc(sizeof(uint32_t));
+  multi_plan_heavy(*mg);
+  mg->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return mg.release();
+}
+
+void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
+  validate_params(p);
+  static const bool attrs = [] {
+    NWGR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_multi), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kMultiLdsMax * sizeof(float))));
+    return true;
+  }();
+  (void)attrs;
+  MultiArgs a{};
+  a.offs = mg.offs.as<uint32_t>();
+  a.lidx = mg.lidx.as<uint16_t>();
+  a.vbase = mg.vbase.as<uint32_t>();
+  a.degf = mg.degf.as<float>();
+  a.order = mg.order.as<uint32_t>();
+  a.heavy = mg.heavy.as<uint32_t>();
+  a.hbase = mg.hbase.as<uint32_t>();
+  a.tel = p.teleport;
+  a.init = p.init;
+  for (int b = 0; b < 2; ++b) {
+    a.r[b] = mg.r[b].as<float>();
+    a.c[b] = mg.c[b].as<float>();
+  }
+  a.out = d_out;
+  a.iters_out = mg.iters.as<uint32_t>();
+  a.err_out = mg.errs.as<double>();
+  a.queue = mg.queue.as<uint32_t>();
+  a.count = mg.count;
+  a.lds_max_n = mg.lds_max_n;
+  a.heavy_min = mg.heavy_min;
+  a.max_iters = p.max_iters;
+  a.dangling = p.dangling;
+  a.tol = p.tol;
+  const double d = p.damping;
+  a.d_hi = static_cast<float>(d);
+  a.d_lo = static_cast<float>(d - static_cast<double>(a.d_hi));
+  a.one_minus_d = static_cast<float>(1.0 - d);
+  hipEvent_t e0, e1;
+  NWGR_HIP(hipEventCreate(&e0));
+  NWGR_HIP(hipEventCreate(&e1));
+  NWGR_HIP(hipMemsetAsync(mg.queue.p, 0, sizeof(uint32_t), s));
+  NWGR_HIP(hipEventRecord(e0, s));
+  const size_t shm = sizeof(float) * mg.lds_max_n;
+  k_multi<<<std::min<uint32_t>(mg.num_cu, mg.count), kMT, shm, s>>>(a);
+  const hipError_t le = hipGetLastError();
+  NWGR_HIP(hipEventRecord(e1, s));
+  NWGR_HIP(hipEventSynchronize(e1));
+  NWGR_HIP(le);
+  float ms = 0;
+  NWGR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  mg.iters_h.resize(mg.count);
+  mg.errs_h.resize(mg.count);
+  NWGR_HIP(hipMemcpy(mg.iters_h.data(), mg.iters.p, sizeof(uint32_t) * mg.count, hipMemcpyDeviceToHost));
+  NWGR_HIP(hipMemcpy(mg.errs_h.data(), mg.errs.p, sizeof(double) * mg.count, hipMemcpyDeviceToHost));
+  uint32_t mx = 0;
+  double emax = 0;
+  bool all = true;
+  for (uint32_t g = 0; g < mg.count; ++g) {
+    mx = std::max(mx, mg.iters_h[g]);
+    emax = std::max(emax, mg.errs_h[g]);
+    all = all && mg.iters_h[g] > 0 && mg.errs_h[g] < p.tol;
+  }
+  res.iters = mx;
+  res.error = emax;
+  res.converged = all ? 1u : 0u;
+  res.ms = ms;
+}
+
+}  // namespace
+
+}  // namespace nwgr
+
+struct nwgr_multi_graph : nwgr::MultiGraph {};
+
+extern "C" {
+
+int nwgr_multi_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                      const uint32_t* vbase, uint32_t count, const uint32_t* degrees, int device,
+                      nwgr_multi_graph** out) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    *out = static_cast<nwgr_multi_graph*>(nwgr::multi_create(offsets, indices, n, m, vbase, count, degrees, device));
+  });
+}
+
+void nwgr_multi_destroy(nwgr_multi_graph* mg) {
+  if (!mg) return;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(mg->device);
+  delete static_cast<nwgr::MultiGraph*>(mg);
+  (void)hipSetDevice(prev);
+}
+
+int nwgr_multi_run_device_f32(nwgr_multi_graph* mg, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,
+                              void* stream) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(mg && p && d_r_out && res, "null argument");
+    nwgr::DeviceGuard dg(mg->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : mg->stream;
+    nwgr::multi_run(*mg, *p, d_r_out, *res, s);
+  });
+}
+
+int nwgr_multi_run_f32(nwgr_multi_graph* mg, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(mg && p && r_out && res, "null argument");
+    nwgr::DeviceGuard dg(mg->device);
+    hipStream_t s = mg->stream;
+    nwgr::DevBuf out, tel, ini;
+    out.alloc(sizeof(float) * mg->n);
+    nwgr_pr_params q = *p;
+    if (p->teleport) {
+      tel.alloc(sizeof(float) * mg->n);
+      NWGR_HIP(hipMemcpyAsync(tel.p, p->teleport, sizeof(float) * mg->n, hipMemcpyHostToDevice, s));
+      q.teleport = tel.as<float>();
+    }
+    if (p->init) {
+      ini.alloc(sizeof(float) * mg->n);
+      NWGR_HIP(hipMemcpyAsync(ini.p, p->init, sizeof(float) * mg->n, hipMemcpyHostToDevice, s));
+      q.init = ini.as<float>();
+    }
+    nwgr::multi_run(*mg, q, out.as<float>(), *res, s);
+    NWGR_HIP(hipMemcpy(r_out, out.p, sizeof(float) * mg->n, hipMemcpyDeviceToHost));
+  });
+}
+
+int nwgr_multi_graph_results(const nwgr_multi_graph* mg, uint32_t* iters, double* errors) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(mg != nullptr, "null argument");
+    NWGR_REQUIRE(mg->iters_h.size() == mg->count, "no run yet");
+    if (iters) std::copy(mg->iters_h.begin(), mg->iters_h.end(), iters);
+    if (errors) std::copy(mg->errs_h.begin(), mg->errs_h.end(), errors);
+  });
+}
+
+int nwgr_multi_get_tuning(const nwgr_multi_graph* mg, uint32_t* lds_max_n, uint32_t* heavy_min) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(mg != nullptr, "null argument");
+    if (lds_max_n) *lds_max_n = mg->lds_max_n;
+    if (heavy_min) *heavy_min = mg->heavy_min;
+  });
+}
+
+int nwgr_multi_set_tuning(nwgr_multi_graph* mg, uint32_t lds_max_n, uint32_t heavy_min) {
+  return nwgr::guarded([&] {
+    NWGR_REQUIRE(mg != nullptr, "null argument");
+    NWGR_REQUIRE(lds_max_n <= nwgr::kMultiLdsMax, "lds_max_n must be <= 36864");
+    NWGR_REQUIRE(heavy_min >= 1, "heavy_min must be >= 1");
+    nwgr::DeviceGuard dg(mg->device);
+    mg->lds_max_n = lds_max_n;
+    if (mg->heavy_min != heavy_min) {
+      mg->heavy_min = heavy_min;
+      nwgr::multi_plan_heavy(*mg);
+    }
+  });
+}
+
+uint32_t nwgr_multi_n(const nwgr_multi_graph* mg) { return mg ? mg->n : 0; }
+uint32_t nwgr_multi_count(const nwgr_multi_graph* mg) { return mg ? mg->count : 0; }
+double nwgr_multi_setup_ms(const nwgr_multi_graph* mg) { return mg ? mg->setup_ms : 0.0; }
+
+}  // extern "C"

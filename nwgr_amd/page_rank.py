@@ -237,3 +237,38 @@ def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshol
     with PageRankGraph(graph, degrees, device) as g:
         r, _ = g.run(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak")
     page_rank[:] = r
+
+
+def build_csr_device(src, dst, n: int, dedup: bool = True, drop_self_loops: bool = True, stream=None):
+    """In-CSR of the device edge list src[i] -> dst[i] (torch int32/uint32
+    tensors on the GPU), built on the device (``nwgr_csr_build_device``).
+
+    Returns (offsets, indices, out_degree) as torch int32 tensors holding the
+    uint32 values; indices has exactly the kept entries."""
+    import torch
+    if src.shape != dst.shape or src.dim() != 1:
+        raise ValueError("src and dst must be 1-D and of equal length")
+    m = int(src.numel())
+    dev = src.device
+    src = src.contiguous().to(torch.int32)
+    dst = dst.contiguous().to(torch.int32)
+    offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    idx = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+    deg = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    m_out = ctypes.c_uint64()
+    flags = (1 if dedup else 0) | (2 if drop_self_loops else 0)
+    check(pagerank_lib().nwgr_csr_build_device(src.data_ptr() if m else None, dst.data_ptr() if m else None, m, n,
+                                               flags, offs.data_ptr(), idx.data_ptr(), deg.data_ptr(),
+                                               ctypes.byref(m_out), stream))
+    return offs, idx[:m_out.value], deg[:n]
+
+
+def graph_from_edges_device(src, dst, n: int, dedup: bool = True, drop_self_loops: bool = True, device: int = 0,
+                            stream=None) -> "PageRankGraph":
+    """PageRankGraph built on the device straight from a device edge list."""
+    h = ctypes.c_void_p()
+    m = int(src.numel())
+    flags = (1 if dedup else 0) | (2 if drop_self_loops else 0)
+    check(pagerank_lib().nwgr_pr_graph_create_from_edges_device(
+        src.data_ptr() if m else None, dst.data_ptr() if m else None, m, n, flags, device, stream, ctypes.byref(h)))
+    return PageRankGraph(_handle=h)
