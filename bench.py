@@ -15,8 +15,11 @@ offset, degree, old and new rank per vertex) x sweeps / summed device time of
 the sweep kernels (HIP events around every captured chunk).
 
 cpu_baseline: the OpenMP restatement of the reference's default page_rank
-(v11, in-place Gauss-Seidel) on the same graph, FP32, timed on a bounded
-number of sweeps on the host cores.
+(v11, in-place Gauss-Seidel) on the same graph, FP32, solved to the same
+tolerance and repeated for about --cpu-seconds on the host cores.
+
+fwd_adjoint_vs_fwd: the metric's third part -- weighted PageRank on R-MAT s20
+ef16, device time of forward + adjoint (dL/dw, dL/dt) over forward alone.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -59,6 +62,9 @@ def parse():
                     help="pagerank: the headline solve; ppr: 256 personalized columns on R-MAT s20; "
                          "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
                          "wpr: weighted forward + adjoint on R-MAT s20")
+    ap.add_argument("--fwd-adjoint", type=int, default=1,
+                    help="1: also time the weighted forward and forward + adjoint on R-MAT s20 (the metric's "
+                         "fwd+adjoint vs fwd ratio); 0: skip")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) and exit: the known-size "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -86,20 +92,78 @@ def load_traffic(scale, ef):
     return d.get("hbm_bytes_per_sweep")
 
 
-def cpu_baseline(g, deg, target_s, threads):
+def cpu_baseline(g, deg, target_s, threads, tol, max_iters):
+    """The reference's default page_rank (v11, in-place Gauss-Seidel) as the
+    OpenMP restatement, FP32 as bench/pr.cpp:89, solved to the same tolerance
+    as the GPU solve; whole solves are repeated (as pr.exe's trials) until
+    about target_s seconds have been spent, after one untimed warm-up solve."""
     import oracle  # the CPU restatement: baseline leg only
-    # calibrate on 2 sweeps, then time a bounded number of sweeps
     t0 = time.perf_counter()
-    oracle.gs_par(g.offsets, g.indices, deg, 0.85, 0.0, 2, nthreads=threads, dtype=np.float32)
-    per = (time.perf_counter() - t0) / 2
-    sweeps = int(max(2, min(100, target_s / max(per, 1e-6))))
+    _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads, dtype=np.float32)
+    per_solve = time.perf_counter() - t0
+    trials = int(max(1, min(1000, round(target_s / max(per_solve, 1e-6)))))
+    sweeps = 0
     t0 = time.perf_counter()
-    _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, 0.0, sweeps, nthreads=threads, dtype=np.float32)
+    for _ in range(trials):
+        _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads, dtype=np.float32)
+        sweeps += it
     dt = time.perf_counter() - t0
-    return {"value": g.m * it / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"R-MAT s{int(np.log2(g.n))} ef16 (same graph), {it} in-place Gauss-Seidel sweeps of the "
-                      f"OpenMP restatement of page_rank v11 (FP32, schedule(dynamic,64)), {dt:.2f} s",
-            "seconds_per_sweep": dt / it}
+    return {"value": g.m * sweeps / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"R-MAT s{int(np.log2(g.n))} ef16 (the same graph): {trials} solves to L1 tol {tol:g} "
+                      f"({it} in-place Gauss-Seidel sweeps each, {sweeps} in all) of the OpenMP restatement of "
+                      f"page_rank v11 (FP32 ranks, dangling mass leaked as the reference, schedule(dynamic,64)), "
+                      f"{dt:.2f} s",
+            "ms_per_solve": dt * 1e3 / trials, "sweeps_per_solve": it, "seconds_per_sweep": dt / sweeps}
+
+
+def fwd_adjoint_cost(seed, steps=5, warmup=2):
+    """Weighted PageRank on R-MAT s20 ef16 (weights U(0,1], teleport
+    Dirichlet(1), loss L = sum_v wl_v r_v with wl ~ N(0,1)): device time of
+    the forward solve and of forward + adjoint (dL/dw, dL/dt), HIP events on
+    the stream both run on; median over `steps` trials."""
+    import torch
+    import nwgr_amd
+    from nwgr_amd import PageRankGraph
+    from nwgr_amd.autograd import weighted_pagerank
+    g = nwgr_amd.rmat(20, 16, seed=seed)
+    rng = np.random.default_rng(seed)
+    w = torch.from_numpy((1.0 - rng.random(g.m)).astype(np.float32)).to("cuda:0")
+    t = torch.from_numpy(rng.dirichlet(np.ones(g.n)).astype(np.float32)).to("cuda:0")
+    wl = torch.from_numpy(rng.standard_normal(g.n).astype(np.float32)).to("cuda:0")
+    pg = PageRankGraph(g)
+    kw = dict(tol=1e-7, max_iters=300, adj_tol=1e-7)
+    fwd, both = [], []
+    for i in range(warmup + steps):
+        wd = w.clone().requires_grad_(True)
+        td = t.clone().requires_grad_(True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        r = weighted_pagerank(pg, wd, td, **kw)
+        e1.record()
+        (r * wl).sum().backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            fwd.append(e0.elapsed_time(e1))
+            both.append(e0.elapsed_time(e2))
+    f, b = float(np.median(fwd)), float(np.median(both))
+    return {"workload": f"weighted pagerank rmat s20 ef16 (n {g.n}, m {g.m}), w ~ U(0,1], teleport Dirichlet(1), "
+                        f"L = sum wl_v r_v, forward tol 1e-7, adjoint tol 1e-7 relative, dangling leak",
+            "fwd_ms": round(f, 3), "fwd_adjoint_ms": round(b, 3), "ratio": round(b / f, 3),
+            "fwd_sweeps": int(pg.last_forward.iters) if hasattr(pg, "last_forward") else None,
+            "adjoint_sweeps": int(pg.last_adjoint.iters) if getattr(pg, "last_adjoint", None) else None}
+
+
+def bench_wpr(args):
+    """Forward + adjoint vs forward time of the differentiable weighted form."""
+    c = fwd_adjoint_cost(args.seed, steps=max(args.steps, 1), warmup=args.warmup)
+    line = {
+        "metric": "weighted PageRank forward+adjoint time / forward time", "value": c["ratio"], "unit": "x",
+        "n_gpus": 1, "steps": max(args.steps, 1), "warmup": args.warmup, "ms_per_step": c["fwd_adjoint_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": c, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_ppr(args):
@@ -205,6 +269,9 @@ def main():
     if args.workload == "multi":
         bench_multi(args)
         return
+    if args.workload == "wpr":
+        bench_wpr(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "
               "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
@@ -269,9 +336,13 @@ def main():
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads (~{args.cpu_seconds:.0f}s)")
-        cpu = cpu_baseline(g, deg, args.cpu_seconds, threads)
+        cpu = cpu_baseline(g, deg, args.cpu_seconds, threads, args.tol, args.max_iters)
         cpu["value"] = round(cpu["value"], 4)
-        log(f"cpu baseline {cpu['value']} GTEPS")
+        log(f"cpu baseline {cpu['value']} GTEPS, {cpu['ms_per_solve']:.1f} ms per solve")
+    fa = None
+    if args.fwd_adjoint:
+        fa = fwd_adjoint_cost(args.seed)
+        log(f"fwd+adjoint / fwd = {fa['ratio']} ({fa['fwd_adjoint_ms']} / {fa['fwd_ms']} ms)")
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
@@ -287,6 +358,8 @@ def main():
     }
     if cpu:
         line["gpu_vs_cpu_per_sweep"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
+        line["gpu_vs_cpu_time_to_tolerance"] = round(cpu["ms_per_solve"] / (wall * 1e3 / args.steps), 1)
+    line["fwd_adjoint_vs_fwd"] = fa
     print(json.dumps(line), flush=True)
 
 

@@ -33,6 +33,7 @@ class _WeightedPageRank(torch.autograd.Function):
         ctx.graph, ctx.damping, ctx.dangling = graph, damping, dangling
         ctx.adj_tol, ctx.adj_max_iters, ctx.has_t = adj_tol, adj_max_iters, t is not None
         ctx.forward_result = res
+        graph.last_forward = res
         ctx.save_for_backward(w, tc if tc is not None else w.new_empty(0), r)
         return r
 

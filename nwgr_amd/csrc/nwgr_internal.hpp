@@ -95,6 +95,9 @@ struct DevBuf {
 constexpr int kBlock = 256;      // threads per workgroup of the auxiliary kernels
 constexpr int kWave = 64;        // one work unit is processed by one wavefront
 constexpr int kSweepThreads = 1024;  // sweep workgroup: one per CU, 16 wavefronts
+// Workgroups of a launch are dealt to the XCDs (each with its own L2) round
+// robin: workgroup b runs on XCD b % kXcds.
+constexpr uint32_t kXcds = 8;
 // Contributions of vertices [0, kLdsCache) -- the hottest ones after the
 // degree relabelling -- are staged in LDS by every sweep workgroup (144 KiB
 // of the 160 KiB per CU).
@@ -384,6 +387,7 @@ void replan(Graph& g, hipStream_t s);
 void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s);
 // cut the chunks of `t` into per-workgroup task ranges (tiled.hip)
 void plan_tasks(Graph& g, Tiled& t, hipStream_t s);
+
 // workgroups of the tile pass for the current tile width
 uint32_t tile_blocks(const Graph& g);
 // the tiled layout the engine choice selects for the in-CSR (adj = false) or

@@ -766,7 +766,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
 
   const uint32_t mask = (1u << a.bits) - 1;
   const uint32_t nq = (mask + 1) / 4;  // float4 per tile
-  const uint32_t t_lo = a.wg_task[blockIdx.x], t_hi = a.wg_task[blockIdx.x + 1];
+  // consecutive task ranges (mostly the same tile) go to workgroups of one
+  // XCD, so a tile's contributions are fetched into that XCD's L2 once
+  const uint32_t G = gridDim.x;
+  const uint32_t vb = G % kXcds == 0 ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+  const uint32_t t_lo = a.wg_task[vb], t_hi = a.wg_task[vb + 1];
   for (uint32_t ti = t_lo; ti < t_hi; ++ti) {
     const TiledTask task = a.tasks[ti];
     if (ti > t_lo) __syncthreads();  // every wave is done with the previous tile
@@ -788,6 +792,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
 
 // Partials staged in LDS per workgroup step of kBlock rows.
 constexpr uint32_t kCombineStage = 4096;
+// Consecutive combine steps kept on one XCD.
+constexpr uint32_t kCombineRun = 8;
 
 template <uint32_t KIND>
 __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
@@ -805,9 +811,20 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
   double err = 0, dang = 0;
   const uint32_t wl = tid & (kWave - 1);
   const uint32_t* __restrict__ pp = a.pair_ptr;
-  // rows with many partials: one wavefront each (lane-strided, then a butterfly)
-  const uint32_t nwaves = gridDim.x * (kBlock / kWave);
-  for (uint32_t i = blockIdx.x * (kBlock / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
+  // Neighbouring rows read neighbouring `rslot` and `partial` entries (every
+  // tile's partials advance with the row), so runs of kCombineRun consecutive
+  // steps go to workgroups of one XCD, the runs dealt to the XCDs in turn
+  // (heavy leading rows stay spread over all of them).
+  const uint32_t G = gridDim.x;
+  uint32_t vb = blockIdx.x;
+  if (G % (kXcds * kCombineRun) == 0) {
+    const uint32_t x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
+    vb = ((l / kCombineRun) * kXcds + x) * kCombineRun + l % kCombineRun;
+  }
+  // rows with many partials: one wavefront each (lane-strided, then a
+  // butterfly)
+  const uint32_t nwaves = G * (kBlock / kWave);
+  for (uint32_t i = vb * (kBlock / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
     const uint32_t row = a.wave_rows[i];
     const uint32_t b = pp[row], e = pp[row + 1];
     const RowIn in = load_row<KIND>(a.s, row);
@@ -829,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
   // the other rows: a thread each, kBlock consecutive rows per step; the
   // step's partials are gathered into LDS by the whole workgroup first
   const uint32_t n = a.s.n;
-  for (uint32_t r0 = blockIdx.x * kBlock; r0 < n; r0 += gridDim.x * kBlock) {
+  for (uint32_t r0 = vb * kBlock; r0 < n; r0 += G * kBlock) {
     const uint32_t row = r0 + tid;
     const bool valid = row < n;
     const uint32_t base = pp[r0], end = pp[min(r0 + kBlock, n)];

@@ -35,10 +35,19 @@ uint32_t bits_for(uint64_t v) {
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
 
 // Row of every CSR entry (binary search in offs), tile of every entry, and
-// entries per tile.
-__global__ void k_rows_and_tiles(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ idx, uint32_t n,
-                                 uint64_t m, uint32_t bits, uint32_t* erow, uint32_t* tkey, uint32_t* tval,
-                                 uint32_t* tile_cnt) {
+// entries per tile.  With `lds_hist`, per-tile counts are first gathered in an
+// LDS histogram of ntiles counters per workgroup (the hottest tiles take most
+// entries, so counting straight into global memory serialises on a few
+// addresses).
+__global__ __launch_bounds__(kT) void k_rows_and_tiles(const uint32_t* __restrict__ offs,
+                                                       const uint32_t* __restrict__ idx, uint32_t n, uint64_t m,
+                                                       uint32_t bits, uint32_t ntiles, bool lds_hist, uint32_t* erow,
+                                                       uint32_t* tkey, uint32_t* tval, uint32_t* tile_cnt) {
+  extern __shared__ uint32_t s_hist[];
+  if (lds_hist) {
+    for (uint32_t i = threadIdx.x; i < ntiles; i += kT) s_hist[i] = 0;
+    __syncthreads();
+  }
   GRID_STRIDE(k, m) {
     uint32_t lo = 0, hi = n;  // offs[lo] <= k < offs[hi]
     while (hi - lo > 1) {
@@ -49,7 +58,15 @@ __global__ void k_rows_and_tiles(const uint32_t* __restrict__ offs, const uint32
     const uint32_t t = idx[k] >> bits;
     tkey[k] = t;
     tval[k] = static_cast<uint32_t>(k);
-    atomicAdd(&tile_cnt[t], 1u);
+    if (lds_hist)
+      atomicAdd(&s_hist[t], 1u);
+    else
+      atomicAdd(&tile_cnt[t], 1u);
+  }
+  if (lds_hist) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ntiles; i += kT)
+      if (s_hist[i]) atomicAdd(&tile_cnt[i], s_hist[i]);
   }
 }
 
@@ -271,8 +288,13 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   tile_cnt.alloc(sizeof(uint32_t) * ntiles);
   NWGR_HIP(hipMemsetAsync(tile_cnt.p, 0, tile_cnt.bytes, s));
   if (m) {
-    k_rows_and_tiles<<<grid_for(m), kT, 0, s>>>(offs, idx, n, m, bits, erow.as<uint32_t>(), tkey.as<uint32_t>(),
-                                                tval.as<uint32_t>(), tile_cnt.as<uint32_t>());
+    // an LDS histogram per workgroup for up to 8192 tiles (32 KiB); fewer,
+    // fuller workgroups keep the flush of the histograms small
+    const bool lds_hist = ntiles <= 8192;
+    const uint32_t grid = lds_hist ? std::min<uint32_t>(grid_for(m), g.num_cu * 4) : grid_for(m);
+    k_rows_and_tiles<<<grid, kT, lds_hist ? sizeof(uint32_t) * ntiles : 0, s>>>(
+        offs, idx, n, m, bits, ntiles, lds_hist, erow.as<uint32_t>(), tkey.as<uint32_t>(), tval.as<uint32_t>(),
+        tile_cnt.as<uint32_t>());
     NWGR_HIP(hipGetLastError());
   }
   std::vector<uint32_t> cnt(ntiles);
