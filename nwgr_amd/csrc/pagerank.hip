@@ -559,19 +559,9 @@ __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uin
   return in;
 }
 
-// Per-wavefront LDS window through which a chunk's partials are written.
-constexpr uint32_t kStageW = 192;
-
-// Orders one wavefront's LDS accesses around it (no workgroup barrier).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // DPP moves inside rows of 16 lanes; lanes whose source falls outside the
 // row read 0.
-constexpr int kDppRowShr = 0x110, kDppRowShl = 0x100;
+constexpr int kDppRowShr = 0x110;
 
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
@@ -583,12 +573,72 @@ __device__ __forceinline__ float dpp_f32(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
-// One lane's share of a chunk.  Gather from LDS; the sum of the fragment
-// still open at the lane's end goes through an inclusive segmented scan
-// over the lane's DPP row (groups are cut at row boundaries), which gives
-// every lane the carry of the fragment entering it; then each fragment that
-// ends in this lane is summed in entry order onto its carry and stored at
-// its ordinal.
+// Lane bit of a wave-uniform mask, as a lane predicate held in SGPRs.
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// Last lane of each DPP row: every group ends there.
+constexpr uint64_t kRowLastLanes = 0x8000800080008000ull;
+
+// Per-wavefront LDS window through which a chunk's fragment sums leave in
+// coalesced stores.  With a 2^15-entry tile the 16 windows fill the rest of
+// the 160 KiB of LDS; a chunk with more fragments passes through it in turns.
+constexpr uint32_t kStageW = 496;
+
+// Orders one wavefront's LDS accesses around it (no workgroup barrier).  The
+// LDS unit executes a wavefront's DS instructions in issue order, so only the
+// compiler must be kept from moving accesses across; a memory fence here would
+// also wait for the wavefront's outstanding global loads and stores (one
+// in-order counter on this target), draining the chunk prefetch.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// The per-entry pass of tile_lane over one LDS window: the running fragment
+// sums, and each sum that ends a fragment (started in this lane) written at
+// its window offset `qa` (bytes; fragment ordinal relative to the window,
+// times 4).  WINDOWED: only offsets inside the window are written (the chunk
+// has more fragments than the window holds).
+template <bool WINDOWED>
+__device__ __forceinline__ void fragment_pass(float* stage, uint32_t q0, const float (&v)[16], const bool (&st)[16],
+                                              bool end15, float& tail, bool& any_start, float& pre, bool& fix) {
+  char* const dst = reinterpret_cast<char*>(stage);
+  uint32_t qa = q0;
+  float acc = v[0];
+  pre = v[0];
+  bool seen = st[0];  // a fragment started in this lane at entries 0..i
+  fix = false;        // the fragment running in from the previous lane ends in this lane
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (i > 0) {
+      qa += st[i] ? 4u : 0u;
+      acc = st[i] ? v[i] : acc + v[i];
+      seen = seen || st[i];
+      pre = seen ? pre : acc;
+    }
+    const bool end = i < 15 ? st[i + 1] : end15;
+    if (end && seen && (!WINDOWED || qa < kStageW * 4u)) *reinterpret_cast<float*>(dst + qa) = acc;
+    fix = fix || (end && !seen);
+  }
+  tail = acc;
+  any_start = seen;
+}
+
+// One lane's share of a chunk: 16 entries, gathered from the LDS tile.
+//
+// Every per-entry predicate is a lane mask in SGPRs, so the per-entry work is
+// a handful of VALU instructions: the start flag (bit 15 of the entry) comes
+// from a sign compare; the running fragment sum, the fragment ordinal and the
+// "first fragment still open" tracker are selects on those masks; a
+// fragment's sum is written (at its ordinal) into the wavefront's LDS window
+// under the mask of the entries that end it, and the window leaves in
+// coalesced stores.  A fragment that runs in from the previous lane is
+// completed after a segmented scan over the lane's DPP row (groups are cut at
+// row boundaries): its in-lane sum `pre` plus the carry of the lanes before,
+// written once per lane.
+//
+// Summation order, fixed: entries of a fragment in order within each lane;
+// a fragment spanning lanes adds the lanes' sums left to right.
 template <bool W>
 __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __restrict__ s_tile, float* stage,
                                           uint32_t mask, const ChunkIn& in, uint32_t wl) {
@@ -597,23 +647,38 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __rest
   const float wt[16] = {in.w0.x, in.w0.y, in.w0.z, in.w0.w, in.w1.x, in.w1.y, in.w1.z, in.w1.w,
                         in.w2.x, in.w2.y, in.w2.z, in.w2.w, in.w3.x, in.w3.y, in.w3.z, in.w3.w};
   float v[16];
-  uint32_t f = 0;  // bit i: entry i starts a fragment
+  bool st[16];  // entry i starts a fragment
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const uint32_t e = (words[i >> 1] >> ((i & 1) * 16)) & 0xffffu;
-    f |= (e >> 15) << i;
+    const uint32_t w = words[i >> 1];
+    const uint32_t e = (i & 1) ? (w >> 16) : (w & 0xffffu);
+    st[i] = (i & 1) ? static_cast<int32_t>(w) < 0 : static_cast<int16_t>(w & 0xffffu) < 0;
     v[i] = s_tile[e & mask];
   }
   if constexpr (W) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[i] *= wt[i];
   }
-  // sum of the fragment open at the lane's end, from its start in this lane
-  float tv = 0.f;
+  // the chunk's first ordinal (lane 0, entry 0 starts a fragment) and its
+  // fragment count, from lane 63's first ordinal and its starts
+  const uint32_t base = __builtin_amdgcn_readfirstlane(in.sp0);
+  uint32_t cnt = 1u + static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(in.sp0), kWave - 1)) - base;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) tv = (((f >> i) & 1u) ? 0.f : tv) + v[i];
-  uint32_t tf = f != 0u;
-  // inclusive segmented scan over the 16 lanes of the row
+  for (int i = 1; i < 16; ++i) cnt += static_cast<uint32_t>(__builtin_amdgcn_ballot_w64(st[i]) >> 63);
+  const uint32_t q0 = (in.sp0 - base) * 4u;  // byte offset of the fragment holding entry 0
+  // entry 15 ends a fragment when the next lane's entry 0 starts one, or at a row end
+  const bool end15 = lane_bit((__builtin_amdgcn_ballot_w64(st[0]) >> 1) | kRowLastLanes);
+  float tail, pre, carry = 0.f;
+  bool seen, fix;
+  for (uint32_t w0 = 0; w0 < cnt; w0 += kStageW) {
+    if (cnt <= kStageW)
+      fragment_pass<false>(stage, q0, v, st, end15, tail, seen, pre, fix);
+    else
+      fragment_pass<true>(stage, q0 - w0 * 4u, v, st, end15, tail, seen, pre, fix);
+    if (w0 == 0) {
+      // inclusive segmented scan of the open tail sums over the 16 lanes of the row
+      float tv = tail;
+      uint32_t tf = seen ? 1u : 0u;
 #define NWGR_SCAN_STEP(D)                                 \
   {                                                       \
     const float ov = dpp_f32<kDppRowShr | (D)>(tv);       \
@@ -621,40 +686,15 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __rest
     tv = tf ? tv : tv + ov;                               \
     tf |= of;                                             \
   }
-  NWGR_SCAN_STEP(1)
-  NWGR_SCAN_STEP(2)
-  NWGR_SCAN_STEP(4)
-  NWGR_SCAN_STEP(8)
+      NWGR_SCAN_STEP(1)
+      NWGR_SCAN_STEP(2)
+      NWGR_SCAN_STEP(4)
+      NWGR_SCAN_STEP(8)
 #undef NWGR_SCAN_STEP
-  const float carry = dpp_f32<kDppRowShr | 1>(tv);  // 0 in the row's first lane
-  const uint32_t next0 = dpp_u32<kDppRowShl | 1>(f & 1u);
-  const uint32_t next_starts = (wl & 15u) == 15u ? 1u : next0;  // groups end at row ends
-  // fragment sums at their last entries
-  float ev[16];
-  uint32_t endm = 0;
-  float acc = carry;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const bool st = (f >> i) & 1u;
-    acc = (st ? 0.f : acc) + v[i];
-    ev[i] = acc;
-    const uint32_t end = i < 15 ? (f >> (i + 1)) & 1u : (next_starts != 0u ? 1u : 0u);
-    endm |= end << i;
-  }
-  // The chunk's fragments have consecutive ordinals [base, base + cnt); they
-  // pass through the wavefront's LDS window kStageW at a time and leave in
-  // coalesced stores.  LDS operations of one wavefront complete in order.
-  const uint32_t base = __builtin_amdgcn_readfirstlane(in.sp0);  // lane 0, entry 0 starts a fragment
-  const uint32_t last = in.sp0 + static_cast<uint32_t>(__popc(f & 0xfffeu));  // ordinal at entry 15
-  const uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last), kWave - 1)) + 1u - base;
-  const uint32_t rel0 = in.sp0 - base;
-  for (uint32_t w0 = 0; w0 < cnt; w0 += kStageW) {
-    uint32_t q = rel0 - w0;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      if (i > 0) q += (f >> i) & 1u;
-      if (((endm >> i) & 1u) && q < kStageW) stage[q] = ev[i];
+      carry = dpp_f32<kDppRowShr | 1>(tv);  // 0 in the row's first lane
     }
+    const uint32_t qf = q0 - w0 * 4u;
+    if (fix && qf < kStageW * 4u) stage[qf / 4u] = pre + carry;
     wave_sync();
     const uint32_t nw = min(kStageW, cnt - w0);
 #pragma unroll
