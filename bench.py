@@ -55,11 +55,15 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iters", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--engine", choices=["auto", "binned", "tiled"], default="auto",
+                    help="sweep engine (auto: the library's choice)")
+    ap.add_argument("--tile-bits", type=int, default=0, help="tiled engine source tile width 2^bits (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
-    ap.add_argument("--workload", choices=["pagerank", "ppr", "multi", "wpr"], default="pagerank",
-                    help="pagerank: the headline solve; ppr: 256 personalized columns on R-MAT s20; "
+    ap.add_argument("--workload", choices=["pagerank", "er", "ppr", "multi", "wpr"], default="pagerank",
+                    help="pagerank: the headline solve on R-MAT s22; er: the same solve on Erdos-Renyi "
+                         "G(n = 2^25, m = 16n); ppr: 256 personalized columns on R-MAT s20; "
                          "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
                          "wpr: weighted forward + adjoint on R-MAT s20")
     ap.add_argument("--fwd-adjoint", type=int, default=1,
@@ -280,10 +284,24 @@ def main():
     from nwgr_amd import PageRankGraph
 
     t0 = time.perf_counter()
-    g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
+    if args.workload == "er":
+        n_er = 1 << 25
+        g = nwgr_amd.erdos_renyi(n_er, 16 * n_er, seed=args.seed)
+        gname = f"erdos-renyi G(n=2^25, m=16n drawn with replacement, dedup, no self-loops, seed {args.seed})"
+    else:
+        g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
+        gname = (f"rmat s{args.scale} ef{args.edge_factor} (a=0.57,b=0.19,c=0.19, seed {args.seed}, dedup, "
+                 f"no self-loops)")
     deg = nwgr_amd.build_degrees(g)
-    log(f"generated R-MAT s{args.scale} ef{args.edge_factor}: n={g.n} m={g.m} in {time.perf_counter() - t0:.1f}s")
+    log(f"generated {gname}: n={g.n} m={g.m} in {time.perf_counter() - t0:.1f}s")
     pg = PageRankGraph(g, deg)
+    if args.engine != "auto" or args.tile_bits:
+        tune = {}
+        if args.engine != "auto":
+            tune["engine"] = {"binned": 0, "tiled": 1}[args.engine]
+        if args.tile_bits:
+            tune["tile_bits"] = args.tile_bits
+        pg.tuning = tune
     log(f"device graph: {pg.device_bytes / 2**20:.0f} MiB, setup {pg.setup_ms:.1f} ms")
     import torch
     kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform", chunk=args.chunk)
@@ -324,23 +342,26 @@ def main():
     n, m = g.n, g.m
     b_iter = 8 * m + 16 * n + 4
     achieved = b_iter * sweeps / (sweep_ms / 1e3) / 1e9
-    traffic = load_traffic(args.scale, args.edge_factor)
+    traffic = load_traffic(args.scale, args.edge_factor) if args.workload == "pagerank" else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_tile+k_combine" if engine == "tiled" else "k_sweep", "engine": engine,
             "algorithmic_bytes_per_sweep": b_iter,
-            "avg_launch_us": round(sweep_ms / launches * 1e3, 2), "launches": launches, "sweeps": sweeps}
+            "avg_launch_us": round(sweep_ms / launches * 1e3, 2), "launches": launches, "sweeps": sweeps,
+            "avg_sweep_us": round(sweep_ms / sweeps * 1e3, 2),
+            "launch": "one sweep: k_tile + k_combine (tiled engine) or k_sweep (binned); the stopping launch "
+                      "of each solve is timed with the sweeps"}
     value = m * sweeps / wall / 1e9
 
     cpu = None
-    if args.cpu_seconds > 0:
+    if args.cpu_seconds > 0 and args.workload == "pagerank":
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads (~{args.cpu_seconds:.0f}s)")
         cpu = cpu_baseline(g, deg, args.cpu_seconds, threads, args.tol, args.max_iters)
         cpu["value"] = round(cpu["value"], 4)
         log(f"cpu baseline {cpu['value']} GTEPS, {cpu['ms_per_solve']:.1f} ms per solve")
     fa = None
-    if args.fwd_adjoint:
+    if args.fwd_adjoint and args.workload == "pagerank":
         fa = fwd_adjoint_cost(args.seed)
         log(f"fwd+adjoint / fwd = {fa['ratio']} ({fa['fwd_adjoint_ms']} / {fa['fwd_ms']} ms)")
 
@@ -348,9 +369,8 @@ def main():
         "metric": METRIC, "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"pagerank rmat s{args.scale} ef{args.edge_factor} (a=0.57,b=0.19,c=0.19, seed "
-                               f"{args.seed}, dedup, no self-loops), damping 0.85, dangling uniform, tol "
-                               f"{args.tol:g} L1, max {args.max_iters} sweeps, FP32",
+        "config": {"workload": f"pagerank {gname}, damping 0.85, dangling uniform, tol {args.tol:g} L1, max "
+                               f"{args.max_iters} sweeps, FP32",
                    "n": n, "m": m, "sweeps_per_solve": iters[0], "device_ms_per_solve": round(dev_ms / args.steps, 4),
                    "setup_ms": round(pg.setup_ms, 1), "engine": engine,
                    "host_output_ms_per_solve": round(host_ms, 3)},

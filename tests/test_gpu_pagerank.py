@@ -256,3 +256,18 @@ def test_config2_rmat_s22_full_size():
     r, res, r64, err = parity(g, deg, "uniform", 1e-6, 100, nthreads=16)
     say(f"rmat s22 sweeps={res.iters} L1={err:.3e} ms={res.ms:.2f}")
     assert abs(float(r.sum(dtype=np.float64)) - 1.0) < 1e-5
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_config3_er_2pow25_full_size():
+    """Erdos-Renyi G(n = 2^25, m = 16n drawn with replacement, deduplicated,
+    self-loops dropped), FP32, tol 1e-6, max 100, vs FP64 at the matched
+    sweep; mass conservation under UNIFORM dangling."""
+    say("er 2^25 (generate)")
+    g = nwgr_amd.erdos_renyi(1 << 25, 16 << 25)
+    deg = nwgr_amd.build_degrees(g)
+    say(f"er 2^25 n={g.n} m={g.m} (solve + oracle)")
+    r, res, r64, err = parity(g, deg, "uniform", 1e-6, 100, nthreads=16)
+    say(f"er 2^25 sweeps={res.iters} L1={err:.3e} ms={res.ms:.2f}")
+    assert abs(float(r.sum(dtype=np.float64)) - 1.0) < 1e-5
