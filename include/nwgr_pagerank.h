@@ -172,7 +172,8 @@ typedef struct nwgr_pr_tuning {
 /* Sweep engines.  BINNED: pull gathers over degree-binned rows, the hottest
  * contributions staged in LDS.  TILED: source tiles staged in LDS, per-tile
  * partial sums added by a second pass.  AUTO (default): TILED for a CSR whose
- * entries average at least two per (row, source tile) run, else BINNED. */
+ * entries average at least two per (row, source tile) run or with more than
+ * 2^21 vertices, else BINNED. */
 enum { NWGR_ENGINE_BINNED = 0, NWGR_ENGINE_TILED = 1, NWGR_ENGINE_AUTO = 2 };
 
 int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t);

@@ -166,9 +166,14 @@ struct SolveProfile {
 };
 
 // Engine choice.  Auto takes the tiled engine for a CSR whose entries form
-// on average at least kTiledMinRun entries per (row, source tile) run.
+// on average at least kTiledMinRun entries per (row, source tile) run ...
 constexpr uint32_t kEngineBinned = 0, kEngineTiled = 1, kEngineAuto = 2;
 constexpr double kTiledMinRun = 2.0;
+// ... or for any CSR whose gathered vector outgrows one XCD's 4 MiB L2: the
+// degree-binned gathers then come from the MALL a cache line per entry, while
+// the tiled engine streams (measured on G(2^25, 16 * 2^25): 5.0 ms per sweep
+// tiled against 8.6 ms binned).
+constexpr uint32_t kTiledMinN = 1u << 21;
 
 struct Tuning {
   uint32_t lds_cache = kLdsCache;  // hot vertices staged in LDS per sweep workgroup
@@ -238,6 +243,7 @@ struct Tiled {
   uint32_t fill_cost = 0;        // tile fill cost the tasks were planned with
   uint32_t ntasks = 0;
   uint32_t nwave_rows = 0;
+  uint32_t row0 = 0;             // rows [0, row0) all have more than kWaveRowMin partials
   DevBuf eloc;       // u16[nchunks * kChunkEdges]
   DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
   DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)

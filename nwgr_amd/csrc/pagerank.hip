@@ -528,6 +528,7 @@ struct CombineArgs {
   const float* __restrict__ partial;
   const uint32_t* __restrict__ wave_rows;
   uint32_t nwave_rows;
+  uint32_t row0;  // rows [0, row0) all have more than kWaveRowMin partials (a wave each)
 };
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -883,22 +884,31 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
     z = wave_sum(z);
     if (wl == 0) finish_row<KIND>(a.s, row, z, Df, in, err, dang);
   }
-  // the other rows: a thread each, kBlock consecutive rows per step; the
-  // step's partials are gathered into LDS by the whole workgroup first
+  // The other rows, from the first row after the leading run of wave rows: a
+  // thread each, in windows of kBlock consecutive rows.  A window is cut into
+  // steps whose partials fit the LDS stage (the longest prefix of its rows
+  // that fits, found with one counting barrier); each step's partials are
+  // gathered into LDS by the whole workgroup, with four loads in flight per
+  // thread, then every thread adds its row's partials in tile order.
   const uint32_t n = a.s.n;
-  for (uint32_t r0 = vb * kBlock; r0 < n; r0 += G * kBlock) {
-    const uint32_t row = r0 + tid;
-    const bool valid = row < n;
-    const uint32_t base = pp[r0], end = pp[min(r0 + kBlock, n)];
-    uint32_t b = 0, e = 0;
-    RowIn in{};
-    if (valid) {
-      b = pp[row];
-      e = pp[row + 1];
-      in = load_row<KIND>(a.s, row);
-    }
-    const bool staged = end - base <= kCombineStage;
-    if (staged) {
+  for (uint32_t w0 = a.row0 + vb * kBlock; w0 < n; w0 += G * kBlock) {
+    const uint32_t w1 = min(w0 + kBlock, n);
+    for (uint32_t r0 = w0; r0 < w1;) {
+      const uint32_t row = r0 + tid;
+      const bool valid = row < w1;
+      const uint32_t base = pp[r0];
+      uint32_t b = 0, e = 0;
+      RowIn in{};
+      if (valid) {
+        b = pp[row];
+        e = pp[row + 1];
+        in = load_row<KIND>(a.s, row);
+      }
+      // rows r0 .. r0 + nfit - 1 fit the stage (a prefix: pp is monotone)
+      uint32_t nfit = static_cast<uint32_t>(__syncthreads_count(valid && e - base <= kCombineStage));
+      const bool staged = nfit > 0;  // else row r0 alone overflows: a wave row, skipped here
+      nfit = max(nfit, 1u);
+      const uint32_t end = staged ? pp[r0 + nfit] : base;
       for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kBlock) {
         uint32_t sl[4];
         float v[4];
@@ -913,18 +923,15 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
         for (int j = 0; j < 4; ++j)
           if (k0 + j * kBlock < end) s_buf[k0 + j * kBlock - base] = v[j];
       }
-    }
-    __syncthreads();
-    if (valid && e - b <= kWaveRowMin) {
-      float z = 0.f;
-      if (staged) {
+      __syncthreads();
+      if (staged && tid < nfit && e - b <= kWaveRowMin) {
+        float z = 0.f;
         for (uint32_t k = b; k < e; ++k) z += s_buf[k - base];
-      } else {
-        for (uint32_t k = b; k < e; ++k) z += a.partial[a.rslot[k]];
+        finish_row<KIND>(a.s, row, z, Df, in, err, dang);
       }
-      finish_row<KIND>(a.s, row, z, Df, in, err, dang);
+      __syncthreads();
+      r0 += nfit;
     }
-    __syncthreads();
   }
   if constexpr (KIND != kCoef) {
     block_sum2(err, dang, s_red);
@@ -1059,6 +1066,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ca.partial = t.partial.as<float>();
   ca.wave_rows = t.wave_rows.as<uint32_t>();
   ca.nwave_rows = t.nwave_rows;
+  ca.row0 = t.row0;
   const size_t shm = sizeof(float) * ((size_t{1} << t.bits) + kSweepThreads / kWave * kStageW);
   const uint32_t nb = t.nblocks;
   switch (sp.kind) {

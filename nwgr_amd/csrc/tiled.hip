@@ -415,6 +415,8 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   std::vector<uint32_t> wave_rows;
   for (uint32_t r = 0; r < n; ++r)
     if (pp[r + 1] - pp[r] > kWaveRowMin) wave_rows.push_back(r);
+  t.row0 = 0;
+  while (t.row0 < wave_rows.size() && wave_rows[t.row0] == t.row0) ++t.row0;
   t.nwave_rows = static_cast<uint32_t>(wave_rows.size());
   t.wave_rows.alloc(sizeof(uint32_t) * std::max<size_t>(wave_rows.size(), 1));
   if (!wave_rows.empty())
@@ -457,7 +459,8 @@ Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
         NWGR_HIP(hipMemcpyAsync(&runs, d.p, sizeof(runs), hipMemcpyDeviceToHost, s));
         NWGR_HIP(hipStreamSynchronize(s));
       }
-      choice = (runs == 0 || static_cast<double>(g.m) >= kTiledMinRun * static_cast<double>(runs)) ? 1 : 0;
+      choice = (runs == 0 || static_cast<double>(g.m) >= kTiledMinRun * static_cast<double>(runs) ||
+                g.n > kTiledMinN) ? 1 : 0;
     }
     if (!choice) return nullptr;
   }
