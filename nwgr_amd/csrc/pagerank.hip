@@ -596,17 +596,17 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // The per-entry pass of tile_lane over one LDS window: the running fragment
-// sums, and each sum that ends a fragment (started in this lane) written at
-// its window offset `qa` (bytes; fragment ordinal relative to the window,
-// times 4).  WINDOWED: only offsets inside the window are written (the chunk
-// has more fragments than the window holds).
+// sums, and each sum that ends a fragment written at its window offset `qa`
+// (bytes; fragment ordinal relative to the window, times 4) -- for the
+// fragment running in from the previous lane, only its in-lane part.
+// WINDOWED: only offsets inside the window are written (the chunk has more
+// fragments than the window holds).
 template <bool WINDOWED>
 __device__ __forceinline__ void fragment_pass(float* stage, uint32_t q0, const float (&v)[16], const bool (&st)[16],
-                                              bool end15, float& tail, bool& any_start, float& pre, bool& fix) {
+                                              bool end15, float& tail, bool& any_start, bool& fix) {
   char* const dst = reinterpret_cast<char*>(stage);
   uint32_t qa = q0;
   float acc = v[0];
-  pre = v[0];
   bool seen = st[0];  // a fragment started in this lane at entries 0..i
   fix = false;        // the fragment running in from the previous lane ends in this lane
 #pragma unroll
@@ -615,10 +615,9 @@ __device__ __forceinline__ void fragment_pass(float* stage, uint32_t q0, const f
       qa += st[i] ? 4u : 0u;
       acc = st[i] ? v[i] : acc + v[i];
       seen = seen || st[i];
-      pre = seen ? pre : acc;
     }
     const bool end = i < 15 ? st[i + 1] : end15;
-    if (end && seen && (!WINDOWED || qa < kStageW * 4u)) *reinterpret_cast<float*>(dst + qa) = acc;
+    if (end && (!WINDOWED || qa < kStageW * 4u)) *reinterpret_cast<float*>(dst + qa) = acc;
     fix = fix || (end && !seen);
   }
   tail = acc;
@@ -635,8 +634,8 @@ __device__ __forceinline__ void fragment_pass(float* stage, uint32_t q0, const f
 // under the mask of the entries that end it, and the window leaves in
 // coalesced stores.  A fragment that runs in from the previous lane is
 // completed after a segmented scan over the lane's DPP row (groups are cut at
-// row boundaries): its in-lane sum `pre` plus the carry of the lanes before,
-// written once per lane.
+// row boundaries): the carry of the lanes before is added to its in-lane sum
+// in the window, once per lane.
 //
 // Summation order, fixed: entries of a fragment in order within each lane;
 // a fragment spanning lanes adds the lanes' sums left to right.
@@ -666,16 +665,19 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __rest
   uint32_t cnt = 1u + static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(in.sp0), kWave - 1)) - base;
 #pragma unroll
   for (int i = 1; i < 16; ++i) cnt += static_cast<uint32_t>(__builtin_amdgcn_ballot_w64(st[i]) >> 63);
-  const uint32_t q0 = (in.sp0 - base) * 4u;  // byte offset of the fragment holding entry 0
+  uint32_t q0 = (in.sp0 - base) * 4u;  // byte offset of the fragment holding entry 0
+  // (opaque to the compiler: otherwise it re-derives every entry's offset from
+  // sp0 and base, one extra subtraction per entry)
+  __asm__("" : "+v"(q0));
   // entry 15 ends a fragment when the next lane's entry 0 starts one, or at a row end
   const bool end15 = lane_bit((__builtin_amdgcn_ballot_w64(st[0]) >> 1) | kRowLastLanes);
-  float tail, pre, carry = 0.f;
+  float tail, carry = 0.f;
   bool seen, fix;
   for (uint32_t w0 = 0; w0 < cnt; w0 += kStageW) {
     if (cnt <= kStageW)
-      fragment_pass<false>(stage, q0, v, st, end15, tail, seen, pre, fix);
+      fragment_pass<false>(stage, q0, v, st, end15, tail, seen, fix);
     else
-      fragment_pass<true>(stage, q0 - w0 * 4u, v, st, end15, tail, seen, pre, fix);
+      fragment_pass<true>(stage, q0 - w0 * 4u, v, st, end15, tail, seen, fix);
     if (w0 == 0) {
       // inclusive segmented scan of the open tail sums over the 16 lanes of the row
       float tv = tail;
@@ -694,8 +696,10 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __rest
 #undef NWGR_SCAN_STEP
       carry = dpp_f32<kDppRowShr | 1>(tv);  // 0 in the row's first lane
     }
+    // the fragment running in from the previous lane: its in-lane sum was
+    // written at its end; add the carry of the lanes before
     const uint32_t qf = q0 - w0 * 4u;
-    if (fix && qf < kStageW * 4u) stage[qf / 4u] = pre + carry;
+    if (fix && qf < kStageW * 4u) stage[qf / 4u] += carry;
     wave_sync();
     const uint32_t nw = min(kStageW, cnt - w0);
 #pragma unroll
@@ -751,16 +755,20 @@ template <uint32_t KIND>
 __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   constexpr bool kW = KIND != kPlain;
   constexpr int kNW = kSweepThreads / kWave;
-  extern __shared__ float s_tile[];  // 2^bits floats, then kStageW per wavefront (dynamic)
-  __shared__ double s_red[2][kNW];
-  __shared__ uint32_t s_flag;
+  // All LDS is dynamic, so the tile starts at address 0 (its gathers need no
+  // base) and is 16-byte aligned for the float4 fill: 2^bits floats of tile,
+  // then kStageW floats per wavefront.  The convergence reduction runs before
+  // the first tile and borrows the start of the stage area.
+  extern __shared__ __align__(16) float s_tile[];
+  double(*s_red)[kNW] = reinterpret_cast<double(*)[kNW]>(s_tile + (1u << a.bits));
+  uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_red + 2);
   const uint32_t tid = threadIdx.x;
   const uint32_t wl = tid & (kWave - 1);
   const uint32_t wv = __builtin_amdgcn_readfirstlane(tid / kWave);
   if constexpr (KIND != kCoef) {
-    if (tid == 0) s_flag = a.st->done;
+    if (tid == 0) *s_flag = a.st->done;
     __syncthreads();
-    if (s_flag) return;
+    if (*s_flag) return;
     // totals of the previous sweep, identical in every workgroup
     double E = 0, D = 0;
     for (uint32_t b = tid; b < a.nparts; b += kSweepThreads) {
