@@ -494,6 +494,18 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   NWGR_HIP(hipStreamSynchronize(stream));
   g->in_deg_h.swap(deg_h);
   g->offs_h.swap(offs_h);
+  // rows without entries form the tail [nz, n) of the degree order (else no
+  // quiet rows are used)
+  {
+    uint32_t nz = 0;
+    while (nz < n && g->offs_h[nz + 1] > g->offs_h[nz]) ++nz;
+    for (uint32_t r = nz; r < n; ++r)
+      if (g->offs_h[r + 1] != g->offs_h[r]) {
+        nz = n;
+        break;
+      }
+    g->nz = nz;
+  }
   replan(*g, stream);  // plans and sweep grid for the default tuning
 
   // --- iteration buffers

@@ -144,6 +144,7 @@ struct State {
   double err;         // error of the last sweep (valid once done)
   double dang;        // dangling mass (or adjoint Q) of the final iterate (valid once done)
   double dsum;        // tiled engine: dangling mass (or Q) of the sweep's input iterate
+  float rq[2];        // quiet rows (no entries): their common rank in the iterate of parity p
 };
 
 // Host-mapped mirror the solver publishes into: progress every sweep, the
@@ -279,6 +280,7 @@ struct SolveSpec {
   const float* tel_src = nullptr;  // teleport in caller order (permuted at init), or null
   const Tiled* tiled = nullptr;    // tiled engine layout, or null for the degree-binned engine
   const float* tw = nullptr;       // weights in the tiled order (weighted kinds, tiled engine)
+  bool quiet = false;              // rows [g.nz, n) kept as the scalar State::rq (plain kind, tiled)
 };
 
 // Device state of the batched personalized solve (ppr.hip).
@@ -326,6 +328,14 @@ struct Graph {
   DevBuf gsrc;         // f32[n] adjoint source term (new labels)
   DevBuf rsave;        // f32[n] forward ranks (new labels) for the gradient
   DevBuf coef_a, coef_b; // f32[n] per-source gradient coefficients
+
+  // Quiet rows: the rows without entries, [nz, n) after the degree relabelling.
+  // Under a uniform teleport with LEAK or UNIFORM dangling all of them hold one
+  // common rank, so the tiled plain solve keeps it as a scalar (State::rq) and
+  // never stores their ranks (their contributions rq / degree are stored).
+  uint32_t nz = 0;
+  uint32_t nq_dangling = 0;  // quiet rows whose divisor is 0 (their rank is dangling mass)
+  bool quiet_ready = false;  // nq_dangling counted
 
   // iteration buffers
   DevBuf r[2], c[2];

@@ -73,6 +73,9 @@ struct SweepArgs {
   float base_u;        // (1-d)/n as float (uniform teleport)
   float one_minus_d;   // 1-d as float (explicit teleport)
   float un;            // 1/n as float
+  uint32_t nrows;      // rows the sweep stores: n, or nz with quiet rows
+  uint32_t nq_dangling;  // quiet rows with divisor 0
+  uint32_t quiet;      // 1: rows [nrows, n) are quiet (common rank State::rq)
   double tol;
   // gradient coefficients (kCoef): a = d r / s, b = a * (P lambda)
   const float* __restrict__ rsave;
@@ -341,6 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       a.st->iters = 0;
       a.st->err = INFINITY;
       a.st->dang = 0;
+      a.st->rq[0] = a.un;  // quiet rows of the initial iterate (used without an init vector only)
+      a.st->rq[1] = a.un;
     }
   }
 }
@@ -898,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
   // that fits, found with one counting barrier); each step's partials are
   // gathered into LDS by the whole workgroup, with four loads in flight per
   // thread, then every thread adds its row's partials in tile order.
-  const uint32_t n = a.s.n;
+  const uint32_t n = a.s.nrows;  // quiet rows [nrows, n) are not stored
   for (uint32_t w0 = a.row0 + vb * kBlock; w0 < n; w0 += G * kBlock) {
     const uint32_t w1 = min(w0 + kBlock, n);
     for (uint32_t r0 = w0; r0 < w1;) {
@@ -941,6 +946,26 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
       r0 += nfit;
     }
   }
+  if constexpr (KIND == kPlain) {
+    if (a.s.quiet) {
+      // The quiet rows' common rank: the row epilogue with z = 0 (the same
+      // operations, so the value a stored row without entries gets),
+      // computed alike by every workgroup.  Their contributions are stored
+      // (one streaming pass); their ranks are not.  Workgroup 0 adds their
+      // residual and dangling mass as count x value.
+      const float u = a.s.dangling == kDanglingUniform ? a.s.un : 0.f;
+      const float sq = fmaf(Df, u, 0.f);
+      const float rn = fmaf(a.s.d_hi, sq, fmaf(a.s.d_lo, sq, a.s.base_u));
+      for (uint32_t v = a.s.nrows + blockIdx.x * kBlock + tid; v < a.s.n; v += G * kBlock)
+        a.s.c_out[v] = rn / a.s.div[v];
+      if (blockIdx.x == 0 && tid == 0) {
+        const float ro = a.s.st->rq[a.s.p ^ 1];
+        a.s.st->rq[a.s.p] = rn;
+        err += static_cast<double>(a.s.n - a.s.nrows) * fabs(static_cast<double>(rn) - static_cast<double>(ro));
+        dang += static_cast<double>(a.s.nq_dangling) * static_cast<double>(rn);
+      }
+    }
+  }
   if constexpr (KIND != kCoef) {
     block_sum2(err, dang, s_red);
     if (tid == 0) {
@@ -953,14 +978,29 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
 __global__ void k_reset_queue(State* st) { st->work[0] = 0; }
 
 // out[v] = r[iperm[v]] * scale.  With `st`, r is the final iterate
-// r_pair[st->iters & 1].
+// r_pair[st->iters & 1], and rows from `quiet_from` on hold its quiet rank.
 __global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t* __restrict__ iperm,
                                                       const float* __restrict__ r0, const float* __restrict__ r1,
-                                                      const State* st, uint32_t n, float scale,
+                                                      const State* st, uint32_t n, uint32_t quiet_from, float scale,
                                                       float* __restrict__ out) {
   const float* r = (st && (st->iters & 1)) ? r1 : r0;
+  const float rq = st ? st->rq[st->iters & 1] : 0.f;
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) out[v] = r[iperm[v]] * scale;
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const uint32_t i = iperm[v];
+    out[v] = (i < quiet_from ? r[i] : rq) * scale;
+  }
+}
+
+// Quiet rows: the count of zero divisors among them.
+__global__ __launch_bounds__(kBlock) void k_quiet_setup(const float* __restrict__ degf, uint32_t nz, uint32_t n,
+                                                        uint32_t* zeros) {
+  uint32_t z = 0;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t v = nz + blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) z += degf[v] == 0.f ? 1u : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o);
+  if ((threadIdx.x & (kWave - 1)) == 0 && z) atomicAdd(zeros, z);
 }
 
 uint64_t fnv_mix(uint64_t h, uint64_t v) {
@@ -1008,6 +1048,9 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
   a.base_u = static_cast<float>((1.0 - d) / g.n);
   a.one_minus_d = static_cast<float>(1.0 - d);
   a.un = static_cast<float>(1.0 / g.n);
+  a.quiet = sp.quiet ? 1u : 0u;
+  a.nrows = sp.quiet ? g.nz : g.n;
+  a.nq_dangling = g.nq_dangling;
   a.tol = p.tol;
   a.rsave = g.rsave.as<float>();
   a.coef_a = g.coef_a.as<float>();
@@ -1166,6 +1209,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   key = fnv_mix(key, p.max_iters);
   key = fnv_mix(key, p.dangling);
   key = fnv_mix(key, has_tel);
+  key = fnv_mix(key, sp.quiet);
   key = fnv_mix(key, reinterpret_cast<uintptr_t>(s));
   for (const void* ptr : {static_cast<const void*>(sp.offs), static_cast<const void*>(sp.idx),
                           static_cast<const void*>(sp.w), static_cast<const void*>(sp.div),
@@ -1316,7 +1360,7 @@ void run_coef_pass(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, const
 
 void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s) {
   k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), src, src, nullptr, g.n, scale, out);
+      g.iperm.as<uint32_t>(), src, src, nullptr, g.n, g.n, scale, out);
   NWGR_HIP(hipGetLastError());
 }
 
@@ -1331,13 +1375,28 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   sp.init = p.init;
   sp.tel_src = p.teleport;
   sp.tiled = select_tiled(g, false, s);
+  // quiet rows share one rank when the teleport is uniform, the initial
+  // iterate uniform and the dangling mass leaked or spread uniformly
+  sp.quiet = sp.tiled && !p.teleport && !p.init && p.dangling != kDanglingTeleport && g.nz < g.n;
+  if (sp.quiet && !g.quiet_ready) {
+    DevBuf zeros;
+    zeros.alloc(sizeof(uint32_t));
+    NWGR_HIP(hipMemsetAsync(zeros.p, 0, sizeof(uint32_t), s));
+    k_quiet_setup<<<std::min<uint32_t>((g.n - g.nz + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+        g.degf.as<float>(), g.nz, g.n, zeros.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    NWGR_HIP(hipMemcpyAsync(&g.nq_dangling, zeros.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+    g.quiet_ready = true;
+  }
   solve(g, sp, p, res, s);
   hipEvent_t e0, e1;
   NWGR_HIP(hipEventCreate(&e0));
   NWGR_HIP(hipEventCreate(&e1));
   NWGR_HIP(hipEventRecord(e0, s));
   k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, 1.0f, d_out);
+      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n,
+      sp.quiet ? g.nz : g.n, 1.0f, d_out);
   hipError_t le = hipGetLastError();
   NWGR_HIP(hipEventRecord(e1, s));
   NWGR_HIP(hipEventSynchronize(e1));
