@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--engine", choices=["auto", "binned", "tiled"], default="auto",
                     help="sweep engine (auto: the library's choice)")
     ap.add_argument("--tile-bits", type=int, default=0, help="tiled engine source tile width 2^bits (0 = default)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="multi workload: pick the decomposition with nwgr_amd.autotune before timing (0: defaults)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
@@ -228,6 +230,10 @@ def bench_multi(args):
     mp = MultiPageRank(g, vbase, deg)
     out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
     kw = dict(damping=0.85, tol=args.tol, max_iters=args.max_iters, dangling="uniform")
+    if args.autotune:
+        from nwgr_amd.autotune import autotune_multi
+        tr = autotune_multi(mp, sweeps=8, reps=1, rounds=2)
+        log(f"autotuned {tr['tuning']} in {tr['seconds']:.1f}s ({tr['trials']} trials)")
     for _ in range(args.warmup):
         mp.run_device(out.data_ptr(), **kw)
     torch.cuda.synchronize()

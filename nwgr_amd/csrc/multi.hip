@@ -30,6 +30,7 @@ constexpr int kMT = 1024;                 // threads per workgroup
 constexpr uint32_t kMultiMaxN = 65536;    // vertices per graph (16-bit local ids)
 constexpr uint32_t kMultiLdsMax = 36864;  // contributions mirrored in LDS at most (144 KiB)
 constexpr uint32_t kDLeak = 0, kDUniform = 1, kDTeleport = 2;
+constexpr int kEntriesPerStep = 4;  // entries of a row loaded together
 
 struct MultiArgs {
   const uint32_t* __restrict__ offs;   // n + 1 (global)
@@ -47,6 +48,7 @@ struct MultiArgs {
   uint32_t* __restrict__ iters_out;    // count
   double* __restrict__ err_out;        // count
   uint32_t* queue;
+  uint32_t m;  // entries; lidx[m .. m + 64) is zero padding
   uint32_t count, lds_max_n, heavy_min, max_iters, dangling;
   double tol;
   float d_hi, d_lo, one_minus_d;
@@ -86,28 +88,6 @@ __device__ __forceinline__ void bsum2(double& a, double& b, double (*s)[kMT / 64
 }
 
 template <bool LDS>
-__device__ __forceinline__ float gather_thread(const MultiArgs& a, const float* __restrict__ cl,
-                                               const float* __restrict__ cg, uint32_t b, uint32_t e) {
-  float z = 0.f;
-  uint32_t k = b;
-  for (; k + 4 <= e; k += 4) {
-    const uint32_t j0 = a.lidx[k], j1 = a.lidx[k + 1], j2 = a.lidx[k + 2], j3 = a.lidx[k + 3];
-    float v0, v1, v2, v3;
-    if constexpr (LDS) {
-      v0 = cl[j0]; v1 = cl[j1]; v2 = cl[j2]; v3 = cl[j3];
-    } else {
-      v0 = cg[j0]; v1 = cg[j1]; v2 = cg[j2]; v3 = cg[j3];
-    }
-    z += v0;
-    z += v1;
-    z += v2;
-    z += v3;
-  }
-  for (; k < e; ++k) z += LDS ? cl[a.lidx[k]] : cg[a.lidx[k]];
-  return z;
-}
-
-template <bool LDS>
 __device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __restrict__ cl,
                                              const float* __restrict__ cg, uint32_t b, uint32_t e, uint32_t wl) {
   float z = 0.f;
@@ -144,28 +124,63 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     float* __restrict__ cout = a.c[p ^ 1] + v0;
     const float Df = static_cast<float>(D);
     double err = 0, dang = 0;
-    auto finish = [&](uint32_t v, float z) {
-      const float t = a.tel ? a.tel[v0 + v] : un;
+    auto finish = [&](uint32_t v, float z, float t, float dg, float ro) {
       const float base = a.tel ? a.one_minus_d * t : base_u;
       const float u = a.dangling == kDUniform ? un : (a.dangling == kDTeleport ? t : 0.f);
       const float s = fmaf(Df, u, z);
       const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
-      const float dg = a.degf[v0 + v];
       rout[v] = rn;
       cout[v] = rn / dg;
-      err += fabs(static_cast<double>(rn) - static_cast<double>(rin[v]));
+      err += fabs(static_cast<double>(rn) - static_cast<double>(ro));
       if (dg == 0.f) dang += static_cast<double>(rn);
     };
+    // Rows of ordinary length, a thread each.  The next row's bounds and
+    // epilogue operands are loaded while the current row is gathered; a row's
+    // entries are loaded kEntriesPerStep at a time with no bounds branch
+    // (loads past its end read the padding of lidx and are masked to 0).
+    // Each row adds its entries in order.
+    const uint32_t pad = a.m;  // an index into the padding of lidx
+    auto row_in = [&](uint32_t v, uint32_t& rb, uint32_t& re, float& t, float& dg, float& ro) {
+      const bool ok = v < ng;
+      const uint32_t vc = ok ? v : 0u;
+      rb = a.offs[v0 + vc];
+      re = ok ? a.offs[v0 + vc + 1] : rb;
+      t = a.tel ? a.tel[v0 + vc] : un;
+      dg = a.degf[v0 + vc];
+      ro = rin[vc];
+    };
+    uint32_t rb, re;
+    float t, dg, ro;
+    row_in(tid, rb, re, t, dg, ro);
     for (uint32_t v = tid; v < ng; v += kMT) {
-      const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
-      if (e - b > a.heavy_min) continue;
-      finish(v, gather_thread<LDS>(a, cl, cg, b, e));
+      uint32_t nb, ne;
+      float nt, ndg, nro;
+      row_in(v + kMT, nb, ne, nt, ndg, nro);
+      if (re - rb <= a.heavy_min) {
+        float z = 0.f;
+        for (uint32_t k = rb; k < re; k += kEntriesPerStep) {
+          uint32_t j[kEntriesPerStep];
+#pragma unroll
+          for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < re ? k + q : pad];
+#pragma unroll
+          for (int q = 0; q < kEntriesPerStep; ++q) {
+            const float x = LDS ? cl[j[q]] : cg[j[q]];
+            z += k + q < re ? x : 0.f;
+          }
+        }
+        finish(v, z, t, dg, ro);
+      }
+      rb = nb;
+      re = ne;
+      t = nt;
+      dg = ndg;
+      ro = nro;
     }
     for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
       const uint32_t v = a.heavy[h] - v0;
       const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
       const float z = gather_wave<LDS>(a, cl, cg, b, e, wl);
-      if (wl == 0) finish(v, z);
+      if (wl == 0) finish(v, z, a.tel ? a.tel[v0 + v] : un, a.degf[v0 + v], rin[v]);
     }
     bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
     if constexpr (LDS) {
@@ -397,6 +412,7 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   a.iters_out = mg.iters.as<uint32_t>();
   a.err_out = mg.errs.as<double>();
   a.queue = mg.queue.as<uint32_t>();
+  a.m = static_cast<uint32_t>(mg.m);
   a.count = mg.count;
   a.lds_max_n = mg.lds_max_n;
   a.heavy_min = mg.heavy_min;
