@@ -31,6 +31,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -125,6 +127,20 @@ void page_rank(const csr_view& graph, const std::vector<uint32_t>& degrees, std:
                Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
   nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
                          threshold, max_iters);
+}
+
+/* The verification print of bench/pr.cpp:47-52: the `n` best-ranked vertices
+ * as "id: rank" lines, ranks by non-increasing value and ties by ascending id
+ * (ranked on the device, nwgr_top_k_f32). */
+template <typename Real>
+void print_n_ranks(const std::vector<Real>& rankings, std::size_t n, std::ostream& os = std::cout) {
+  std::vector<float> r(rankings.begin(), rankings.end());
+  const uint32_t k = static_cast<uint32_t>(std::min<std::size_t>(n, r.size()));
+  std::vector<uint32_t> ids(k);
+  std::vector<float> vals(k);
+  nwgr_detail::check(nwgr_top_k_f32(r.data(), static_cast<uint32_t>(r.size()), k, ids.data(), vals.data(),
+                                    nwgr_detail::device()));
+  for (uint32_t i = 0; i < k; ++i) os << std::to_string(ids[i]) + ": " << std::to_string(vals[i]) << std::endl;
 }
 
 }  // namespace nw::graph

@@ -267,6 +267,17 @@ uint32_t nwgr_multi_n(const nwgr_multi_graph* mg);
 uint32_t nwgr_multi_count(const nwgr_multi_graph* mg);
 double nwgr_multi_setup_ms(const nwgr_multi_graph* mg);
 
+/* ---- verification: top-k ranked vertices --------------------------------
+ * d_ids[0 .. k) = the k vertices of largest rank, by non-increasing rank and
+ * ascending vertex id among equal ranks; d_vals (may be NULL) their ranks.
+ * Replaces the verification print of bench/pr.cpp:47-52 (print_n_ranks over
+ * proxysort, util/proxysort.hpp:36-65, whose tie order is unspecified).
+ * The _device form takes device arrays and a hipStream_t; nwgr_top_k_f32
+ * takes host arrays and runs on `device`.  n < 2^31, k <= n. */
+int nwgr_top_k_device_f32(const float* d_ranks, uint32_t n, uint32_t k, uint32_t* d_ids, float* d_vals,
+                          void* stream);
+int nwgr_top_k_f32(const float* ranks, uint32_t n, uint32_t k, uint32_t* ids, float* vals, int device);
+
 /* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED or NWGR_ENGINE_TILED
  * (-1 before the first solve or for a null handle). */
 int nwgr_pr_last_engine(const nwgr_pr_graph* g);

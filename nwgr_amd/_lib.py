@@ -108,6 +108,10 @@ def _declare_pagerank(L):
     L.nwgr_pr_last_profile.argtypes = [vp, P(f64), P(u32), P(u32)]
     L.nwgr_pr_last_engine.restype = i32
     L.nwgr_pr_last_engine.argtypes = [vp]
+    L.nwgr_top_k_device_f32.restype = i32
+    L.nwgr_top_k_device_f32.argtypes = [vp, u32, u32, vp, vp, vp]
+    L.nwgr_top_k_f32.restype = i32
+    L.nwgr_top_k_f32.argtypes = [vp, u32, u32, vp, vp, i32]
     L.nwgr_ppr_batch_f32.restype = i32
     L.nwgr_ppr_batch_f32.argtypes = [vp, P(PrParams), u32, vp, vp, P(PrResult)]
     L.nwgr_ppr_batch_device_f32.restype = i32
@@ -191,6 +195,32 @@ def _declare_graphgen(L):
     L.nwgr_splitmix64_at.argtypes = [u64, u64]
     L.nwgr_gen_last_error.restype = ctypes.c_char_p
     L.nwgr_gen_last_error.argtypes = []
+    # graph ingestion (include/nwgr_io.h)
+    cp = ctypes.c_char_p
+    L.nwgr_read_mm.restype = i32
+    L.nwgr_read_mm.argtypes = [cp, i32, P(vp)]
+    L.nwgr_read_nw_edge_list.restype = i32
+    L.nwgr_read_nw_edge_list.argtypes = [cp, P(vp)]
+    L.nwgr_write_nw_edge_list.restype = i32
+    L.nwgr_write_nw_edge_list.argtypes = [cp, u32, u64, vp, vp, i32]
+    L.nwgr_write_nw_adjacency.restype = i32
+    L.nwgr_write_nw_adjacency.argtypes = [cp, u32, u64, vp, vp]
+    L.nwgr_read_nw_adjacency.restype = i32
+    L.nwgr_read_nw_adjacency.argtypes = [cp, P(vp)]
+    L.nwgr_edge_list_n.restype = u32
+    L.nwgr_edge_list_n.argtypes = [vp]
+    L.nwgr_edge_list_m.restype = u64
+    L.nwgr_edge_list_m.argtypes = [vp]
+    L.nwgr_edge_list_directedness.restype = i32
+    L.nwgr_edge_list_directedness.argtypes = [vp]
+    for f in ("src", "dst"):
+        fn = getattr(L, "nwgr_edge_list_" + f)
+        fn.restype = ctypes.POINTER(ctypes.c_uint32)
+        fn.argtypes = [vp]
+    L.nwgr_edge_list_values.restype = ctypes.POINTER(ctypes.c_double)
+    L.nwgr_edge_list_values.argtypes = [vp]
+    L.nwgr_edge_list_free.restype = None
+    L.nwgr_edge_list_free.argtypes = [vp]
 
 
 def graphgen_lib():

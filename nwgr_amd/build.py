@@ -2,8 +2,8 @@
 
 * ``libnwgr_pagerank.so`` -- HIP kernels + C ABI (include/nwgr_pagerank.h),
   compiled by hipcc for gfx950 only.
-* ``libnwgr_graphgen.so`` -- host-only generators / CSR builder
-  (include/nwgr_graphgen.h), compiled by g++.
+* ``libnwgr_graphgen.so`` -- host-only generators / CSR builder and graph
+  ingestion (include/nwgr_graphgen.h, include/nwgr_io.h), compiled by g++.
 
 Always recompiles (the build check runs it on a machine without a GPU;
 hipcc cross-compiles).  Usage: ``python -m nwgr_amd.build``.
@@ -33,7 +33,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["graph.hip", "pagerank.hip", "weighted.hip", "tiled.hip", "ppr.hip", "multi.hip", "csr.hip"]
+HIP_SOURCES = ["graph.hip", "pagerank.hip", "weighted.hip", "tiled.hip", "ppr.hip", "multi.hip", "csr.hip",
+               "topk.hip"]
 
 PAGERANK_LIB = os.path.join(PKG, "libnwgr_pagerank.so")
 GRAPHGEN_LIB = os.path.join(PKG, "libnwgr_graphgen.so")
@@ -67,7 +68,8 @@ def build_graphgen() -> str:
     cxx = shutil.which("g++") or "g++"
     tmp = GRAPHGEN_LIB + ".tmp"
     _run([cxx, "-std=c++20", "-O3", "-march=x86-64-v3", "-fPIC", "-shared", "-pthread",
-          f"-I{INCLUDE}", os.path.join(CSRC, "graphgen.cpp"), "-o", tmp])
+          f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "graphgen.cpp"), os.path.join(CSRC, "graphio.cpp"),
+          "-o", tmp])
     os.replace(tmp, GRAPHGEN_LIB)
     return GRAPHGEN_LIB
 

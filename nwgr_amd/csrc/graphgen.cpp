@@ -23,22 +23,29 @@
 #include <thread>
 #include <vector>
 
-struct nwgr_host_csr {
-  uint32_t n = 0;
-  uint64_t m = 0;
-  std::vector<uint32_t> offsets;
-  std::vector<uint32_t> indices;
-  std::vector<uint32_t> outdeg;
-};
+#include "host_internal.hpp"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
+namespace nwgr_host {
 int fail(const std::string& msg) {
   g_err = msg;
   return 1;
 }
+
+int pick_threads(int nthreads) {
+  if (nthreads > 0) return nthreads;
+  unsigned hc = std::thread::hardware_concurrency();
+  return hc ? static_cast<int>(std::min(hc, 64u)) : 1;
+}
+}  // namespace nwgr_host
+
+namespace {
+
+using nwgr_host::fail;
+using nwgr_host::pick_threads;
 
 constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;
 
@@ -50,12 +57,6 @@ inline uint64_t mix64(uint64_t z) {
 
 // k-th output (0-based) of SplitMix64 seeded with key.
 inline uint64_t sm64_at(uint64_t key, uint64_t k) { return mix64(key + (k + 1) * kGolden); }
-
-int pick_threads(int nthreads) {
-  if (nthreads > 0) return nthreads;
-  unsigned hc = std::thread::hardware_concurrency();
-  return hc ? static_cast<int>(std::min(hc, 64u)) : 1;
-}
 
 // Static split of [0, n) over T threads; fn(lo, hi).
 template <class F>

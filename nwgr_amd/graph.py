@@ -144,3 +144,92 @@ def transpose(graph: CSRGraph, nthreads: int = 0):
 
 def splitmix64_at(key: int, k: int) -> int:
     return int(graphgen_lib().nwgr_splitmix64_at(key, k))
+
+
+# ---------------------------------------------------------------------------
+# Ingestion: the inputs the reference's drivers load (bench/common.hpp:82-101)
+
+
+@dataclass
+class EdgeList:
+    """Directed edge list src[i] -> dst[i] over n vertices, with the file's
+    values when it has them (a MatrixMarket pattern file has none)."""
+
+    n: int
+    src: np.ndarray
+    dst: np.ndarray
+    values: np.ndarray | None = None
+    directed: bool = True
+
+    @property
+    def m(self) -> int:
+        return len(self.src)
+
+
+def _take_edge_list(h) -> EdgeList:
+    L = graphgen_lib()
+    try:
+        n, m = L.nwgr_edge_list_n(h), L.nwgr_edge_list_m(h)
+        src = np.ctypeslib.as_array(L.nwgr_edge_list_src(h), (m,)).copy() if m else np.zeros(0, np.uint32)
+        dst = np.ctypeslib.as_array(L.nwgr_edge_list_dst(h), (m,)).copy() if m else np.zeros(0, np.uint32)
+        vp = L.nwgr_edge_list_values(h)
+        vals = np.ctypeslib.as_array(vp, (m,)).copy() if (vp and m) else None
+        return EdgeList(n, src, dst, vals, L.nwgr_edge_list_directedness(h) == 1)
+    finally:
+        L.nwgr_edge_list_free(h)
+
+
+def read_mm(path: str, nthreads: int = 0) -> EdgeList:
+    """MatrixMarket coordinate file -> directed edge list, as read_mm<directed>
+    (include/nwgraph/io/mmio.hpp:171-217): 1-based entries, a symmetric file's
+    off-diagonal entries in both directions, n = max(rows, largest id)."""
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_read_mm(str(path).encode(), nthreads, ctypes.byref(h)))
+    return _take_edge_list(h)
+
+
+def read_nw_edge_list(path: str) -> EdgeList:
+    """The reference's binary edge list (index_edge_list::deserialize,
+    include/nwgraph/edge_list.hpp:215-233)."""
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_read_nw_edge_list(str(path).encode(), ctypes.byref(h)))
+    return _take_edge_list(h)
+
+
+def write_nw_edge_list(path: str, el: EdgeList) -> None:
+    """index_edge_list::serialize (include/nwgraph/edge_list.hpp:197-203)."""
+    src = np.ascontiguousarray(el.src, dtype=np.uint32)
+    dst = np.ascontiguousarray(el.dst, dtype=np.uint32)
+    _check_gen(graphgen_lib().nwgr_write_nw_edge_list(str(path).encode(), el.n, len(src), src.ctypes.data,
+                                                       dst.ctypes.data, 1 if el.directed else 0))
+
+
+def write_nw_adjacency(path: str, graph: CSRGraph) -> None:
+    """indexed_struct_of_arrays::serialize (containers/compressed.hpp:303-313)."""
+    _check_gen(graphgen_lib().nwgr_write_nw_adjacency(str(path).encode(), graph.n, graph.m,
+                                                       graph.offsets.ctypes.data, graph.indices.ctypes.data))
+
+
+def read_nw_adjacency(path: str) -> CSRGraph:
+    """indexed_struct_of_arrays::deserialize (containers/compressed.hpp:320-333)."""
+    h = ctypes.c_void_p()
+    _check_gen(graphgen_lib().nwgr_read_nw_adjacency(str(path).encode(), ctypes.byref(h)))
+    return _take(h)
+
+
+def load_graph(path: str, nthreads: int = 0) -> EdgeList:
+    """load_graph<directed> (bench/common.hpp:82-101): an "NW" binary edge
+    list or a MatrixMarket file, told apart by the first word."""
+    with open(path, "rb") as f:
+        head = f.read(16)
+    if head.startswith(b"NW"):
+        return read_nw_edge_list(path)
+    if head.startswith(b"%%MatrixMarket"):
+        return read_mm(path, nthreads)
+    raise ValueError(f"did not recognize graph input file {path}")
+
+
+def adjacency_from_edge_list(el: EdgeList, dedup: bool = False, drop_self_loops: bool = False,
+                             nthreads: int = 0) -> CSRGraph:
+    """build_adjacency<1> (bench/common.hpp:103-107) of a loaded edge list."""
+    return build_adjacency(el.src, el.dst, n=el.n, dedup=dedup, drop_self_loops=drop_self_loops, nthreads=nthreads)

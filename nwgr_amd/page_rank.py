@@ -272,3 +272,21 @@ def graph_from_edges_device(src, dst, n: int, dedup: bool = True, drop_self_loop
     check(pagerank_lib().nwgr_pr_graph_create_from_edges_device(
         src.data_ptr() if m else None, dst.data_ptr() if m else None, m, n, flags, device, stream, ctypes.byref(h)))
     return PageRankGraph(_handle=h)
+
+
+def top_k(ranks, k: int, device: int = 0):
+    """The k best-ranked vertices, by non-increasing rank and ascending id
+    among ties (nwgr_top_k_f32; the verification of bench/pr.cpp:47-52):
+    returns (ids uint32[k], ranks float32[k])."""
+    r = np.ascontiguousarray(ranks, dtype=np.float32)
+    if not 0 <= k <= len(r):
+        raise ValueError("k must be in [0, len(ranks)]")
+    ids = np.empty(k, np.uint32)
+    vals = np.empty(k, np.float32)
+    check(pagerank_lib().nwgr_top_k_f32(r.ctypes.data, len(r), k, ids.ctypes.data, vals.ctypes.data, device))
+    return ids, vals
+
+
+def top_k_device(ranks_ptr: int, n: int, k: int, ids_ptr: int, vals_ptr: int | None = None, stream=None) -> None:
+    """Device form of top_k on caller-owned device arrays (e.g. torch tensors)."""
+    check(pagerank_lib().nwgr_top_k_device_f32(ranks_ptr, n, k, ids_ptr, vals_ptr, stream))

@@ -1,7 +1,8 @@
 // Calls the C++20 drop-in nw::graph::page_rank the way bench/pr.cpp:137-139
 // calls the reference: float ranks, damping 0.85f, tolerance and max_iters
 // from the command line.  Input file: "n m" then n+1 offsets, m indices and
-// n degrees (whitespace separated).  Prints one rank per line.
+// n degrees (whitespace separated).  Prints one rank per line, then with a
+// fourth argument k the "id: rank" lines of the k best-ranked vertices.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -36,5 +37,10 @@ int main(int argc, char** argv) {
     return 1;
   }
   for (float r : rankings) std::printf("%.9g\n", r);
+  if (argc > 4) {  // the pr.exe --verify print (bench/pr.cpp:47-52) of the top `argv[4]`
+    std::fflush(stdout);
+    std::cout << "Verifying\n";
+    nw::graph::print_n_ranks(rankings, static_cast<size_t>(std::atol(argv[4])));
+  }
   return 0;
 }

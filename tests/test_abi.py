@@ -26,6 +26,21 @@ def test_graphgen_header_symbols_exported():
         assert hasattr(L, name), name
 
 
+def test_io_header_symbols_exported():
+    L = _lib.graphgen_lib()
+    names = _lib.declared_functions("nwgr_io.h")
+    assert "nwgr_read_mm" in names and "nwgr_read_nw_adjacency" in names
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_every_header_is_checked():
+    """Every C header under include/ belongs to one of the two libraries."""
+    hdrs = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert hdrs == ["nwgr_graphgen.h", "nwgr_io.h", "nwgr_pagerank.h"]
+    assert "nwgr_top_k_f32" in _lib.declared_functions("nwgr_pagerank.h")
+
+
 def test_library_is_gfx950_hip_code():
     """The product library carries gfx950 device code (no other target)."""
     with open(_lib.PAGERANK_SO, "rb") as f:

@@ -239,11 +239,18 @@ def test_cpp_dropin_runs(karate, tmp_path):
     with open(gf, "w") as f:
         f.write(f"{g.n} {g.m}\n" + " ".join(map(str, g.offsets)) + "\n" + " ".join(map(str, g.indices)) + "\n"
                 + " ".join(map(str, deg)) + "\n")
-    out = subprocess.run([str(exe), str(gf), "1e-7", "100"], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([str(exe), str(gf), "1e-7", "100", "10"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    r = np.array([float(x) for x in out.stdout.split()])
+    head, tail = out.stdout.split("Verifying\n")
+    r = np.array([float(x) for x in head.split()])
     ans = np.array(d["answer"])
     assert np.all(np.abs(r - ans) <= 0.005 * ans)
+    # print_n_ranks: the 10 best as "id: rank", by rank then id
+    lines = tail.strip().splitlines()
+    ids = [int(x.split(":")[0]) for x in lines]
+    want = sorted(range(g.n), key=lambda v: (-np.float32(r[v]), v))[:10]
+    assert ids == want, (ids, want)
+    assert all(abs(float(x.split(":")[1]) - r[v]) <= 1e-6 for x, v in zip(lines, ids))
 
 
 @pytest.mark.slow
