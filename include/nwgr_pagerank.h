@@ -122,6 +122,26 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
                            nwgr_pr_result* res, void* stream);
 
+/* Residual ("delta") PageRank.  Replaces page_rank_v13
+ * (experimental/algorithms/page_rank.hpp:632-698, pr.exe version 13,
+ * bench/pr.cpp:146) and the serial page_rank_v3 (:566-630), which returns
+ * the same ranks: a residual it keeps is at or below the threshold and never
+ * pushes.  Ranks start at 0, residuals at 1 - d; each iteration pushes every
+ * residual above p->tol (the reference's `threshold`) into its vertex's
+ * rank and sends residual * d / degree along its out-edges; the sum a vertex
+ * receives is its next residual.  Stops after the first iteration that pushes
+ * from no vertex of non-zero degree, or after p->max_iters iterations.  The
+ * ranks are not normalised (divide by n for the LEAK PageRank they
+ * approximate).  p->damping, p->tol, p->max_iters, p->chunk and p->trace
+ * are used; a teleport, an initial vector or a dangling mode other than LEAK
+ * is rejected.  The sweeps run on the engine the handle's tuning selects.
+ * res->iters = iterations, res->error = pushes of the last one,
+ * res->converged = 1 when that is 0; nwgr_pr_last_errors gives the pushes of
+ * every iteration. */
+int nwgr_pr_delta_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res);
+int nwgr_pr_delta_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
+                                 nwgr_pr_result* res, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Differentiable weighted PageRank (no reference counterpart: the weighted
  * form of the same Jacobi iteration, for gradient-based callers such as the

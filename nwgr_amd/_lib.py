@@ -102,6 +102,10 @@ def _declare_pagerank(L):
     L.nwgr_pr_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
     L.nwgr_pr_run_device_f32.restype = i32
     L.nwgr_pr_run_device_f32.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
+    L.nwgr_pr_delta_run_f32.restype = i32
+    L.nwgr_pr_delta_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
+    L.nwgr_pr_delta_run_device_f32.restype = i32
+    L.nwgr_pr_delta_run_device_f32.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
     L.nwgr_pr_last_errors.restype = u32
     L.nwgr_pr_last_errors.argtypes = [vp, vp, u32]
     L.nwgr_pr_last_profile.restype = i32

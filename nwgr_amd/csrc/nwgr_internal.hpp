@@ -157,6 +157,7 @@ struct HostFlag {
   volatile uint32_t pad;
   volatile double last_err;     // error of sweep `progress`
   volatile double prev_err;     // error of sweep `progress - 1` (0 if none)
+  volatile double first_err;    // error of the initial iterate (kDelta: pushes of iteration 1)
 };
 
 // Timing of the last solve, from events around each captured chunk.
@@ -263,9 +264,13 @@ enum SolveKind : uint32_t {
   kPlain = 0,     // r' = (1-d) t + d (sum c_j + D u),  c = r / deg
   kWeighted = 1,  // same with per-edge weights, c = r / s (s = out-weight sums)
   kAdjoint = 2,   // lambda' = d (s > 0 ? (sum w lambda)/s : Q) + g   on the transpose
-  kNumKinds = 3,
-  kCoef = 3,      // one pass on the transpose: gradient coefficients from the final lambda
+  kDelta = 3,     // residual push: res = sum of deltas; rank += res, delta = res d / deg if res > thr
+  kNumKinds = 4,
+  kCoef = 4,      // one pass on the transpose: gradient coefficients from the final lambda
 };
+
+// Per-edge weights enter the sweep of these kinds.
+constexpr bool weighted_kind(uint32_t kind) { return kind == kWeighted || kind == kAdjoint || kind == kCoef; }
 
 // What one fixed-point solve iterates over.
 struct SolveSpec {
@@ -281,6 +286,7 @@ struct SolveSpec {
   const Tiled* tiled = nullptr;    // tiled engine layout, or null for the degree-binned engine
   const float* tw = nullptr;       // weights in the tiled order (weighted kinds, tiled engine)
   bool quiet = false;              // rows [g.nz, n) kept as the scalar State::rq (plain kind, tiled)
+  float thr = 0.f;                 // push threshold (kDelta)
 };
 
 // Device state of the batched personalized solve (ppr.hip).
@@ -422,6 +428,9 @@ inline size_t iter_len(uint32_t n) {
 // g.r[res.iters & 1] (new labels).
 void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
+// residual ("delta") solve, page_rank_v13; p.tol is the push threshold and
+// res.iters counts iterations (pushes), res.error the last one's push count
+void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
 // one gradient-coefficient pass over the transpose (kind kCoef)
 void run_coef_pass(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, const float* lam, hipStream_t s);
 // k personalized columns with one-hot teleports at d_seeds (caller ids);

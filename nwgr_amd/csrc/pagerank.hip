@@ -76,6 +76,7 @@ struct SweepArgs {
   uint32_t nrows;      // rows the sweep stores: n, or nz with quiet rows
   uint32_t nq_dangling;  // quiet rows with divisor 0
   uint32_t quiet;      // 1: rows [nrows, n) are quiet (common rank State::rq)
+  float thr;           // push threshold (kDelta)
   double tol;
   // gradient coefficients (kCoef): a = d r / s, b = a * (P lambda)
   const float* __restrict__ rsave;
@@ -96,8 +97,10 @@ struct InitArgs {
   State* st;
   uint32_t n;
   uint32_t adjoint;
+  uint32_t delta;      // residual solve: iteration 1's push of the residual res0
   uint32_t dangling;
   float un;
+  float thr, d, res0;  // push threshold, damping, initial residual 1 - d (kDelta)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -198,6 +201,7 @@ __device__ __forceinline__ RowIn load_row(const SweepArgs& a, uint32_t row) {
 // Row epilogue.
 //  forward kinds: teleport + dangling + damping, residual, next contribution
 //  adjoint: lambda' = d (s > 0 ? z/s : Q) + g, residual, Q partial of lambda'
+//  delta: residual from the row's deltas, then the next push
 //  coef: gradient coefficients from P lambda = z/s
 template <uint32_t KIND>
 __device__ __forceinline__ void finish_row(const SweepArgs& a, uint32_t row, float z, float Df, const RowIn& in,
@@ -220,6 +224,17 @@ __device__ __forceinline__ void finish_row(const SweepArgs& a, uint32_t row, flo
       dang += static_cast<double>(ln) * static_cast<double>(a.un);
     else if (a.dangling == kDanglingTeleport)
       dang += static_cast<double>(ln) * static_cast<double>(in.t);
+  } else if constexpr (KIND == kDelta) {
+    // z is the row's sum of deltas (none negative), the residual of phase 2
+    // (experimental/algorithms/page_rank.hpp:677-690); phase 1 of the next
+    // iteration (:661-674) pushes it when above the threshold.  `err` counts
+    // the pushing rows of non-zero degree (the reference's `changed`).
+    const float res = z > 0.f ? z : 0.f;
+    const bool push = res > a.thr;
+    const bool send = push && in.div > 0.f;
+    a.r_out[row] = push ? in.old + res : in.old;
+    a.c_out[row] = send ? res * a.d_hi / in.div : 0.f;
+    if (send) err += 1.0;
   } else {  // kCoef
     float ca = 0.f, cb = 0.f;
     if (in.div > 0.f) {
@@ -238,7 +253,7 @@ __device__ __forceinline__ void finish_row(const SweepArgs& a, uint32_t row, flo
 template <uint32_t KIND, int U, int P>
 __device__ __forceinline__ void process_rows(const SweepArgs& a, const float* __restrict__ lds, uint32_t K,
                                              const Seg& sg, uint32_t ul, float Df, double& err, double& dang) {
-  constexpr bool kW = KIND != kPlain;
+  constexpr bool kW = weighted_kind(KIND);
   const uint32_t wl = threadIdx.x & (kWave - 1);
   const uint32_t G = sg.G;
   const uint32_t lg = __builtin_ctz(G);
@@ -307,9 +322,21 @@ __device__ __forceinline__ void process_rows(const SweepArgs& a, const float* __
 
 __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
   __shared__ double s_red[2][kBlock / 64];
-  double dang = 0, unused = 0;
+  double dang = 0, sends = 0;
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
+    if (a.delta) {
+      // iteration 1, phase 1: every vertex holds the residual 1 - d
+      // (experimental/algorithms/page_rank.hpp:645-651) and pushes it when
+      // it exceeds the threshold; ranks start at 0
+      const float dg = a.div[i];
+      const bool push = a.res0 > a.thr;
+      const bool send = push && dg > 0.f;
+      a.r0[i] = push ? a.res0 : 0.f;
+      a.c0[i] = send ? a.res0 * a.d / dg : 0.f;
+      if (send) sends += 1.0;
+      continue;
+    }
     const uint32_t src = a.perm[i];
     float t = a.un;
     if (a.tel_src) {
@@ -331,9 +358,10 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       if (dg == 0.f) dang += static_cast<double>(r);
     }
   }
-  block_sum2(dang, unused, s_red);
+  block_sum2(dang, sends, s_red);
   if (threadIdx.x == 0) {
-    a.part0[2 * blockIdx.x] = INFINITY;  // "previous error" seen by sweep 1
+    // "previous error" seen by sweep 1: none, or the pushes of iteration 1
+    a.part0[2 * blockIdx.x] = a.delta ? sends : INFINITY;
     a.part0[2 * blockIdx.x + 1] = dang;
     if (blockIdx.x == 0) {
       a.st->ctr[0] = 0;
@@ -355,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
 // the queue on its own until the queue is empty.
 template <uint32_t KIND>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
-  constexpr bool kW = KIND != kPlain;
+  constexpr bool kW = weighted_kind(KIND);
   constexpr int kNW = kSweepThreads / kWave;
   extern __shared__ float s_cache[];  // lds_k floats (dynamic)
   __shared__ double s_red[2][kNW];
@@ -394,7 +422,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
       D += s_red[1][i];
     }
     const uint32_t done_sweeps = a.st->ctr[a.p ^ 1];
-    if ((done_sweeps > 0 && E < a.tol) || done_sweeps >= a.max_iters) {
+    // the residual solve may stop before its first sweep (iteration 1
+    // pushed nothing); the others always sweep once
+    if (((done_sweeps > 0 || KIND == kDelta) && E < a.tol) || done_sweeps >= a.max_iters) {
       if (blockIdx.x == 0 && tid == 0) {
         a.st->done = 1;
         a.st->iters = done_sweeps;
@@ -403,6 +433,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
         if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
         a.hflag->iters = done_sweeps;
         a.hflag->err = E;
+        if (done_sweeps == 0) a.hflag->first_err = E;
         a.hflag->done = 1;
       }
       return;
@@ -415,6 +446,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
       a.hflag->last_err = E;
       a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
       a.hflag->progress = done_sweeps;
+      if (done_sweeps == 0) a.hflag->first_err = E;
     }
     Df = static_cast<float>(D);
     queue = &a.st->work[a.p];
@@ -758,7 +790,7 @@ __device__ __forceinline__ void fill_tile(const float4* __restrict__ src, float4
 
 template <uint32_t KIND>
 __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
-  constexpr bool kW = KIND != kPlain;
+  constexpr bool kW = weighted_kind(KIND);
   constexpr int kNW = kSweepThreads / kWave;
   // All LDS is dynamic, so the tile starts at address 0 (its gathers need no
   // base) and is 16-byte aligned for the float4 fill: 2^bits floats of tile,
@@ -795,7 +827,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
       D += s_red[1][i];
     }
     const uint32_t done_sweeps = a.st->ctr[a.p ^ 1];
-    if ((done_sweeps > 0 && E < a.tol) || done_sweeps >= a.max_iters) {
+    // the residual solve may stop before its first sweep (iteration 1
+    // pushed nothing); the others always sweep once
+    if (((done_sweeps > 0 || KIND == kDelta) && E < a.tol) || done_sweeps >= a.max_iters) {
       if (blockIdx.x == 0 && tid == 0) {
         a.st->done = 1;
         a.st->iters = done_sweeps;
@@ -804,6 +838,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
         if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
         a.hflag->iters = done_sweeps;
         a.hflag->err = E;
+        if (done_sweeps == 0) a.hflag->first_err = E;
         a.hflag->done = 1;
       }
       return;
@@ -815,6 +850,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
       a.hflag->last_err = E;
       a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
       a.hflag->progress = done_sweeps;
+      if (done_sweeps == 0) a.hflag->first_err = E;
     }
   }
 
@@ -1051,6 +1087,7 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
   a.quiet = sp.quiet ? 1u : 0u;
   a.nrows = sp.quiet ? g.nz : g.n;
   a.nq_dangling = g.nq_dangling;
+  a.thr = sp.thr;
   a.tol = p.tol;
   a.rsave = g.rsave.as<float>();
   a.coef_a = g.coef_a.as<float>();
@@ -1065,6 +1102,7 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
     for (const void* f : {reinterpret_cast<const void*>(&k_sweep<kPlain>),
                           reinterpret_cast<const void*>(&k_sweep<kWeighted>),
                           reinterpret_cast<const void*>(&k_sweep<kAdjoint>),
+                          reinterpret_cast<const void*>(&k_sweep<kDelta>),
                           reinterpret_cast<const void*>(&k_sweep<kCoef>)})
       NWGR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     return true;
@@ -1075,6 +1113,7 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
     case kPlain: k_sweep<kPlain><<<grid, kSweepThreads, shm, s>>>(a); break;
     case kWeighted: k_sweep<kWeighted><<<grid, kSweepThreads, shm, s>>>(a); break;
     case kAdjoint: k_sweep<kAdjoint><<<grid, kSweepThreads, shm, s>>>(a); break;
+    case kDelta: k_sweep<kDelta><<<grid, kSweepThreads, shm, s>>>(a); break;
     default: k_sweep<kCoef><<<grid, kSweepThreads, shm, s>>>(a); break;
   }
 }
@@ -1086,6 +1125,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
     for (const void* f : {reinterpret_cast<const void*>(&k_tile<kPlain>),
                           reinterpret_cast<const void*>(&k_tile<kWeighted>),
                           reinterpret_cast<const void*>(&k_tile<kAdjoint>),
+                          reinterpret_cast<const void*>(&k_tile<kDelta>),
                           reinterpret_cast<const void*>(&k_tile<kCoef>)})
       NWGR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     return true;
@@ -1132,6 +1172,10 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
     case kAdjoint:
       k_tile<kAdjoint><<<nb, kSweepThreads, shm, s>>>(ta);
       k_combine<kAdjoint><<<g.grid_c, kBlock, 0, s>>>(ca);
+      break;
+    case kDelta:
+      k_tile<kDelta><<<nb, kSweepThreads, shm, s>>>(ta);
+      k_combine<kDelta><<<g.grid_c, kBlock, 0, s>>>(ca);
       break;
     default:
       k_tile<kCoef><<<nb, kSweepThreads, shm, s>>>(ta);
@@ -1210,6 +1254,9 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   key = fnv_mix(key, p.dangling);
   key = fnv_mix(key, has_tel);
   key = fnv_mix(key, sp.quiet);
+  uint32_t thr_bits;
+  std::memcpy(&thr_bits, &sp.thr, 4);
+  key = fnv_mix(key, thr_bits);
   key = fnv_mix(key, reinterpret_cast<uintptr_t>(s));
   for (const void* ptr : {static_cast<const void*>(sp.offs), static_cast<const void*>(sp.idx),
                           static_cast<const void*>(sp.w), static_cast<const void*>(sp.div),
@@ -1238,8 +1285,12 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.st = g.state.as<State>();
   ia.n = g.n;
   ia.adjoint = sp.kind == kAdjoint;
+  ia.delta = sp.kind == kDelta;
   ia.dangling = p.dangling;
   ia.un = static_cast<float>(1.0 / g.n);
+  ia.thr = sp.thr;
+  ia.d = static_cast<float>(p.damping);
+  ia.res0 = static_cast<float>(1.0 - static_cast<double>(ia.d));
   k_init<<<sp.tiled ? g.grid_c : g.grid, kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());
 
@@ -1408,6 +1459,56 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   res.ms += ms;
 }
 
+void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
+  validate_params(p);
+  NWGR_REQUIRE(!p.teleport && !p.init && p.dangling == kDanglingLeak,
+               "the residual solve takes no teleport, initial vector or dangling mode");
+  if (p.max_iters == 0) {
+    // no iteration: the ranks keep their initial value 0
+    NWGR_HIP(hipMemsetAsync(d_out, 0, sizeof(float) * g.n, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+    res = nwgr_pr_result{};
+    g.last_errs.clear();
+    return;
+  }
+  SolveSpec sp;
+  sp.kind = kDelta;
+  sp.plan = &g.fwd;
+  sp.offs = g.offs.as<uint32_t>();
+  sp.idx = g.idx.as<uint32_t>();
+  sp.div = g.degf.as<float>();
+  sp.tiled = select_tiled(g, false, s);
+  sp.thr = static_cast<float>(p.tol);
+  // iteration 1's push runs in the init kernel, each sweep is phase 2 of one
+  // iteration fused with phase 1 of the next; a sweep's "error" is its push
+  // count, so the solve stops after the first sweep that pushed nothing
+  nwgr_pr_params q = p;
+  q.max_iters = p.max_iters - 1;
+  q.tol = 0.5;
+  solve(g, sp, q, res, s);
+  hipEvent_t e0, e1;
+  NWGR_HIP(hipEventCreate(&e0));
+  NWGR_HIP(hipEventCreate(&e1));
+  NWGR_HIP(hipEventRecord(e0, s));
+  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, g.n, 1.0f, d_out);
+  hipError_t le = hipGetLastError();
+  NWGR_HIP(hipEventRecord(e1, s));
+  NWGR_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  NWGR_HIP(le);
+  res.ms += ms;
+  // per-iteration push counts: iteration 1 (init), then one per sweep
+  const double first = g.hflag->first_err;
+  g.last_errs.insert(g.last_errs.begin(), first);
+  if (res.iters == 0) res.error = first;
+  res.iters += 1;
+  res.converged = res.error < 0.5 ? 1u : 0u;
+}
+
 }  // namespace nwgr
 
 using nwgr::guarded;
@@ -1552,6 +1653,30 @@ int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* lau
     if (sweep_ms) *sweep_ms = g->last_profile.chunk_ms;
     if (launches) *launches = g->last_profile.launches;
     if (chunks) *chunks = g->last_profile.chunks;
+  });
+}
+
+int nwgr_pr_delta_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,
+                                 void* stream) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && p && d_r_out && res, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
+    nwgr::run_delta(*g, *p, d_r_out, *res, s);
+  });
+}
+
+int nwgr_pr_delta_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && p && r_out && res, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = g->own_stream;
+    const size_t nb = sizeof(float) * g->n;
+    nwgr::DevBuf out;
+    out.alloc(nb);
+    nwgr::run_delta(*g, *p, out.as<float>(), *res, s);
+    NWGR_HIP(hipMemcpyAsync(r_out, out.p, nb, hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
   });
 }
 

@@ -161,6 +161,29 @@ class PageRankGraph:
         check(pagerank_lib().nwgr_pr_run_f32(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
         return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
+    def run_delta(self, damping=0.85, threshold=1e-4, max_iters=20, chunk=0, out=None):
+        """Residual ("delta") solve as page_rank_v13
+        (experimental/algorithms/page_rank.hpp:632-698): returns (ranks
+        float32[n] in caller order, unnormalised, RunResult with iters =
+        iterations); last_errors() then holds the pushes of every iteration."""
+        p = _params(damping, threshold, max_iters, "leak", chunk, False)
+        if out is None:
+            out = np.empty(self.n, np.float32)
+        elif out.dtype != np.float32 or out.shape != (self.n,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float32 array of n entries")
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_delta_run_f32(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
+        return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run_delta_device(self, out_ptr: int, damping=0.85, threshold=1e-4, max_iters=20, chunk=0,
+                         stream=None) -> RunResult:
+        """Residual solve with the ranks written to device memory (raw pointer)."""
+        p = _params(damping, threshold, max_iters, "leak", chunk, False)
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_delta_run_device_f32(self._h, ctypes.byref(p), out_ptr, ctypes.byref(res),
+                                                          stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
     def run_device(self, out_ptr: int, damping=0.85, tol=1e-4, max_iters=20, dangling="leak",
                    teleport_ptr=None, init_ptr=None, chunk=0, stream=None) -> RunResult:
         """Solve with device-resident vectors given as raw pointers."""
@@ -236,6 +259,21 @@ def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshol
         raise ValueError("page_rank must be pre-sized to graph.size()")
     with PageRankGraph(graph, degrees, device) as g:
         r, _ = g.run(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak")
+    page_rank[:] = r
+
+
+def page_rank_delta(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, max_iters=20,
+                    num_threads=None, *, device: int = 0) -> None:
+    """Drop-in for page_rank_v13 (experimental/algorithms/page_rank.hpp:
+    632-634), the residual-push PageRank of pr.exe version 13; also returns
+    what the serial page_rank_v3 (:566-568) returns.
+
+    ``page_rank`` is a pre-sized float array filled in place with the
+    unnormalised ranks; ``num_threads`` is accepted and ignored."""
+    if len(page_rank) != graph.size():
+        raise ValueError("page_rank must be pre-sized to graph.size()")
+    with PageRankGraph(graph, degrees, device) as g:
+        r, _ = g.run_delta(damping=damping_factor, threshold=threshold, max_iters=max_iters)
     page_rank[:] = r
 
 

@@ -69,6 +69,9 @@ def _lib(fast: bool = False):
                             ("orc_pr_jacobi_par_f32", f32, _f32p)):
         getattr(L, fn).restype = i32
         getattr(L, fn).argtypes = [u32, _u32p, _u32p, _u32p, real, real, u32, i32, realp, _vp]
+    for fn, realp in (("orc_pr_delta_f64", _f64p), ("orc_pr_delta_f32", _f32p)):
+        getattr(L, fn).restype = i32
+        getattr(L, fn).argtypes = [u32, _u32p, _u32p, _u32p, f64, f64, u32, i32, realp, _vp]
     L.orc_wpr_forward_f64.restype = i32
     L.orc_wpr_forward_f64.argtypes = [u32, _u32p, _u32p, _f64p, _f64p, f64, f64, u32, i32, _f64p, _vp]
     L.orc_wpr_forward_f64_par.restype = i32
@@ -147,6 +150,29 @@ def jacobi_par(offsets, indices, deg, damping=0.85, tol=1e-4, max_iters=20, nthr
     it = _lib(fast).orc_pr_jacobi_par_f32(n, offsets, indices, deg, damping, tol, max_iters, nthreads,
                                           r, None)
     return r, it
+
+
+def delta(offsets, indices, deg, damping=0.85, threshold=1e-4, max_iters=20, dtype=np.float64, keep=False):
+    """Residual ("delta") PageRank as page_rank_v13
+    (experimental/algorithms/page_rank.hpp:632-698), or as page_rank_v3
+    (:566-630, sub-threshold residuals kept until a delta arrives) with
+    keep=True.  Returns
+    (ranks, iterations, pushes) with pushes[i] the number of pushing vertices
+    of non-zero degree in iteration i + 1.  dtype float32 runs the float
+    restatement (the reference's operation order in float)."""
+    offsets, indices, deg = _u32(offsets), _u32(indices), _u32(deg)
+    n = len(offsets) - 1
+    r = np.empty(n, dtype)
+    pushes = np.zeros(max(max_iters, 1), np.int32)
+    fn = _lib().orc_pr_delta_f64 if dtype == np.float64 else _lib().orc_pr_delta_f32
+    if dtype == np.float32:
+        damping = float(np.float32(damping))
+        threshold = float(np.float32(threshold))
+    it = fn(n, offsets, indices, deg, damping, threshold, max_iters, int(keep), r,
+            pushes.ctypes.data_as(ctypes.c_void_p))
+    if it < 0:
+        raise MemoryError("oracle delta: allocation failed")
+    return r, it, pushes[:it]
 
 
 def wpr_forward(offsets, indices, w, t, damping=0.85, tol=1e-10, max_iters=1000, dangling=LEAK,

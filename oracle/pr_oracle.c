@@ -32,6 +32,8 @@
  *   orc_wpr_*           weighted PageRank and its adjoint (no reference
  *                       counterpart: the differentiable form of the same
  *                       iteration, see the functions).
+ *   orc_pr_delta_f64/_f32  residual PageRank, page_rank_v13 / page_rank_v3
+ *                       (experimental/algorithms/page_rank.hpp:566-698).
  */
 #include <math.h>
 #include <stdint.h>
@@ -498,4 +500,83 @@ int orc_wpr_adjoint_f64_par(uint32_t n, const uint32_t* offsets, const uint32_t*
   free(opos);
   free(orow);
   return (int)it;
+}
+
+/* Residual ("delta") PageRank: page_rank_v13
+ * (experimental/algorithms/page_rank.hpp:632-698) with keep = 0, and the
+ * serial page_rank_v3 (:566-630) with keep = 1.  Ranks start at 0 and every
+ * vertex's residual at 1 - d.  Each iteration:
+ *   phase 1: a vertex whose residual exceeds `threshold` adds it to its rank
+ *            and, if its degree is non-zero, sends delta = res * d / deg
+ *            (its residual becomes 0; a vertex that does not push sends 0);
+ *   phase 2: sum_i = sum over row i of the positive deltas; residual_i =
+ *            sum_i if sum_i > 0, else 0 (v13) or unchanged (v3: a residual
+ *            at or below the threshold waits for later deltas, but is
+ *            replaced, not added to, when one arrives);
+ * the loop stops after the first iteration whose phase 1 pushed nothing, or
+ * after max_iters iterations.  Ranks are not normalised (they sum to about n
+ * times the leaked-mass fraction).  Real = float (f32) or double (f64), with
+ * the reference's operation order; row sums in row order.  pushes[it] (when
+ * given, max_iters entries) receives the number of pushing vertices of
+ * non-zero degree in iteration it + 1 (the reference's `changed` is this
+ * count > 0).
+ * Returns the iterations performed (phase 1 + phase 2 pairs). */
+#define ORC_DELTA_BODY(REAL)                                                                       \
+  if (n == 0 || max_iters == 0) {                                                                  \
+    for (uint32_t i = 0; i < n; ++i) rank[i] = 0;                                                  \
+    return 0;                                                                                      \
+  }                                                                                                \
+  REAL* delta = (REAL*)malloc(sizeof(REAL) * n);                                                   \
+  REAL* res = (REAL*)malloc(sizeof(REAL) * n);                                                     \
+  if (!delta || !res) {                                                                            \
+    free(delta);                                                                                   \
+    free(res);                                                                                     \
+    return -1;                                                                                     \
+  }                                                                                                \
+  const REAL d = (REAL)damping, thr = (REAL)threshold;                                             \
+  for (uint32_t i = 0; i < n; ++i) {                                                               \
+    rank[i] = 0;                                                                                   \
+    delta[i] = 0;                                                                                  \
+    res[i] = (REAL)(1.0 - (double)d);                                                              \
+  }                                                                                                \
+  uint32_t it = 0;                                                                                 \
+  for (; it < max_iters;) {                                                                        \
+    int sends = 0;                                                                                 \
+    for (uint32_t s = 0; s < n; ++s) {                                                             \
+      delta[s] = 0;                                                                                \
+      if (res[s] > thr) {                                                                          \
+        const REAL old = res[s];                                                                   \
+        rank[s] += old;                                                                            \
+        res[s] = 0;                                                                                \
+        if (deg[s] > 0) {                                                                          \
+          delta[s] = old * d / (REAL)deg[s];                                                       \
+          ++sends;                                                                                 \
+        }                                                                                          \
+      }                                                                                            \
+    }                                                                                              \
+    for (uint32_t i = 0; i < n; ++i) {                                                             \
+      REAL sum = 0;                                                                                \
+      for (uint32_t k = offsets[i]; k < offsets[i + 1]; ++k)                                       \
+        if (delta[indices[k]] > 0) sum += delta[indices[k]];                                       \
+      if (sum > 0)                                                                                 \
+        res[i] = sum;                                                                              \
+      else if (!keep)                                                                              \
+        res[i] = 0;                                                                                \
+    }                                                                                              \
+    ++it;                                                                                          \
+    if (pushes) pushes[it - 1] = sends;                                                            \
+    if (!sends) break;                                                                             \
+  }                                                                                                \
+  free(delta);                                                                                     \
+  free(res);                                                                                       \
+  return (int)it;
+
+int orc_pr_delta_f64(uint32_t n, const uint32_t* offsets, const uint32_t* indices, const uint32_t* deg,
+                     double damping, double threshold, uint32_t max_iters, int keep, double* rank, int* pushes) {
+  ORC_DELTA_BODY(double)
+}
+
+int orc_pr_delta_f32(uint32_t n, const uint32_t* offsets, const uint32_t* indices, const uint32_t* deg,
+                     double damping, double threshold, uint32_t max_iters, int keep, float* rank, int* pushes) {
+  ORC_DELTA_BODY(float)
 }

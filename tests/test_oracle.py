@@ -251,3 +251,81 @@ def test_parallel_weighted_equals_serial(mode):
     assert a[3] == b[3]
     np.testing.assert_allclose(a[1], b[1], rtol=0, atol=1e-15)
     np.testing.assert_allclose(a[2], b[2], rtol=0, atol=1e-15)
+
+
+# --- residual ("delta") PageRank, page_rank_v13 / page_rank_v3 -------------------
+
+def _path(L):
+    """0 -> 1 -> ... -> L-1 (row i lists its in-neighbours)."""
+    return csr_from_edges(L, np.arange(L - 1), np.arange(1, L))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_delta_path_closed_form(dtype):
+    """On a path every residual is (1-d) d^t after t hops and is pushed once:
+    rank_j = (1-d) sum_{i<=j} d^i, iteration t sends from L - t vertices."""
+    L = 7
+    g = _path(L)
+    deg = oracle.out_degree(L, g.indices)
+    r, it, pushes = oracle.delta(g.offsets, g.indices, deg, D, 0.0, 50, dtype)
+    want = np.array([(1 - D) * sum(D ** i for i in range(j + 1)) for j in range(L)])
+    np.testing.assert_allclose(r, want, rtol=1e-6 if dtype == np.float32 else 1e-14)
+    assert it == L
+    np.testing.assert_array_equal(pushes, np.arange(L - 1, -1, -1))
+
+
+def test_delta_threshold_and_iteration_cap():
+    # 0 -> 1: the delta 0.1275 reaching vertex 1 is below the threshold 0.13,
+    # so iteration 2 pushes nothing and the residual is dropped
+    g = csr_from_edges(2, [0], [1])
+    deg = oracle.out_degree(2, g.indices)
+    r, it, pushes = oracle.delta(g.offsets, g.indices, deg, D, 0.13, 20)
+    np.testing.assert_allclose(r, [0.15, 0.15], rtol=1e-15)
+    assert it == 2 and list(pushes) == [1, 0]
+    # threshold at 1 - d: nothing ever pushes
+    r, it, pushes = oracle.delta(g.offsets, g.indices, deg, D, 1 - D, 20)
+    assert it == 1 and list(pushes) == [0] and not r.any()
+    # caps: one iteration pushes the initial residuals, none leaves zeros
+    path = _path(5)
+    pdeg = oracle.out_degree(5, path.indices)
+    r, it, _ = oracle.delta(path.offsets, path.indices, pdeg, D, 0.0, 1)
+    assert it == 1
+    np.testing.assert_allclose(r, np.full(5, 1 - D), rtol=1e-15)
+    r, it, pushes = oracle.delta(path.offsets, path.indices, pdeg, D, 0.0, 0)
+    assert it == 0 and len(pushes) == 0 and not r.any()
+
+
+def test_delta_converges_to_leak_fixed_point():
+    """With a vanishing threshold the pushed mass sums the Neumann series:
+    rank / n = (1-d)/n (I - d P)^-1 1, the LEAK fixed point."""
+    g = random_graph(80, 400, seed=21)
+    deg = oracle.out_degree(g.n, g.indices)
+    r, it, pushes = oracle.delta(g.offsets, g.indices, deg, D, 1e-16, 1000)
+    assert pushes[-1] == 0 and it < 1000
+    want = dense_fixed_point(g, deg, oracle.LEAK)
+    assert l1(r / g.n, want) < 1e-12
+    # a plausible mistake, dangling mass spread uniformly, is far off
+    assert l1(want, dense_fixed_point(g, deg, oracle.UNIFORM)) > 1e-3
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_delta_v3_keeps_nothing_that_pushes(seed):
+    """v3 keeps a residual at or below the threshold until a delta replaces it;
+    such a residual never pushes, so v3 and v13 give the same ranks."""
+    g = random_graph(300, 1500, seed=seed)
+    deg = oracle.out_degree(g.n, g.indices)
+    for thr in (1e-4, 1e-3, 0.05):
+        a = oracle.delta(g.offsets, g.indices, deg, D, thr, 60)
+        b = oracle.delta(g.offsets, g.indices, deg, D, thr, 60, keep=True)
+        np.testing.assert_array_equal(a[0], b[0])
+        assert a[1] == b[1]
+        np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_delta_float_restatement_close_to_double():
+    g = random_graph(500, 4000, seed=5)
+    deg = oracle.out_degree(g.n, g.indices)
+    r64, it64, _ = oracle.delta(g.offsets, g.indices, deg, D, 1e-7, 100)
+    r32, it32, _ = oracle.delta(g.offsets, g.indices, deg, D, 1e-7, 100, np.float32)
+    assert abs(it64 - it32) <= 1
+    assert l1(r32.astype(np.float64) / g.n, r64 / g.n) < 1e-6
