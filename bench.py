@@ -63,11 +63,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
-    ap.add_argument("--workload", choices=["pagerank", "er", "ppr", "multi", "wpr"], default="pagerank",
+    ap.add_argument("--workload", choices=["pagerank", "er", "ppr", "multi", "wpr", "delta"], default="pagerank",
                     help="pagerank: the headline solve on R-MAT s22; er: the same solve on Erdos-Renyi "
                          "G(n = 2^25, m = 16n); ppr: 256 personalized columns on R-MAT s20; "
                          "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
-                         "wpr: weighted forward + adjoint on R-MAT s20")
+                         "wpr: weighted forward + adjoint on R-MAT s20; delta: residual PageRank "
+                         "(page_rank_v13, threshold 1e-4) on R-MAT s22")
     ap.add_argument("--fwd-adjoint", type=int, default=1,
                     help="1: also time the weighted forward and forward + adjoint on R-MAT s20 (the metric's "
                          "fwd+adjoint vs fwd ratio); 0: skip")
@@ -217,6 +218,69 @@ def bench_ppr(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_delta(args):
+    """Residual ("delta") PageRank as pr.exe version 13 (page_rank_v13) on
+    R-MAT s22 ef16: threshold 1e-4 (pr.exe's default tolerance), up to 100
+    iterations.  One iteration is one sweep (8M + 16N algorithmic bytes:
+    index and gathered delta per edge; degree, rank in and out, delta out
+    per vertex).  CPU baseline: the oracle's serial float restatement of
+    v13 for a few iterations (kind "port", one thread)."""
+    import torch
+    import nwgr_amd
+    import oracle
+    from nwgr_amd import PageRankGraph
+    thr = 1e-4
+    g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    kw = dict(damping=0.85, threshold=thr, max_iters=args.max_iters, chunk=args.chunk)
+    for _ in range(args.warmup):
+        pg.run_delta_device(out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, sweep_ms, launches = 0, 0.0, 0
+    for _ in range(args.steps):
+        res = pg.run_delta_device(out.data_ptr(), **kw)
+        prof = pg.last_profile()
+        iters += res.iters
+        sweep_ms += prof["sweep_ms"]
+        launches += prof["launches"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pushes = pg.last_errors()
+    n, m = g.n, g.m
+    b_iter = 8 * m + 16 * n
+    sweeps = iters - args.steps  # iteration 1 of each solve is the init kernel's push
+    achieved = b_iter * sweeps / (sweep_ms / 1e3) / 1e9
+    cpu = None
+    if args.cpu_seconds > 0:
+        k_cpu = 20
+        t1 = time.perf_counter()
+        oracle.delta(g.offsets, g.indices, deg, 0.85, thr, k_cpu, np.float32)
+        t_cpu = time.perf_counter() - t1
+        cpu = {"value": round(m * k_cpu / t_cpu / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": f"oracle float restatement of page_rank_v13, {k_cpu} iterations on the same graph "
+                         f"(serial, {t_cpu:.1f} s)"}
+    line = {
+        "metric": "residual PageRank (page_rank_v13) GTEPS", "value": round(m * iters / wall / 1e9, 3),
+        "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"page_rank_v13 on rmat s{args.scale} ef{args.edge_factor}, threshold {thr:g}, "
+                               f"max {args.max_iters} iterations, FP32", "n": n, "m": m,
+                   "iterations_per_solve": res.iters, "pushes_first_last": [int(pushes[0]), int(pushes[-1])],
+                   "engine": pg.last_engine()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_tile+k_combine<delta>" if pg.last_engine() == "tiled" else "k_sweep<delta>",
+                     "algorithmic_bytes_per_sweep": b_iter, "avg_launch_us": round(sweep_ms / launches * 1e3, 2),
+                     "avg_sweep_us": round(sweep_ms / max(sweeps, 1) * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_multi(args):
     """2,048 R-MAT graphs, n log-uniform in [1k, 64k], edge factor 8, every
     graph solved to its own tolerance in one persistent launch."""
@@ -281,6 +345,9 @@ def main():
         return
     if args.workload == "wpr":
         bench_wpr(args)
+        return
+    if args.workload == "delta":
+        bench_delta(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "

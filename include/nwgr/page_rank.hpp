@@ -72,10 +72,14 @@ inline int& device() {
   return dev;
 }
 
+/* Which solve `run` performs: the Jacobi sweeps of page_rank, or the
+ * residual push of page_rank_v13 / page_rank_v3. */
+enum class solver { sweeps, residual };
+
 template <typename Real>
 void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
          const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank, Real damping_factor,
-         Real threshold, std::size_t max_iters) {
+         Real threshold, std::size_t max_iters, solver kind = solver::sweeps) {
   static_assert(std::is_floating_point_v<Real>, "Real must be a floating-point type");
   if (page_rank.size() < n) throw std::runtime_error("nwgr: page_rank must hold graph.size() entries");
   if (degrees.size() < n) throw std::runtime_error("nwgr: degrees must hold graph.size() entries");
@@ -92,11 +96,14 @@ void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t 
   p.max_iters = max_iters > 0xffffffffu ? 0xffffffffu : static_cast<uint32_t>(max_iters);
   p.dangling = NWGR_DANGLING_LEAK;
   nwgr_pr_result res;
+  auto solve = [&](float* out) {
+    check(kind == solver::residual ? nwgr_pr_delta_run_f32(g, &p, out, &res) : nwgr_pr_run_f32(g, &p, out, &res));
+  };
   if constexpr (std::is_same_v<Real, float>) {
-    check(nwgr_pr_run_f32(g, &p, page_rank.data(), &res));
+    solve(page_rank.data());
   } else {
     std::vector<float> tmp(n);
-    check(nwgr_pr_run_f32(g, &p, tmp.data(), &res));
+    solve(tmp.data());
     for (uint32_t i = 0; i < n; ++i) page_rank[i] = static_cast<Real>(tmp[i]);
   }
 }
@@ -127,6 +134,44 @@ void page_rank(const csr_view& graph, const std::vector<uint32_t>& degrees, std:
                Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
   nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
                          threshold, max_iters);
+}
+
+/* Residual ("delta") PageRank, the signature of page_rank_v13
+ * (experimental/algorithms/page_rank.hpp:632-634, pr.exe version 13):
+ * ranks start at 0, residuals at 1 - d; residuals above `threshold` are
+ * pushed into the rank and sent on as residual * d / degree; unnormalised
+ * result (nwgr_pr_delta_run_f32). */
+template <nwgr_csr_graph Graph, typename Real>
+void page_rank_v13(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+                   std::vector<Real>& page_rank, Real damping_factor, Real threshold, std::size_t max_iters,
+                   std::size_t /*num_threads*/) {
+  static_assert(sizeof(typename Graph::vertex_id_type) == sizeof(uint32_t),
+                "the device path takes uint32 vertex ids (util/defaults.hpp:23-24)");
+  const auto& offs = graph.indices_;
+  const auto& nbrs = std::get<0>(graph.to_be_indexed_);
+  const uint32_t n = static_cast<uint32_t>(graph.size());
+  std::vector<uint32_t> deg(degrees.begin(), degrees.end());
+  nwgr_detail::run<Real>(reinterpret_cast<const uint32_t*>(offs.data()),
+                         reinterpret_cast<const uint32_t*>(nbrs.data()), n,
+                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), deg, page_rank, damping_factor,
+                         threshold, max_iters, nwgr_detail::solver::residual);
+}
+
+template <typename Real>
+void page_rank_v13(const csr_view& graph, const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank,
+                   Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
+  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
+                         threshold, max_iters, nwgr_detail::solver::residual);
+}
+
+/* page_rank_v3 (experimental/algorithms/page_rank.hpp:566-568), the serial
+ * residual push: it keeps a residual at or below the threshold until a delta
+ * replaces it, and such a residual never pushes, so its ranks are those of
+ * page_rank_v13. */
+template <typename Graph, typename Real>
+void page_rank_v3(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+                  std::vector<Real>& page_rank, Real damping_factor, Real threshold, std::size_t max_iters) {
+  page_rank_v13(graph, degrees, page_rank, damping_factor, threshold, max_iters, 1);
 }
 
 /* The verification print of bench/pr.cpp:47-52: the `n` best-ranked vertices

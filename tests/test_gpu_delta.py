@@ -137,3 +137,37 @@ def test_delta_rmat_s20(ename):
     pg.tuning = ENGINES[ename]
     for thr in (1e-4, 1e-7):
         delta_parity(g, deg, thr, 100, pg)
+
+
+def test_cpp_dropin_v13_and_v3(karate, tmp_path):
+    """The C++ page_rank_v13 / page_rank_v3 templates (include/nwgr/page_rank.hpp)
+    as pr.exe versions 13 and 3 call them (bench/pr.cpp:110,146)."""
+    import os
+    import shutil
+    import subprocess
+    from nwgr_amd import _lib
+    from tests.conftest import ROOT
+    _, g, deg = karate
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ missing")
+    exe = tmp_path / "dropin_pr"
+    subprocess.run([gxx, "-std=c++20", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dropin_pr.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(_lib.PAGERANK_SO), "-lnwgr_pagerank",
+                    "-Wl,-rpath," + os.path.dirname(_lib.PAGERANK_SO), "-Wl,--allow-shlib-undefined"],
+                   check=True)
+    gf = tmp_path / "karate.txt"
+    with open(gf, "w") as f:
+        f.write(f"{g.n} {g.m}\n" + " ".join(map(str, g.offsets)) + "\n" + " ".join(map(str, g.indices)) + "\n"
+                + " ".join(map(str, deg)) + "\n")
+    r32, _, _ = oracle.delta(g.offsets, g.indices, deg, D, 1e-5, 50, np.float32)
+    outs = []
+    for version in ("13", "3"):
+        out = subprocess.run([str(exe), str(gf), "1e-5", "50", "0", version], capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stderr
+        r = np.array([float(x) for x in out.stdout.split()], np.float32)
+        np.testing.assert_allclose(r, r32, rtol=1e-5)
+        outs.append(r)
+    np.testing.assert_array_equal(outs[0], outs[1])
