@@ -58,17 +58,22 @@ def parse():
     ap.add_argument("--engine", choices=["auto", "binned", "tiled"], default="auto",
                     help="sweep engine (auto: the library's choice)")
     ap.add_argument("--tile-bits", type=int, default=0, help="tiled engine source tile width 2^bits (0 = default)")
+    ap.add_argument("--combine", choices=["auto", "gather", "accumulate"], default="auto",
+                    help="tiled engine second pass (auto: the library's choice)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="multi workload: pick the decomposition with nwgr_amd.autotune before timing (0: defaults)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
-    ap.add_argument("--workload", choices=["pagerank", "er", "ppr", "multi", "wpr", "delta"], default="pagerank",
+    ap.add_argument("--workload", choices=["pagerank", "er", "ppr", "multi", "wpr", "delta", "shard"],
+                    default="pagerank",
                     help="pagerank: the headline solve on R-MAT s22; er: the same solve on Erdos-Renyi "
                          "G(n = 2^25, m = 16n); ppr: 256 personalized columns on R-MAT s20; "
                          "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
                          "wpr: weighted forward + adjoint on R-MAT s20; delta: residual PageRank "
-                         "(page_rank_v13, threshold 1e-4) on R-MAT s22")
+                         "(page_rank_v13, threshold 1e-4) on R-MAT s22; shard: one GPU's row share of R-MAT s22 "
+                         "(--shards slices)")
+    ap.add_argument("--shards", type=int, default=8, help="shard workload: number of row slices")
     ap.add_argument("--fwd-adjoint", type=int, default=1,
                     help="1: also time the weighted forward and forward + adjoint on R-MAT s20 (the metric's "
                          "fwd+adjoint vs fwd ratio); 0: skip")
@@ -103,24 +108,42 @@ def cpu_baseline(g, deg, target_s, threads, tol, max_iters):
     """The reference's default page_rank (v11, in-place Gauss-Seidel) as the
     OpenMP restatement, FP32 as bench/pr.cpp:89, solved to the same tolerance
     as the GPU solve; whole solves are repeated (as pr.exe's trials) until
-    about target_s seconds have been spent, after one untimed warm-up solve."""
+    about 3/4 of target_s seconds have been spent, after one untimed warm-up
+    solve, and reported as min/avg/max per solve (Times::print,
+    bench/common.hpp:205-231).  The Jacobi form (page_rank_v10, OpenMP, FP32)
+    is timed beside it with the rest of the budget."""
     import oracle  # the CPU restatement: baseline leg only
     t0 = time.perf_counter()
     _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads, dtype=np.float32)
     per_solve = time.perf_counter() - t0
-    trials = int(max(1, min(1000, round(target_s / max(per_solve, 1e-6)))))
+    trials = int(max(1, min(1000, round(0.75 * target_s / max(per_solve, 1e-6)))))
     sweeps = 0
-    t0 = time.perf_counter()
+    times = []
     for _ in range(trials):
+        t1 = time.perf_counter()
         _, it = oracle.gs_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads, dtype=np.float32)
+        times.append(time.perf_counter() - t1)
         sweeps += it
-    dt = time.perf_counter() - t0
+    dt = sum(times)
+    # Jacobi (v10) restatement: same tolerance, a few solves
+    t1 = time.perf_counter()
+    _, jit = oracle.jacobi_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads)
+    j_one = time.perf_counter() - t1
+    j_trials = int(max(1, min(100, round(0.25 * target_s / max(j_one, 1e-6)))))
+    t1 = time.perf_counter()
+    for _ in range(j_trials):
+        _, jit = oracle.jacobi_par(g.offsets, g.indices, deg, 0.85, tol, max_iters, nthreads=threads)
+    jdt = time.perf_counter() - t1
     return {"value": g.m * sweeps / dt / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
             "sample": f"R-MAT s{int(np.log2(g.n))} ef16 (the same graph): {trials} solves to L1 tol {tol:g} "
                       f"({it} in-place Gauss-Seidel sweeps each, {sweeps} in all) of the OpenMP restatement of "
                       f"page_rank v11 (FP32 ranks, dangling mass leaked as the reference, schedule(dynamic,64)), "
                       f"{dt:.2f} s",
-            "ms_per_solve": dt * 1e3 / trials, "sweeps_per_solve": it, "seconds_per_sweep": dt / sweeps}
+            "ms_per_solve": dt * 1e3 / trials, "sweeps_per_solve": it, "seconds_per_sweep": dt / sweeps,
+            "ms_per_solve_min_avg_max": [round(min(times) * 1e3, 2), round(dt * 1e3 / trials, 2),
+                                         round(max(times) * 1e3, 2)],
+            "jacobi_v10": {"ms_per_solve": round(jdt * 1e3 / j_trials, 2), "sweeps_per_solve": jit,
+                           "gteps": round(g.m * jit * j_trials / jdt / 1e9, 3), "solves": j_trials}}
 
 
 def fwd_adjoint_cost(seed, steps=5, warmup=2):
@@ -273,10 +296,74 @@ def bench_delta(args):
                    "engine": pg.last_engine()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_tile+k_combine<delta>" if pg.last_engine() == "tiled" else "k_sweep<delta>",
+                     "kernel": {"tiled": "k_tile+k_combine<delta>", "tiled-accumulate": "k_tile+k_accum<delta>"}.get(
+                         pg.last_engine(), "k_sweep<delta>"),
                      "algorithmic_bytes_per_sweep": b_iter, "avg_launch_us": round(sweep_ms / launches * 1e3, 2),
                      "avg_sweep_us": round(sweep_ms / max(sweeps, 1) * 1e3, 2)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_shard(args):
+    """One GPU's share of a row-partitioned solve of R-MAT s22 ef16: the
+    destination rows cut into `--shards` contiguous slices of about equal
+    entry counts, each run as the same n-vertex graph with entries only in its
+    slice (the full gathered vector resident, as each GPU would hold it).
+    Per share: device time per sweep over fixed-length solves (tol 0, 20
+    sweeps, UNIFORM dangling); algorithmic bytes 8 M_k + 16 N_k.  The rows
+    outside the slice have no entries and cost only their contribution
+    store (4 B each), which a real share would not make."""
+    import torch
+    import nwgr_amd
+    from nwgr_amd import PageRankGraph
+    G = args.shards
+    g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
+    deg = nwgr_amd.build_degrees(g)
+    cuts = nwgr_amd.row_cuts(g, G)
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    kw = dict(damping=0.85, tol=0.0, max_iters=20, dangling="uniform")
+
+    def per_sweep_us(graph):
+        with PageRankGraph(graph, deg) as pg:
+            for _ in range(max(args.warmup, 1)):
+                pg.run_device(out.data_ptr(), **kw)
+            ms, sw = 0.0, 0
+            for _ in range(args.steps):
+                res = pg.run_device(out.data_ptr(), **kw)
+                ms += pg.last_profile()["sweep_ms"]
+                sw += res.iters
+            return ms / sw * 1e3, pg.last_engine()
+
+    full_us, engine = per_sweep_us(g)
+    log(f"whole graph: {full_us:.1f} us per sweep ({engine})")
+    shares = []
+    offs = g.offsets.astype(np.int64)
+    for k in range(G):
+        lo, hi = int(cuts[k]), int(cuts[k + 1])
+        us, eng = per_sweep_us(nwgr_amd.row_shard(g, lo, hi))
+        m_k, n_k = int(offs[hi] - offs[lo]), hi - lo
+        b_k = 8 * m_k + 16 * n_k
+        shares.append({"rows": [lo, hi], "m": m_k, "us_per_sweep": round(us, 2), "engine": eng,
+                       "gbs": round(b_k / (us / 1e6) / 1e9, 1)})
+        log(f"share {k}: rows [{lo}, {hi}) m={m_k} {us:.1f} us per sweep")
+    worst = max(shares, key=lambda s: s["us_per_sweep"])
+    b_w = 8 * worst["m"] + 16 * (worst["rows"][1] - worst["rows"][0])
+    line = {
+        "metric": f"per-GPU sweep time of one of {G} row shares, R-MAT s{args.scale} (slowest share)",
+        "value": worst["us_per_sweep"], "unit": "us", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"rmat s{args.scale} ef{args.edge_factor} rows cut into {G} slices of equal entry "
+                               f"counts, each share run alone with the full gathered vector; 20 sweeps per solve",
+                   "n": g.n, "m": g.m, "whole_graph_us_per_sweep": round(full_us, 2), "shares": shares,
+                   "sum_of_shares_us": round(sum(s["us_per_sweep"] for s in shares), 2)},
+        "roofline": {"bound": "hbm", "achieved": worst["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(worst["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": {"tiled": "k_tile+k_combine", "tiled-accumulate": "k_tile+k_accum"}.get(
+                         worst["engine"], "k_sweep"),
+                     "algorithmic_bytes_per_sweep": b_w},
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 
@@ -349,6 +436,9 @@ def main():
     if args.workload == "delta":
         bench_delta(args)
         return
+    if args.workload == "shard":
+        bench_shard(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PageRank path here runs on a single MI355X "
               "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
@@ -368,12 +458,14 @@ def main():
     deg = nwgr_amd.build_degrees(g)
     log(f"generated {gname}: n={g.n} m={g.m} in {time.perf_counter() - t0:.1f}s")
     pg = PageRankGraph(g, deg)
-    if args.engine != "auto" or args.tile_bits:
+    if args.engine != "auto" or args.tile_bits or args.combine != "auto":
         tune = {}
         if args.engine != "auto":
             tune["engine"] = {"binned": 0, "tiled": 1}[args.engine]
         if args.tile_bits:
             tune["tile_bits"] = args.tile_bits
+        if args.combine != "auto":
+            tune["tile_combine"] = {"gather": 0, "accumulate": 1}[args.combine]
         pg.tuning = tune
     log(f"device graph: {pg.device_bytes / 2**20:.0f} MiB, setup {pg.setup_ms:.1f} ms")
     import torch
@@ -418,11 +510,12 @@ def main():
     traffic = load_traffic(args.scale, args.edge_factor) if args.workload == "pagerank" else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_tile+k_combine" if engine == "tiled" else "k_sweep", "engine": engine,
+            "kernel": {"tiled": "k_tile+k_combine", "tiled-accumulate": "k_tile+k_accum"}.get(engine, "k_sweep"),
+            "engine": engine,
             "algorithmic_bytes_per_sweep": b_iter,
             "avg_launch_us": round(sweep_ms / launches * 1e3, 2), "launches": launches, "sweeps": sweeps,
             "avg_sweep_us": round(sweep_ms / sweeps * 1e3, 2),
-            "launch": "one sweep: k_tile + k_combine (tiled engine) or k_sweep (binned); the stopping launch "
+            "launch": "one sweep: k_tile + k_combine or k_accum (tiled engine) or k_sweep (binned); the stopping launch "
                       "of each solve is timed with the sweeps"}
     value = m * sweeps / wall / 1e9
 
@@ -450,8 +543,20 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:
+        # time to tolerance on the problem the CPU path solves: the reference
+        # leaks the dangling mass (page_rank.hpp:94,133), which takes more
+        # sweeps to reach the same L1 tolerance than the UNIFORM headline
+        kl = dict(kw, dangling="leak")
+        res_l = pg.run_device(out_d.data_ptr(), **kl)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            res_l = pg.run_device(out_d.data_ptr(), **kl)
+        torch.cuda.synchronize()
+        leak_ms = (time.perf_counter() - t1) * 1e3 / args.steps
+        line["gpu_leak_solve"] = {"ms_per_solve": round(leak_ms, 4), "sweeps_per_solve": res_l.iters}
         line["gpu_vs_cpu_per_sweep"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
-        line["gpu_vs_cpu_time_to_tolerance"] = round(cpu["ms_per_solve"] / (wall * 1e3 / args.steps), 1)
+        line["gpu_vs_cpu_time_to_tolerance"] = round(cpu["ms_per_solve"] / leak_ms, 1)
     line["fwd_adjoint_vs_fwd"] = fa
     print(json.dumps(line), flush=True)
 

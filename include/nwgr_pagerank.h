@@ -187,7 +187,22 @@ typedef struct nwgr_pr_tuning {
   uint32_t tile_bits;          /* tiled engine: source tile width 2^tile_bits, 13 .. 15 */
   uint32_t tile_fill_cost;     /* tiled engine: chunks of 512 entries one tile fill is
                                   charged when the chunks are cut into workgroup ranges */
+  uint32_t tile_combine;       /* tiled engine, second pass: NWGR_COMBINE_GATHER,
+                                  NWGR_COMBINE_ACCUMULATE or NWGR_COMBINE_AUTO */
 } nwgr_pr_tuning;
+
+/* Second pass of the tiled engine.  GATHER: every row reads its partial sums
+ * (one per source tile it has entries in) through a row-ordered slot list,
+ * tiles ascending.  ACCUMULATE: rows are cut into blocks of at most 16384;
+ * a workgroup streams the block's partials tile by tile (each tile's partials
+ * of a block are contiguous) into a double-precision LDS accumulator per row,
+ * then applies the row epilogue: no scattered reads, for graphs whose rows
+ * have about one entry per source tile (low locality, e.g. Erdos-Renyi).
+ * The double sum of a row's float partials is exact, hence independent of
+ * the order the atomics land in, whenever the partials lie within a factor
+ * of about 2^19 of each other.  AUTO: ACCUMULATE when the CSR has fewer than
+ * 1.5 entries per fragment (run of one row inside one tile), else GATHER. */
+enum { NWGR_COMBINE_GATHER = 0, NWGR_COMBINE_ACCUMULATE = 1, NWGR_COMBINE_AUTO = 2 };
 
 /* Sweep engines.  BINNED: pull gathers over degree-binned rows, the hottest
  * contributions staged in LDS.  TILED: source tiles staged in LDS, per-tile
@@ -298,8 +313,9 @@ int nwgr_top_k_device_f32(const float* d_ranks, uint32_t n, uint32_t k, uint32_t
                           void* stream);
 int nwgr_top_k_f32(const float* ranks, uint32_t n, uint32_t k, uint32_t* ids, float* vals, int device);
 
-/* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED or NWGR_ENGINE_TILED
- * (-1 before the first solve or for a null handle). */
+/* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED, NWGR_ENGINE_TILED
+ * (second pass GATHER) or 2 (tiled, second pass ACCUMULATE); -1 before the
+ * first solve or for a null handle. */
 int nwgr_pr_last_engine(const nwgr_pr_graph* g);
 
 /* Thread-local message of the last failure on this thread ("" if none). */

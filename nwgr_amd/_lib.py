@@ -65,10 +65,12 @@ class PrTuning(ctypes.Structure):
         ("engine", ctypes.c_uint32),
         ("tile_bits", ctypes.c_uint32),
         ("tile_fill_cost", ctypes.c_uint32),
+        ("tile_combine", ctypes.c_uint32),
     ]
 
 
 ENGINE_BINNED, ENGINE_TILED, ENGINE_AUTO = 0, 1, 2
+COMBINE_GATHER, COMBINE_ACCUMULATE, COMBINE_AUTO = 0, 1, 2
 
 
 _pr = None

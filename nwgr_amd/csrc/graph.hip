@@ -320,6 +320,7 @@ void replan(Graph& g, hipStream_t stream) {
   NWGR_REQUIRE(t.engine <= kEngineAuto, "engine must be 0 (degree-binned), 1 (tiled) or 2 (auto)");
   NWGR_REQUIRE(t.tile_bits >= 13 && t.tile_bits <= kMaxTileBits, "tile_bits must be in [13, 15]");
   NWGR_REQUIRE(t.tile_fill_cost <= 65536, "tile_fill_cost must be <= 65536");
+  NWGR_REQUIRE(t.tile_combine <= kCombineAuto, "tile_combine must be 0 (gather), 1 (accumulate) or 2 (auto)");
   build_plan(g, g.fwd, g.in_deg_h, g.offs_h, {}, stream);
   if (g.has_t) build_plan(g, g.adj, g.t_deg_h, g.t_offs_h, g.t_rows_h, stream);
   // tiled layouts follow the tile width; their task ranges follow the fill cost

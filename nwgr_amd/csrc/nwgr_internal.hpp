@@ -170,6 +170,7 @@ struct SolveProfile {
 // Engine choice.  Auto takes the tiled engine for a CSR whose entries form
 // on average at least kTiledMinRun entries per (row, source tile) run ...
 constexpr uint32_t kEngineBinned = 0, kEngineTiled = 1, kEngineAuto = 2;
+constexpr uint32_t kCombineGather = 0, kCombineAccumulate = 1, kCombineAuto = 2;
 constexpr double kTiledMinRun = 2.0;
 // ... or for any CSR whose gathered vector outgrows one XCD's 4 MiB L2: the
 // degree-binned gathers then come from the MALL a cache line per entry, while
@@ -188,6 +189,7 @@ struct Tuning {
   uint32_t engine = kEngineAuto;   // kEngineBinned, kEngineTiled or kEngineAuto
   uint32_t tile_bits = 15;         // tiled engine: log2 of the source tile width (13 .. 15)
   uint32_t tile_fill_cost = 40;    // tiled engine: chunk-equivalents charged per tile fill
+  uint32_t tile_combine = kCombineAuto;  // tiled engine second pass: gather, accumulate or auto
 };
 
 // Work decomposition of one CSR orientation: segments over row positions and
@@ -257,7 +259,19 @@ struct Tiled {
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
   DevBuf wave_rows;  // u32[nwave_rows] rows with more than kWaveRowMin partials
   std::vector<uint32_t> tile_chunk0;  // host: first chunk of each tile (ntiles + 1)
+  // accumulate form of the second pass (built on first use): rows cut into
+  // blocks of at most kAccRows rows and about kAccFrags fragments; inside a
+  // tile, a block's fragments are consecutive ordinals (rows ascend)
+  bool acc_built = false;
+  bool acc_on = false;   // the second pass accumulates (else it gathers through rslot)
+  uint32_t nab = 0;      // row blocks
+  DevBuf ablo;           // u32[nab + 1] first row of each block
+  DevBuf aseg;           // u32[(nab + 1) * ntiles] first ordinal of block b's fragments in tile t
+  DevBuf afrow;          // u16[nsub] row of each fragment minus its block's first row
 };
+
+constexpr uint32_t kAccRows = 16384;       // rows per accumulate block (double accumulators in LDS)
+constexpr uint32_t kAccFrags = 1u << 18;   // fragments per accumulate block (cut earlier when exceeded)
 
 // Kinds of fixed-point solve sharing the sweep machinery.
 enum SolveKind : uint32_t {
@@ -407,6 +421,9 @@ void ensure_transpose(Graph& g, hipStream_t s);
 void replan(Graph& g, hipStream_t s);
 // tiled layout of a CSR over rows [0, n) (tiled.hip); no-op once built
 void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s);
+// row blocks and segment table of the accumulate form of the second pass
+// (tiled.hip); no-op once built
+void build_accum(Graph& g, Tiled& t, hipStream_t s);
 // cut the chunks of `t` into per-workgroup task ranges (tiled.hip)
 void plan_tasks(Graph& g, Tiled& t, hipStream_t s);
 

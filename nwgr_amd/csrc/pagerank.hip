@@ -880,6 +880,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   }
 }
 
+// The quiet rows' common rank (plain kind, rows [nrows, n) without entries):
+// the row epilogue with z = 0 (the same operations, so the value a stored row
+// without entries gets), computed alike by every workgroup of NT threads.
+// Their contributions are stored (one streaming pass); their ranks are not.
+// Workgroup 0 adds their residual and dangling mass as count x value.
+template <uint32_t KIND, uint32_t NT>
+__device__ __forceinline__ void quiet_rows(const SweepArgs& a, float Df, double& err, double& dang) {
+  if constexpr (KIND == kPlain) {
+    if (a.quiet) {
+      const float u = a.dangling == kDanglingUniform ? a.un : 0.f;
+      const float sq = fmaf(Df, u, 0.f);
+      const float rn = fmaf(a.d_hi, sq, fmaf(a.d_lo, sq, a.base_u));
+      for (uint32_t v = a.nrows + blockIdx.x * NT + threadIdx.x; v < a.n; v += gridDim.x * NT)
+        a.c_out[v] = rn / a.div[v];
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const float ro = a.st->rq[a.p ^ 1];
+        a.st->rq[a.p] = rn;
+        err += static_cast<double>(a.n - a.nrows) * fabs(static_cast<double>(rn) - static_cast<double>(ro));
+        dang += static_cast<double>(a.nq_dangling) * static_cast<double>(rn);
+      }
+    }
+  }
+}
+
 // Partials staged in LDS per workgroup step of kBlock rows.
 constexpr uint32_t kCombineStage = 4096;
 // Consecutive combine steps kept on one XCD.
@@ -982,31 +1006,115 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
       r0 += nfit;
     }
   }
-  if constexpr (KIND == kPlain) {
-    if (a.s.quiet) {
-      // The quiet rows' common rank: the row epilogue with z = 0 (the same
-      // operations, so the value a stored row without entries gets),
-      // computed alike by every workgroup.  Their contributions are stored
-      // (one streaming pass); their ranks are not.  Workgroup 0 adds their
-      // residual and dangling mass as count x value.
-      const float u = a.s.dangling == kDanglingUniform ? a.s.un : 0.f;
-      const float sq = fmaf(Df, u, 0.f);
-      const float rn = fmaf(a.s.d_hi, sq, fmaf(a.s.d_lo, sq, a.s.base_u));
-      for (uint32_t v = a.s.nrows + blockIdx.x * kBlock + tid; v < a.s.n; v += G * kBlock)
-        a.s.c_out[v] = rn / a.s.div[v];
-      if (blockIdx.x == 0 && tid == 0) {
-        const float ro = a.s.st->rq[a.s.p ^ 1];
-        a.s.st->rq[a.s.p] = rn;
-        err += static_cast<double>(a.s.n - a.s.nrows) * fabs(static_cast<double>(rn) - static_cast<double>(ro));
-        dang += static_cast<double>(a.s.nq_dangling) * static_cast<double>(rn);
-      }
-    }
-  }
+  quiet_rows<KIND, kBlock>(a.s, Df, err, dang);
   if constexpr (KIND != kCoef) {
     block_sum2(err, dang, s_red);
     if (tid == 0) {
       a.s.part_out[2 * blockIdx.x] = err;
       a.s.part_out[2 * blockIdx.x + 1] = dang;
+    }
+  }
+}
+
+// Accumulate form of the second pass: one workgroup per block of at most
+// kAccRows consecutive rows at a time.  Inside every source tile a block's
+// fragments are consecutive ordinals, so the workgroup streams its partials
+// tile by tile (a wavefront per tile, four loads in flight per lane) into a
+// double accumulator per row in LDS, then applies the row epilogue to the
+// block's rows.  No scattered reads: the partials and their 16-bit block-local
+// rows are read once, in order.
+struct AccArgs {
+  const uint32_t* __restrict__ blo;   // [nab + 1] first row of each block
+  const uint32_t* __restrict__ seg;   // [(nab + 1) * ntiles] first ordinal of block b in tile t
+  const uint16_t* __restrict__ frow;  // [nsub] row of each fragment in its block
+  uint32_t nab, ntiles;
+};
+
+template <uint32_t KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_accum(CombineArgs a, AccArgs q) {
+  constexpr uint32_t kNW = kSweepThreads / kWave;
+  extern __shared__ double s_acc[];  // kAccRows + kWave spare slots (dynamic)
+  __shared__ double s_red[2][kNW];
+  __shared__ uint32_t s_flag;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wl = tid & (kWave - 1);
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+  float Df = 0.f;
+  if constexpr (KIND != kCoef) {
+    if (tid == 0) s_flag = a.s.st->done;
+    __syncthreads();
+    if (s_flag) return;
+    Df = static_cast<float>(a.s.st->dsum);
+  }
+  double err = 0, dang = 0;
+  const uint32_t nrows = a.s.nrows;  // quiet rows [nrows, n) are not stored
+  for (uint32_t b = blockIdx.x; b < q.nab; b += gridDim.x) {
+    const uint32_t lo = q.blo[b];
+    if (lo >= nrows) break;  // blocks ascend
+    const uint32_t cnt = min(q.blo[b + 1], nrows) - lo;
+    for (uint32_t i = tid; i < cnt; i += kSweepThreads) s_acc[i] = 0.0;
+    __syncthreads();
+    const uint32_t* __restrict__ s0p = q.seg + static_cast<size_t>(b) * q.ntiles;
+    const uint32_t* __restrict__ s1p = s0p + q.ntiles;
+    // Wavefront wv takes tiles wv, wv + kNW, ...: in groups of 64, lane l
+    // loads the segment bounds of the group's l-th tile, and the wavefront
+    // walks them two tiles at a time (bounds broadcast with readlane), four
+    // loads per lane from each segment, all issued before the first add.
+    // Lanes past a segment's end read entry 0 and add it to their own spare
+    // slot after the block's rows, so every load and add is unconditional
+    // and no two lanes share a spare slot.
+    for (uint32_t g0 = 0; g0 + wv < q.ntiles; g0 += kNW * kWave) {
+      const uint32_t tl = g0 + wv + kNW * wl;
+      const uint32_t lb0 = tl < q.ntiles ? s0p[tl] : 0u, lb1 = tl < q.ntiles ? s1p[tl] : 0u;
+      const uint32_t ng = min(static_cast<uint32_t>(kWave), (q.ntiles - g0 - wv + kNW - 1) / kNW);
+      for (uint32_t i = 0; i < ng; i += 2) {
+        const uint32_t a0 = __builtin_amdgcn_readlane(lb0, i), a1 = __builtin_amdgcn_readlane(lb1, i);
+        const uint32_t c0 = __builtin_amdgcn_readlane(lb0, i + 1), c1 = __builtin_amdgcn_readlane(lb1, i + 1);
+        const uint32_t la = a1 - a0, lc = i + 1 < ng ? c1 - c0 : 0u;
+        const uint32_t len = max(la, lc);
+        for (uint32_t o = wl; o < len; o += 4 * kWave) {
+          float v[8];
+          uint32_t r[8];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t oj = o + j * kWave;
+            const uint32_t ka = oj < la ? a0 + oj : 0u, kc = oj < lc ? c0 + oj : 0u;
+            v[j] = a.partial[ka];
+            r[j] = oj < la ? static_cast<uint32_t>(q.frow[ka]) : kAccRows + wl;
+            v[4 + j] = a.partial[kc];
+            r[4 + j] = oj < lc ? static_cast<uint32_t>(q.frow[kc]) : kAccRows + wl;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) atomicAdd(&s_acc[r[j]], static_cast<double>(v[j]));
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cnt; i += kSweepThreads) {
+      const uint32_t row = lo + i;
+      const RowIn in = load_row<KIND>(a.s, row);
+      finish_row<KIND>(a.s, row, static_cast<float>(s_acc[i]), Df, in, err, dang);
+    }
+    __syncthreads();  // the next block clears the accumulators
+  }
+  quiet_rows<KIND, kSweepThreads>(a.s, Df, err, dang);
+  if constexpr (KIND != kCoef) {
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    if (wl == 0) {
+      s_red[0][wv] = err;
+      s_red[1][wv] = dang;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double te = 0, td = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < kNW; ++i) {
+        te += s_red[0][i];
+        td += s_red[1][i];
+      }
+      a.s.part_out[2 * blockIdx.x] = te;
+      a.s.part_out[2 * blockIdx.x + 1] = td;
     }
   }
 }
@@ -1160,6 +1268,45 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ca.row0 = t.row0;
   const size_t shm = sizeof(float) * ((size_t{1} << t.bits) + kSweepThreads / kWave * kStageW);
   const uint32_t nb = t.nblocks;
+  if (t.acc_on) {
+    static const bool acc_attrs = [] {
+      const int bytes = static_cast<int>((kAccRows + kWave) * sizeof(double));
+      for (const void* f : {reinterpret_cast<const void*>(&k_accum<kPlain>),
+                            reinterpret_cast<const void*>(&k_accum<kWeighted>),
+                            reinterpret_cast<const void*>(&k_accum<kAdjoint>),
+                            reinterpret_cast<const void*>(&k_accum<kDelta>),
+                            reinterpret_cast<const void*>(&k_accum<kCoef>)})
+        NWGR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+      return true;
+    }();
+    (void)acc_attrs;
+    AccArgs q{t.ablo.as<uint32_t>(), t.aseg.as<uint32_t>(), t.afrow.as<uint16_t>(), t.nab, t.ntiles};
+    const size_t ashm = sizeof(double) * (kAccRows + kWave);
+    switch (sp.kind) {
+      case kPlain:
+        k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
+        k_accum<kPlain><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
+        break;
+      case kWeighted:
+        k_tile<kWeighted><<<nb, kSweepThreads, shm, s>>>(ta);
+        k_accum<kWeighted><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
+        break;
+      case kAdjoint:
+        k_tile<kAdjoint><<<nb, kSweepThreads, shm, s>>>(ta);
+        k_accum<kAdjoint><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
+        break;
+      case kDelta:
+        k_tile<kDelta><<<nb, kSweepThreads, shm, s>>>(ta);
+        k_accum<kDelta><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
+        break;
+      default:
+        k_tile<kCoef><<<nb, kSweepThreads, shm, s>>>(ta);
+        k_accum<kCoef><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
+        break;
+    }
+    NWGR_HIP(hipGetLastError());
+    return;
+  }
   switch (sp.kind) {
     case kPlain:
       k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
@@ -1380,7 +1527,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   res.converged = (res.iters > 0 && res.error < p.tol) ? 1u : 0u;
   res.ms = ms;
   g.last_profile = prof;
-  g.last_engine = sp.tiled ? static_cast<int>(kEngineTiled) : static_cast<int>(kEngineBinned);
+  g.last_engine = !sp.tiled ? 0 : (sp.tiled->acc_on ? 2 : 1);
   g.last_errs.assign(std::min(res.iters, g.hist_cap), 0.0);
   if (!g.last_errs.empty())
     NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
@@ -1584,6 +1731,7 @@ int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t) {
     t->engine = u.engine;
     t->tile_bits = u.tile_bits;
     t->tile_fill_cost = u.tile_fill_cost;
+    t->tile_combine = u.tile_combine;
   });
 }
 
@@ -1602,6 +1750,7 @@ int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
     g->tune.engine = t->engine;
     g->tune.tile_bits = t->tile_bits;
     g->tune.tile_fill_cost = t->tile_fill_cost;
+    g->tune.tile_combine = t->tile_combine;
     try {
       nwgr::replan(*g, g->own_stream);
     } catch (...) {

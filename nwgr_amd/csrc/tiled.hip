@@ -155,6 +155,64 @@ __global__ void k_gather_w(const float* __restrict__ src, const uint32_t* __rest
   GRID_STRIDE(i, count) dst[i] = src[pos[i]];
 }
 
+// Accumulate form of the second pass.  Row of every fragment from the
+// row-ordered slot list (dummy fragments, which no row lists, keep row n).
+__global__ void k_frag_rows(const uint32_t* __restrict__ pair_ptr, const uint32_t* __restrict__ rslot, uint32_t n,
+                            uint32_t nslots, uint32_t* __restrict__ sprow) {
+  GRID_STRIDE(k, nslots) {
+    uint32_t lo = 0, hi = n;  // pair_ptr[lo] <= k < pair_ptr[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (pair_ptr[mid] <= k) lo = mid; else hi = mid;
+    }
+    sprow[rslot[k]] = lo;
+  }
+}
+
+// Row of every fragment relative to the first row of its block.
+__global__ void k_frag_local(const uint32_t* __restrict__ sprow, const uint32_t* __restrict__ blo, uint32_t nab,
+                             uint32_t nsub, uint32_t n, uint16_t* __restrict__ frow) {
+  GRID_STRIDE(k, nsub) {
+    const uint32_t r = sprow[k];
+    uint32_t v = 0;
+    if (r < n) {
+      uint32_t lo = 0, hi = nab;  // blo[lo] <= r < blo[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (blo[mid] <= r) lo = mid; else hi = mid;
+      }
+      v = r - blo[lo];
+    }
+    frow[k] = static_cast<uint16_t>(v);
+  }
+}
+
+// First fragment ordinal of every tile (a chunk starts a fragment), nsub
+// after the last; an empty tile gets its successor's.
+__global__ void k_tile_ord(const uint32_t* __restrict__ lane_sp0, const uint32_t* __restrict__ chunk0,
+                           uint32_t ntiles, uint32_t nchunks, uint32_t nsub, uint32_t* __restrict__ tord) {
+  GRID_STRIDE(t, static_cast<uint64_t>(ntiles) + 1) {
+    const uint32_t c = t < ntiles ? chunk0[t] : nchunks;
+    tord[t] = c < nchunks ? lane_sp0[static_cast<size_t>(c) * 64] : nsub;
+  }
+}
+
+// Segment table: the first ordinal of tile t whose fragment row is >= the
+// first row of block b (rows ascend inside a tile; dummies carry row n).
+__global__ void k_acc_seg(const uint32_t* __restrict__ sprow, const uint32_t* __restrict__ tord,
+                          const uint32_t* __restrict__ blo, uint32_t nab, uint32_t ntiles, uint32_t* __restrict__ seg) {
+  GRID_STRIDE(i, static_cast<uint64_t>(nab + 1) * ntiles) {
+    const uint32_t b = static_cast<uint32_t>(i / ntiles), t = static_cast<uint32_t>(i % ntiles);
+    const uint32_t key = blo[b];
+    uint32_t lo = tord[t], hi = tord[t + 1];
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (sprow[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    seg[i] = lo;
+  }
+}
+
 template <class Fn>
 void cub_call(Fn&& fn, hipStream_t s) {
   size_t bytes = 0;
@@ -426,6 +484,64 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   t.built = true;
 }
 
+void build_accum(Graph& g, Tiled& t, hipStream_t s) {
+  if (t.acc_built) return;
+  const uint32_t n = g.n, nt = t.ntiles, nsub = t.nsub;
+  std::vector<uint32_t> pp(static_cast<size_t>(n) + 1);
+  NWGR_HIP(hipMemcpyAsync(pp.data(), t.pair_ptr.p, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
+                          hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  // blocks of consecutive rows: at most kAccRows rows, cut before the
+  // fragments would pass kAccFrags (a row alone may exceed it)
+  std::vector<uint32_t> blo{0};
+  uint64_t frags = 0;
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t f = pp[r + 1] - pp[r];
+    if (r > blo.back() && (r - blo.back() >= kAccRows || frags + f > kAccFrags)) {
+      blo.push_back(r);
+      frags = 0;
+    }
+    frags += f;
+  }
+  blo.push_back(n);
+  t.nab = static_cast<uint32_t>(blo.size() - 1);
+  t.ablo.alloc(sizeof(uint32_t) * blo.size());
+  NWGR_HIP(hipMemcpyAsync(t.ablo.p, blo.data(), sizeof(uint32_t) * blo.size(), hipMemcpyHostToDevice, s));
+  DevBuf sprow, tord, chunk0;
+  sprow.alloc(sizeof(uint32_t) * std::max<uint32_t>(nsub, 1));
+  if (nsub) {
+    k_fill_u32<<<grid_for(nsub), kT, 0, s>>>(sprow.as<uint32_t>(), nsub, n);
+    NWGR_HIP(hipGetLastError());
+  }
+  if (pp[n]) {
+    k_frag_rows<<<grid_for(pp[n]), kT, 0, s>>>(t.pair_ptr.as<uint32_t>(), t.rslot.as<uint32_t>(), n, pp[n],
+                                                sprow.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  t.afrow.alloc(sizeof(uint16_t) * std::max<uint32_t>(nsub, 1));
+  if (nsub) {
+    k_frag_local<<<grid_for(nsub), kT, 0, s>>>(sprow.as<uint32_t>(), t.ablo.as<uint32_t>(), t.nab, nsub, n,
+                                                t.afrow.as<uint16_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  tord.alloc(sizeof(uint32_t) * (static_cast<size_t>(nt) + 1));
+  chunk0.alloc(sizeof(uint32_t) * (static_cast<size_t>(nt) + 1));
+  NWGR_HIP(hipMemcpyAsync(chunk0.p, t.tile_chunk0.data(), sizeof(uint32_t) * (static_cast<size_t>(nt) + 1),
+                          hipMemcpyHostToDevice, s));
+  k_tile_ord<<<grid_for(static_cast<uint64_t>(nt) + 1), kT, 0, s>>>(t.lane_sp0.as<uint32_t>(), chunk0.as<uint32_t>(),
+                                                                     nt, t.nchunks, nsub, tord.as<uint32_t>());
+  NWGR_HIP(hipGetLastError());
+  const uint64_t nseg = static_cast<uint64_t>(t.nab + 1) * nt;
+  t.aseg.alloc(sizeof(uint32_t) * std::max<uint64_t>(nseg, 1));
+  if (nseg) {
+    k_acc_seg<<<grid_for(nseg), kT, 0, s>>>(sprow.as<uint32_t>(), tord.as<uint32_t>(), t.ablo.as<uint32_t>(), t.nab,
+                                             nt, t.aseg.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+  }
+  NWGR_HIP(hipStreamSynchronize(s));
+  t.acc_built = true;
+}
+
 void tiled_weights(Graph& g, Tiled& t, const float* weights, hipStream_t s) {
   const uint64_t mp = static_cast<uint64_t>(t.nchunks) * kChunkEdges;
   if (!t.w.p || t.w.bytes < sizeof(float) * std::max<uint64_t>(mp, 1)) t.w.alloc(sizeof(float) * std::max<uint64_t>(mp, 1));
@@ -466,6 +582,15 @@ Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
   }
   build_tiled(g, t, offs, idx, s);
   if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
+  // second pass: accumulate when rows have about one entry per tile run
+  const uint32_t mode = g.tune.tile_combine;
+  const bool acc = mode == kCombineAccumulate ||
+                   (mode == kCombineAuto && 2 * static_cast<uint64_t>(g.m) < 3 * static_cast<uint64_t>(t.nsub));
+  if (acc) build_accum(g, t, s);
+  if (acc != t.acc_on) {
+    t.acc_on = acc;
+    g.drop_graphs();
+  }
   return &t;
 }
 

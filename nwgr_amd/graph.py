@@ -142,6 +142,32 @@ def transpose(graph: CSRGraph, nthreads: int = 0):
     return off, ind[:m], perm[:m]
 
 
+def row_cuts(graph: CSRGraph, parts: int) -> np.ndarray:
+    """Boundaries (parts + 1 row ids) of `parts` contiguous slices of the
+    destination rows with about equal entry counts: the 1-D row partition a
+    multi-GPU solve would give each GPU."""
+    if parts < 1:
+        raise ValueError("parts must be >= 1")
+    targets = (np.arange(1, parts, dtype=np.int64) * graph.m) // parts
+    cuts = np.searchsorted(graph.offsets.astype(np.int64), targets, side="left")
+    return np.concatenate([[0], cuts, [graph.n]]).astype(np.int64)
+
+
+def row_shard(graph: CSRGraph, lo: int, hi: int) -> CSRGraph:
+    """One GPU's share of a row-partitioned solve: the same n vertices (sources
+    keep their ids, so the gathered vector is the full one) with entries only
+    in rows [lo, hi).  A sweep over it computes rows [lo, hi) of the full
+    graph's sweep from the same input vector."""
+    if not 0 <= lo <= hi <= graph.n:
+        raise ValueError("need 0 <= lo <= hi <= n")
+    offs = graph.offsets.astype(np.int64)
+    new = np.empty(graph.n + 1, np.int64)
+    new[:lo + 1] = 0
+    new[lo:hi + 1] = offs[lo:hi + 1] - offs[lo]
+    new[hi + 1:] = offs[hi] - offs[lo]
+    return CSRGraph(new, graph.indices[offs[lo]:offs[hi]])
+
+
 def splitmix64_at(key: int, k: int) -> int:
     return int(graphgen_lib().nwgr_splitmix64_at(key, k))
 

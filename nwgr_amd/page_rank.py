@@ -132,9 +132,11 @@ class PageRankGraph:
         return {"sweep_ms": ms.value, "launches": launches.value, "chunks": chunks.value}
 
     def last_engine(self) -> str | None:
-        """Engine of the last solve: "binned", "tiled", or None before any."""
+        """Engine of the last solve: "binned", "tiled" (second pass gathering
+        each row's partials), "tiled-accumulate" (second pass accumulating
+        row blocks in LDS), or None before any."""
         e = pagerank_lib().nwgr_pr_last_engine(self._h)
-        return {0: "binned", 1: "tiled"}.get(e)
+        return {0: "binned", 1: "tiled", 2: "tiled-accumulate"}.get(e)
 
     def last_errors(self) -> np.ndarray:
         buf = np.zeros(4096, np.float64)
