@@ -265,6 +265,7 @@ struct Tiled {
   bool acc_built = false;
   bool acc_on = false;   // the second pass accumulates (else it gathers through rslot)
   uint32_t nab = 0;      // row blocks
+  uint32_t acc_rows = 0; // rows per block at most (kAccRows; the kernel also takes kAccRows / 2)
   DevBuf ablo;           // u32[nab + 1] first row of each block
   DevBuf aseg;           // u32[(nab + 1) * ntiles] first ordinal of block b's fragments in tile t
   DevBuf afrow;          // u16[nsub] row of each fragment minus its block's first row

@@ -1028,12 +1028,60 @@ struct AccArgs {
   const uint32_t* __restrict__ seg;   // [(nab + 1) * ntiles] first ordinal of block b in tile t
   const uint16_t* __restrict__ frow;  // [nsub] row of each fragment in its block
   uint32_t nab, ntiles;
+  uint32_t cap;                       // rows per block at most (accumulators before the spare slots)
 };
+
+// Adds one block's partials into the LDS accumulators.  Wavefront wv takes
+// tiles wv, wv + kNW, ...: in groups of 64, lane l loads the segment bounds of
+// the group's l-th tile, and the wavefront walks them P tiles at a time
+// (bounds broadcast with readlane), U loads per lane from each segment, all
+// issued before the first add.  Lanes past a segment's end read entry 0 and
+// add it to their own spare slot after the block's rows (`spare` + lane), so
+// every load and add is unconditional and no two lanes share a spare slot.
+template <int P, int U>
+__device__ __forceinline__ void acc_tiles(const float* __restrict__ partial, const uint16_t* __restrict__ frow,
+                                          double* s_acc, const uint32_t* __restrict__ s0p,
+                                          const uint32_t* __restrict__ s1p, uint32_t ntiles, uint32_t wv, uint32_t wl,
+                                          uint32_t spare) {
+  constexpr uint32_t kNW = kSweepThreads / kWave;
+  for (uint32_t g0 = 0; g0 + wv < ntiles; g0 += kNW * kWave) {
+    const uint32_t tl = g0 + wv + kNW * wl;
+    const uint32_t lb0 = tl < ntiles ? s0p[tl] : 0u, lb1 = tl < ntiles ? s1p[tl] : 0u;
+    const uint32_t ng = min(static_cast<uint32_t>(kWave), (ntiles - g0 - wv + kNW - 1) / kNW);
+    for (uint32_t i = 0; i < ng; i += P) {
+      uint32_t b0[P], ln[P];
+      uint32_t len = 0;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const uint32_t idx = min(i + p, static_cast<uint32_t>(kWave - 1));
+        b0[p] = __builtin_amdgcn_readlane(lb0, idx);
+        const uint32_t e = __builtin_amdgcn_readlane(lb1, idx);
+        ln[p] = i + p < ng ? e - b0[p] : 0u;
+        len = max(len, ln[p]);
+      }
+      for (uint32_t o = wl; o < len; o += U * kWave) {
+        float v[P * U];
+        uint32_t r[P * U];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+          for (int j = 0; j < U; ++j) {
+            const uint32_t oj = o + j * kWave;
+            const uint32_t k = oj < ln[p] ? b0[p] + oj : 0u;
+            v[p * U + j] = partial[k];
+            r[p * U + j] = oj < ln[p] ? static_cast<uint32_t>(frow[k]) : spare + wl;
+          }
+#pragma unroll
+        for (int x = 0; x < P * U; ++x) atomicAdd(&s_acc[r[x]], static_cast<double>(v[x]));
+      }
+    }
+  }
+}
 
 template <uint32_t KIND>
 __global__ __launch_bounds__(kSweepThreads) void k_accum(CombineArgs a, AccArgs q) {
   constexpr uint32_t kNW = kSweepThreads / kWave;
-  extern __shared__ double s_acc[];  // kAccRows + kWave spare slots (dynamic)
+  extern __shared__ double s_acc[];  // q.cap accumulators + kWave spare slots (dynamic)
   __shared__ double s_red[2][kNW];
   __shared__ uint32_t s_flag;
   const uint32_t tid = threadIdx.x;
@@ -1056,39 +1104,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_accum(CombineArgs a, AccArgs 
     __syncthreads();
     const uint32_t* __restrict__ s0p = q.seg + static_cast<size_t>(b) * q.ntiles;
     const uint32_t* __restrict__ s1p = s0p + q.ntiles;
-    // Wavefront wv takes tiles wv, wv + kNW, ...: in groups of 64, lane l
-    // loads the segment bounds of the group's l-th tile, and the wavefront
-    // walks them two tiles at a time (bounds broadcast with readlane), four
-    // loads per lane from each segment, all issued before the first add.
-    // Lanes past a segment's end read entry 0 and add it to their own spare
-    // slot after the block's rows, so every load and add is unconditional
-    // and no two lanes share a spare slot.
-    for (uint32_t g0 = 0; g0 + wv < q.ntiles; g0 += kNW * kWave) {
-      const uint32_t tl = g0 + wv + kNW * wl;
-      const uint32_t lb0 = tl < q.ntiles ? s0p[tl] : 0u, lb1 = tl < q.ntiles ? s1p[tl] : 0u;
-      const uint32_t ng = min(static_cast<uint32_t>(kWave), (q.ntiles - g0 - wv + kNW - 1) / kNW);
-      for (uint32_t i = 0; i < ng; i += 2) {
-        const uint32_t a0 = __builtin_amdgcn_readlane(lb0, i), a1 = __builtin_amdgcn_readlane(lb1, i);
-        const uint32_t c0 = __builtin_amdgcn_readlane(lb0, i + 1), c1 = __builtin_amdgcn_readlane(lb1, i + 1);
-        const uint32_t la = a1 - a0, lc = i + 1 < ng ? c1 - c0 : 0u;
-        const uint32_t len = max(la, lc);
-        for (uint32_t o = wl; o < len; o += 4 * kWave) {
-          float v[8];
-          uint32_t r[8];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const uint32_t oj = o + j * kWave;
-            const uint32_t ka = oj < la ? a0 + oj : 0u, kc = oj < lc ? c0 + oj : 0u;
-            v[j] = a.partial[ka];
-            r[j] = oj < la ? static_cast<uint32_t>(q.frow[ka]) : kAccRows + wl;
-            v[4 + j] = a.partial[kc];
-            r[4 + j] = oj < lc ? static_cast<uint32_t>(q.frow[kc]) : kAccRows + wl;
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) atomicAdd(&s_acc[r[j]], static_cast<double>(v[j]));
-        }
-      }
-    }
+    if (q.cap > kAccRows / 2)
+      acc_tiles<2, 4>(a.partial, q.frow, s_acc, s0p, s1p, q.ntiles, wv, wl, q.cap);
+    else
+      acc_tiles<4, 2>(a.partial, q.frow, s_acc, s0p, s1p, q.ntiles, wv, wl, q.cap);
     __syncthreads();
     for (uint32_t i = tid; i < cnt; i += kSweepThreads) {
       const uint32_t row = lo + i;
@@ -1280,8 +1299,8 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
       return true;
     }();
     (void)acc_attrs;
-    AccArgs q{t.ablo.as<uint32_t>(), t.aseg.as<uint32_t>(), t.afrow.as<uint16_t>(), t.nab, t.ntiles};
-    const size_t ashm = sizeof(double) * (kAccRows + kWave);
+    AccArgs q{t.ablo.as<uint32_t>(), t.aseg.as<uint32_t>(), t.afrow.as<uint16_t>(), t.nab, t.ntiles, t.acc_rows};
+    const size_t ashm = sizeof(double) * (t.acc_rows + kWave);
     switch (sp.kind) {
       case kPlain:
         k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);

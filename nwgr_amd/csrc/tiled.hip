@@ -493,11 +493,12 @@ void build_accum(Graph& g, Tiled& t, hipStream_t s) {
   NWGR_HIP(hipStreamSynchronize(s));
   // blocks of consecutive rows: at most kAccRows rows, cut before the
   // fragments would pass kAccFrags (a row alone may exceed it)
+  t.acc_rows = kAccRows;
   std::vector<uint32_t> blo{0};
   uint64_t frags = 0;
   for (uint32_t r = 0; r < n; ++r) {
     const uint32_t f = pp[r + 1] - pp[r];
-    if (r > blo.back() && (r - blo.back() >= kAccRows || frags + f > kAccFrags)) {
+    if (r > blo.back() && (r - blo.back() >= t.acc_rows || frags + f > kAccFrags)) {
       blo.push_back(r);
       frags = 0;
     }
