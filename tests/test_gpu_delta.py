@@ -56,7 +56,7 @@ def test_delta_rmat_parity(ename, thr):
     pg = PageRankGraph(g, deg)
     pg.tuning = ENGINES[ename]
     delta_parity(g, deg, thr, 100, pg)
-    assert pg.last_engine() == ("binned" if ename == "binned" else "tiled")
+    assert pg.last_engine() == "binned" if ename == "binned" else pg.last_engine().startswith("tiled")
 
 
 @pytest.mark.parametrize("ename", ["binned", "tiled15"])

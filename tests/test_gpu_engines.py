@@ -47,7 +47,7 @@ def test_engine_parity(gname, ename):
     pg.tuning = ENGINES[ename]
     for mode in ("uniform", "leak"):
         parity(g, deg, mode, 1e-6, 100, device_graph=pg)
-        assert pg.last_engine() == ("binned" if ename == "binned" else "tiled")
+        assert pg.last_engine() == "binned" if ename == "binned" else pg.last_engine().startswith("tiled")
 
 
 def test_engines_agree_and_switch_back_and_forth():
@@ -120,7 +120,7 @@ def test_weighted_adjoint_per_engine(ename):
     wd = torch.from_numpy(w).to(dev).requires_grad_(True)
     td = torch.from_numpy(t).to(dev).requires_grad_(True)
     r = weighted_pagerank(pg, wd, td, tol=1e-7, max_iters=300, dangling="teleport", adj_tol=1e-7)
-    assert pg.last_engine() == ("binned" if ename == "binned" else "tiled")
+    assert pg.last_engine() == "binned" if ename == "binned" else pg.last_engine().startswith("tiled")
     (r * torch.from_numpy(wl).to(dev)).sum().backward()
     w64, t64, wl64 = w.astype(np.float64), t.astype(np.float64), wl.astype(np.float64)
     r64, _ = oracle.wpr_forward(g.offsets, g.indices, w64, t64, 0.85, 1e-15, 5000, oracle.TELEPORT, nthreads=8)
