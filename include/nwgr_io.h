@@ -44,13 +44,30 @@ enum { NWGR_UNDIRECTED = 0, NWGR_DIRECTED = 1 }; /* the reference's directedness
  * Returns 0, or non-zero with a message from nwgr_gen_last_error(). */
 int nwgr_read_mm(const char* path, int nthreads, nwgr_edge_list** out);
 
-/* "NW Graph index_edge_list" binary (unipartite, no attributes) -> edge list. */
+/* Same with the reference's other reader forms: directedness
+ * NWGR_UNDIRECTED reads every entry once (read_mm<undirected>, mm_fill
+ * :130-170), NWGR_DIRECTED as nwgr_read_mm; flags NWGR_MM_INT_ATTRIBUTE also
+ * keeps an int attribute per edge as read_mm<..., int> reads it (the sign and
+ * digits the value field starts with, 1 in a pattern file), which
+ * bench/ProcessEdgelist.cpp:37-50 writes out. */
+enum { NWGR_MM_INT_ATTRIBUTE = 1 };
+int nwgr_read_mm_as(const char* path, int nthreads, int directedness, int flags, nwgr_edge_list** out);
+
+/* "NW Graph index_edge_list" binary (unipartite) -> edge list.  The first two
+ * arrays are the ids; a third array of 4-byte values is kept as the int
+ * attribute, further arrays are skipped (the reference reads the arrays its
+ * own type has, containers/soa.hpp:302-309). */
 int nwgr_read_nw_edge_list(const char* path, nwgr_edge_list** out);
 
 /* Edge list -> "NW Graph index_edge_list" binary (unipartite, no
  * attributes); directedness NWGR_DIRECTED or NWGR_UNDIRECTED. */
 int nwgr_write_nw_edge_list(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                             int directedness);
+/* Same with an int attribute array after the ids (attr[m], or NULL for none):
+ * the layout of edge_list<directedness, int>::serialize, the output of
+ * bench/ProcessEdgelist.cpp. */
+int nwgr_write_nw_edge_list_attr(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                                 const int32_t* attr, int directedness);
 
 /* In-CSR (offsets n+1, indices m) -> "NW GRAPH indexed_struct_of_arrays"
  * binary, and back (the arrays are returned in a host CSR freed with
@@ -66,6 +83,7 @@ int nwgr_edge_list_directedness(const nwgr_edge_list* e);
 const uint32_t* nwgr_edge_list_src(const nwgr_edge_list* e);
 const uint32_t* nwgr_edge_list_dst(const nwgr_edge_list* e);
 const double* nwgr_edge_list_values(const nwgr_edge_list* e); /* NULL when the file has none */
+const int32_t* nwgr_edge_list_int_values(const nwgr_edge_list* e); /* NULL unless kept (see above) */
 void nwgr_edge_list_free(nwgr_edge_list* e);
 
 #ifdef __cplusplus

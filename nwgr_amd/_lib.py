@@ -205,10 +205,14 @@ def _declare_graphgen(L):
     cp = ctypes.c_char_p
     L.nwgr_read_mm.restype = i32
     L.nwgr_read_mm.argtypes = [cp, i32, P(vp)]
+    L.nwgr_read_mm_as.restype = i32
+    L.nwgr_read_mm_as.argtypes = [cp, i32, i32, i32, P(vp)]
     L.nwgr_read_nw_edge_list.restype = i32
     L.nwgr_read_nw_edge_list.argtypes = [cp, P(vp)]
     L.nwgr_write_nw_edge_list.restype = i32
     L.nwgr_write_nw_edge_list.argtypes = [cp, u32, u64, vp, vp, i32]
+    L.nwgr_write_nw_edge_list_attr.restype = i32
+    L.nwgr_write_nw_edge_list_attr.argtypes = [cp, u32, u64, vp, vp, vp, i32]
     L.nwgr_write_nw_adjacency.restype = i32
     L.nwgr_write_nw_adjacency.argtypes = [cp, u32, u64, vp, vp]
     L.nwgr_read_nw_adjacency.restype = i32
@@ -225,6 +229,8 @@ def _declare_graphgen(L):
         fn.argtypes = [vp]
     L.nwgr_edge_list_values.restype = ctypes.POINTER(ctypes.c_double)
     L.nwgr_edge_list_values.argtypes = [vp]
+    L.nwgr_edge_list_int_values.restype = ctypes.POINTER(ctypes.c_int32)
+    L.nwgr_edge_list_int_values.argtypes = [vp]
     L.nwgr_edge_list_free.restype = None
     L.nwgr_edge_list_free.argtypes = [vp]
 

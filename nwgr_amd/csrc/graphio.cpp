@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +32,8 @@ struct nwgr_edge_list {
   std::vector<uint32_t> src, dst;
   std::vector<double> values;
   bool has_values = false;
+  std::vector<int32_t> ivalues;  // integer attribute (read_mm<..., int>, or the third array of an NW file)
+  bool has_ivalues = false;
 };
 
 namespace {
@@ -88,13 +91,27 @@ inline bool scan_u64(const char*& p, const char* end, uint64_t& v) {
 struct Part {
   std::vector<uint32_t> src, dst;
   std::vector<double> val;
+  std::vector<int32_t> ival;
   uint64_t max_id = 0;
   uint64_t lines = 0;
   std::string err;
 };
 
-// Parses the entry lines of s[b, e) (whole lines).
-void parse_range(const std::string& s, size_t b, size_t e, bool symmetric, bool pattern, Part& out) {
+// The integer attribute `stringstream >> int` reads from a value field: an
+// optional sign and the digits after it (0 when there are none, clamped to
+// the int range), 1 for a pattern file (mm_fill's `T v(1.0)`, io/mmio.hpp:109).
+inline int32_t int_prefix(const char* q) {
+  char* stop = nullptr;
+  errno = 0;
+  const long long v = std::strtoll(q, &stop, 10);
+  if (stop == q) return 0;
+  return static_cast<int32_t>(std::clamp<long long>(v, INT32_MIN, INT32_MAX));
+}
+
+// Parses the entry lines of s[b, e) (whole lines).  `mirror`: a symmetric
+// file's off-diagonal entry also gives the reversed edge (directed lists);
+// `ints`: also keep the integer attribute of every entry.
+void parse_range(const std::string& s, size_t b, size_t e, bool mirror, bool pattern, bool ints, Part& out) {
   const char* p = s.data() + b;
   const char* end = s.data() + e;
   std::string tmp;
@@ -117,6 +134,7 @@ void parse_range(const std::string& s, size_t b, size_t e, bool symmetric, bool 
       return;
     }
     double v = 1.0;
+    int32_t iv = 1;
     if (!pattern) {
       while (q < eol && is_space(*q)) ++q;
       tmp.assign(q, static_cast<size_t>(eol - q));
@@ -127,14 +145,17 @@ void parse_range(const std::string& s, size_t b, size_t e, bool symmetric, bool 
         out.err = "MatrixMarket entry without a value: " + std::string(p, static_cast<size_t>(eol - p));
         return;
       }
+      if (ints) iv = int_prefix(tmp.c_str());
     }
     out.src.push_back(static_cast<uint32_t>(i - 1));
     out.dst.push_back(static_cast<uint32_t>(j - 1));
     if (!pattern) out.val.push_back(v);
-    if (symmetric && i != j) {
+    if (ints) out.ival.push_back(iv);
+    if (mirror && i != j) {
       out.src.push_back(static_cast<uint32_t>(j - 1));
       out.dst.push_back(static_cast<uint32_t>(i - 1));
       if (!pattern) out.val.push_back(v);
+      if (ints) out.ival.push_back(iv);
     }
     out.max_id = std::max({out.max_id, i, j});
     ++out.lines;
@@ -147,7 +168,14 @@ void parse_range(const std::string& s, size_t b, size_t e, bool symmetric, bool 
 extern "C" {
 
 int nwgr_read_mm(const char* path, int nthreads, nwgr_edge_list** out) {
+  return nwgr_read_mm_as(path, nthreads, NWGR_DIRECTED, 0, out);
+}
+
+int nwgr_read_mm_as(const char* path, int nthreads, int directedness, int flags, nwgr_edge_list** out) {
   if (!path || !out) return fail("null argument");
+  if (directedness != NWGR_DIRECTED && directedness != NWGR_UNDIRECTED) return fail("bad directedness");
+  if (flags & ~NWGR_MM_INT_ATTRIBUTE) return fail("unknown read_mm flags");
+  const bool ints = (flags & NWGR_MM_INT_ATTRIBUTE) != 0;
   *out = nullptr;
   try {
     std::string s;
@@ -201,7 +229,9 @@ int nwgr_read_mm(const char* path, int nthreads, nwgr_edge_list** out) {
     std::vector<Part> parts(static_cast<size_t>(T));
     std::vector<std::thread> th;
     for (int t = 0; t < T; ++t)
-      th.emplace_back([&, t] { parse_range(s, cut[t], cut[t + 1], symmetric, pattern, parts[t]); });
+      th.emplace_back([&, t] {
+        parse_range(s, cut[t], cut[t + 1], symmetric && directedness == NWGR_DIRECTED, pattern, ints, parts[t]);
+      });
     for (auto& x : th) x.join();
     uint64_t lines = 0, m = 0, max_id = 0;
     for (auto& pt : parts) {
@@ -216,15 +246,18 @@ int nwgr_read_mm(const char* path, int nthreads, nwgr_edge_list** out) {
     if (m > 0xffffffffULL) return fail("edge count exceeds the uint32 offset range");
     auto el = std::make_unique<nwgr_edge_list>();
     el->n = static_cast<uint32_t>(std::max<uint64_t>(n0, max_id));  // ids are 1-based: max_id = largest id + 1
-    el->directedness = NWGR_DIRECTED;
+    el->directedness = directedness;
     el->has_values = !pattern;
+    el->has_ivalues = ints;
     el->src.reserve(m);
     el->dst.reserve(m);
     if (!pattern) el->values.reserve(m);
+    if (ints) el->ivalues.reserve(m);
     for (auto& pt : parts) {
       el->src.insert(el->src.end(), pt.src.begin(), pt.src.end());
       el->dst.insert(el->dst.end(), pt.dst.begin(), pt.dst.end());
       if (!pattern) el->values.insert(el->values.end(), pt.val.begin(), pt.val.end());
+      if (ints) el->ivalues.insert(el->ivalues.end(), pt.ival.begin(), pt.ival.end());
     }
     (void)n1;
     *out = el.release();
@@ -236,12 +269,17 @@ int nwgr_read_mm(const char* path, int nthreads, nwgr_edge_list** out) {
 
 int nwgr_write_nw_edge_list(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                             int directedness) {
+  return nwgr_write_nw_edge_list_attr(path, n, m, src, dst, nullptr, directedness);
+}
+
+int nwgr_write_nw_edge_list_attr(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                                 const int32_t* attr, int directedness) {
   if (!path || (m && (!src || !dst))) return fail("null argument");
   if (directedness != NWGR_DIRECTED && directedness != NWGR_UNDIRECTED) return fail("bad directedness");
   std::ofstream f(path, std::ios::binary);
   if (!f) return fail(std::string("cannot write ") + path);
   const int32_t d = directedness;
-  const uint64_t N = n, st = m, arrays = 2, el = sizeof(uint32_t);
+  const uint64_t N = n, st = m, arrays = attr ? 3 : 2, el = sizeof(uint32_t);
   const bool is_open = false;
   f.write(kEdgeListMagic, sizeof(kEdgeListMagic));
   f.write(reinterpret_cast<const char*>(&d), sizeof(d));
@@ -249,10 +287,11 @@ int nwgr_write_nw_edge_list(const char* path, uint32_t n, uint64_t m, const uint
   f.write(reinterpret_cast<const char*>(&is_open), sizeof(is_open));
   f.write(reinterpret_cast<const char*>(&st), sizeof(st));
   f.write(reinterpret_cast<const char*>(&arrays), sizeof(arrays));
-  for (const uint32_t* a : {src, dst}) {
+  for (const void* a : {static_cast<const void*>(src), static_cast<const void*>(dst), static_cast<const void*>(attr)}) {
+    if (!a) continue;
     f.write(reinterpret_cast<const char*>(&st), sizeof(st));
     f.write(reinterpret_cast<const char*>(&el), sizeof(el));
-    if (m) f.write(reinterpret_cast<const char*>(a), static_cast<std::streamsize>(m * sizeof(uint32_t)));
+    if (m) f.write(static_cast<const char*>(a), static_cast<std::streamsize>(m * sizeof(uint32_t)));
   }
   return f ? 0 : fail(std::string("write failed: ") + path);
 }
@@ -274,7 +313,9 @@ int nwgr_read_nw_edge_list(const char* path, nwgr_edge_list** out) {
     f.read(reinterpret_cast<char*>(&is_open), sizeof(is_open));
     f.read(reinterpret_cast<char*>(&st), sizeof(st));
     f.read(reinterpret_cast<char*>(&arrays), sizeof(arrays));
-    if (!f || arrays != 2) return fail("NW edge list: only unipartite lists without attributes are supported");
+    // the reference reads as many arrays as its own edge list type has and
+    // ignores the count (soa.hpp:302-309): ids first, then attributes
+    if (!f || arrays < 2) return fail("NW edge list: fewer than two arrays");
     if (N > 0xffffffffULL || st > 0xffffffffULL) return fail("NW edge list exceeds the uint32 id/offset range");
     auto el = std::make_unique<nwgr_edge_list>();
     el->n = static_cast<uint32_t>(N);
@@ -286,6 +327,22 @@ int nwgr_read_nw_edge_list(const char* path, nwgr_edge_list** out) {
       if (!f || len != st || esz != sizeof(uint32_t)) return fail("NW edge list: unexpected array header");
       a->resize(len);
       if (len) f.read(reinterpret_cast<char*>(a->data()), static_cast<std::streamsize>(len * esz));
+      if (!f) return fail("NW edge list: file truncated");
+    }
+    if (arrays >= 3) {
+      // a 4-byte attribute (process_edge_list writes int weights) is kept;
+      // other attribute arrays are skipped
+      uint64_t len = 0, esz = 0;
+      f.read(reinterpret_cast<char*>(&len), sizeof(len));
+      f.read(reinterpret_cast<char*>(&esz), sizeof(esz));
+      if (!f || len != st) return fail("NW edge list: unexpected attribute array header");
+      if (esz == sizeof(int32_t)) {
+        el->ivalues.resize(len);
+        if (len) f.read(reinterpret_cast<char*>(el->ivalues.data()), static_cast<std::streamsize>(len * esz));
+        el->has_ivalues = true;
+      } else {
+        f.seekg(static_cast<std::streamoff>(len * esz), std::ios::cur);
+      }
       if (!f) return fail("NW edge list: file truncated");
     }
     for (uint64_t k = 0; k < st; ++k)
@@ -371,6 +428,9 @@ const uint32_t* nwgr_edge_list_src(const nwgr_edge_list* e) { return e ? e->src.
 const uint32_t* nwgr_edge_list_dst(const nwgr_edge_list* e) { return e ? e->dst.data() : nullptr; }
 const double* nwgr_edge_list_values(const nwgr_edge_list* e) {
   return e && e->has_values ? e->values.data() : nullptr;
+}
+const int32_t* nwgr_edge_list_int_values(const nwgr_edge_list* e) {
+  return e && e->has_ivalues ? e->ivalues.data() : nullptr;
 }
 void nwgr_edge_list_free(nwgr_edge_list* e) { delete e; }
 

@@ -178,14 +178,16 @@ def splitmix64_at(key: int, k: int) -> int:
 
 @dataclass
 class EdgeList:
-    """Directed edge list src[i] -> dst[i] over n vertices, with the file's
-    values when it has them (a MatrixMarket pattern file has none)."""
+    """Edge list src[i] -> dst[i] over n vertices, with the file's values when
+    it has them (a MatrixMarket pattern file has none) and, when asked for or
+    stored in an NW file, an int attribute per edge (`ints`)."""
 
     n: int
     src: np.ndarray
     dst: np.ndarray
     values: np.ndarray | None = None
     directed: bool = True
+    ints: np.ndarray | None = None
 
     @property
     def m(self) -> int:
@@ -200,17 +202,22 @@ def _take_edge_list(h) -> EdgeList:
         dst = np.ctypeslib.as_array(L.nwgr_edge_list_dst(h), (m,)).copy() if m else np.zeros(0, np.uint32)
         vp = L.nwgr_edge_list_values(h)
         vals = np.ctypeslib.as_array(vp, (m,)).copy() if (vp and m) else None
-        return EdgeList(n, src, dst, vals, L.nwgr_edge_list_directedness(h) == 1)
+        ip = L.nwgr_edge_list_int_values(h)
+        ints = (np.ctypeslib.as_array(ip, (m,)).copy() if m else np.zeros(0, np.int32)) if ip else None
+        return EdgeList(n, src, dst, vals, L.nwgr_edge_list_directedness(h) == 1, ints)
     finally:
         L.nwgr_edge_list_free(h)
 
 
-def read_mm(path: str, nthreads: int = 0) -> EdgeList:
-    """MatrixMarket coordinate file -> directed edge list, as read_mm<directed>
+def read_mm(path: str, nthreads: int = 0, directed: bool = True, int_attribute: bool = False) -> EdgeList:
+    """MatrixMarket coordinate file -> edge list, as read_mm<directed>
     (include/nwgraph/io/mmio.hpp:171-217): 1-based entries, a symmetric file's
-    off-diagonal entries in both directions, n = max(rows, largest id)."""
+    off-diagonal entries in both directions, n = max(rows, largest id).
+    directed=False reads every entry once (read_mm<undirected>);
+    int_attribute=True also keeps the int attribute read_mm<..., int> reads."""
     h = ctypes.c_void_p()
-    _check_gen(graphgen_lib().nwgr_read_mm(str(path).encode(), nthreads, ctypes.byref(h)))
+    _check_gen(graphgen_lib().nwgr_read_mm_as(str(path).encode(), nthreads, 1 if directed else 0,
+                                              1 if int_attribute else 0, ctypes.byref(h)))
     return _take_edge_list(h)
 
 
@@ -222,12 +229,19 @@ def read_nw_edge_list(path: str) -> EdgeList:
     return _take_edge_list(h)
 
 
-def write_nw_edge_list(path: str, el: EdgeList) -> None:
-    """index_edge_list::serialize (include/nwgraph/edge_list.hpp:197-203)."""
+def write_nw_edge_list(path: str, el: EdgeList, with_ints: bool = False) -> None:
+    """index_edge_list::serialize (include/nwgraph/edge_list.hpp:197-203);
+    with_ints writes el.ints as a third array (edge_list<..., int>)."""
     src = np.ascontiguousarray(el.src, dtype=np.uint32)
     dst = np.ascontiguousarray(el.dst, dtype=np.uint32)
-    _check_gen(graphgen_lib().nwgr_write_nw_edge_list(str(path).encode(), el.n, len(src), src.ctypes.data,
-                                                       dst.ctypes.data, 1 if el.directed else 0))
+    attr = None
+    if with_ints:
+        if el.ints is None or len(el.ints) != len(src):
+            raise ValueError("with_ints needs one int attribute per edge")
+        attr = np.ascontiguousarray(el.ints, dtype=np.int32)
+    _check_gen(graphgen_lib().nwgr_write_nw_edge_list_attr(str(path).encode(), el.n, len(src), src.ctypes.data,
+                                                            dst.ctypes.data, None if attr is None else attr.ctypes.data,
+                                                            1 if el.directed else 0))
 
 
 def write_nw_adjacency(path: str, graph: CSRGraph) -> None:

@@ -146,3 +146,84 @@ def test_nw_adjacency_round_trip_and_layout(tmp_path):
     p.write_bytes(raw[:-4])  # truncated
     with pytest.raises(ValueError):
         nwgr_amd.read_nw_adjacency(str(p))
+
+
+def _int_prefix(field):
+    """What `stringstream >> int` takes from a value field (0 if no digits)."""
+    import re
+    m = re.match(r"\s*([+-]?\d+)", field)
+    return int(m.group(1)) if m else 0
+
+
+def test_read_mm_undirected_and_int_attribute(tmp_path):
+    text = ("%%MatrixMarket matrix coordinate real symmetric\n% values as the int reader sees them\n"
+            "4 4 6\n2 1 3.5\n3 1 -2.7\n3 3 1e3\n4 2 .5\n4 3 +7\n4 1 -.5\n")
+    p = tmp_path / "v.mtx"
+    p.write_text(text)
+    fields = [ln.split(None, 2)[2] for ln in text.splitlines()[3:]]
+    want_int = [_int_prefix(f) for f in fields]
+    pairs = [tuple(int(x) - 1 for x in ln.split()[:2]) for ln in text.splitlines()[3:]]
+    # undirected: every entry once, directedness 0
+    u = nwgr_amd.read_mm(str(p), directed=False, int_attribute=True)
+    assert not u.directed and u.n == 4
+    assert list(zip(u.src.tolist(), u.dst.tolist())) == pairs
+    assert u.ints.tolist() == want_int == [3, -2, 1, 0, 7, 0]
+    np.testing.assert_array_equal(u.values, [float(f) for f in fields])
+    # directed: off-diagonal entries mirrored right after, attribute repeated
+    d = nwgr_amd.read_mm(str(p), int_attribute=True)
+    exp_pairs, exp_int = [], []
+    for (i, j), v in zip(pairs, want_int):
+        exp_pairs.append((i, j)); exp_int.append(v)
+        if i != j:
+            exp_pairs.append((j, i)); exp_int.append(v)
+    assert d.directed and list(zip(d.src.tolist(), d.dst.tolist())) == exp_pairs
+    assert d.ints.tolist() == exp_int
+    # without the flag no attribute is kept
+    assert nwgr_amd.read_mm(str(p)).ints is None
+    # a pattern file's attribute is 1 (mm_fill's `T v(1.0)`)
+    q = tmp_path / "p.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate pattern general\n3 3 2\n1 2\n3 1\n")
+    assert nwgr_amd.read_mm(str(q), int_attribute=True).ints.tolist() == [1, 1]
+
+
+def test_process_edge_list_converter_layout(tmp_path):
+    """-d / -u outputs: edge_list<..., int>::serialize with three arrays, read
+    back by load_graph (the first two arrays are the ids)."""
+    from nwgr_amd.process_edge_list import main as convert
+    src = os.path.join(MTX, "USAir97.mtx")
+    dpath, upath = tmp_path / "d.el", tmp_path / "u.el"
+    assert convert(["-d", str(dpath), "-u", str(upath), src]) == 0
+    for path, directed in ((dpath, True), (upath, False)):
+        el = nwgr_amd.read_mm(src, directed=directed, int_attribute=True)
+        raw = path.read_bytes()
+        assert raw[:27] == b"NW Graph index_edge_list\0\0\0"
+        d, N, is_open, cnt, arrays = struct.unpack_from("<iQ?QQ", raw, 27)
+        assert (d, N, is_open, cnt, arrays) == (1 if directed else 0, el.n, False, el.m, 3)
+        off = 27 + struct.calcsize("<iQ?QQ")
+        for a, dt in ((el.src, np.uint32), (el.dst, np.uint32), (el.ints, np.int32)):
+            c, esz = struct.unpack_from("<QQ", raw, off)
+            assert (c, esz) == (el.m, 4)
+            np.testing.assert_array_equal(np.frombuffer(raw, dt, el.m, off + 16), a)
+            off += 16 + 4 * el.m
+        assert off == len(raw)
+        back = nwgr_amd.load_graph(str(path))
+        assert back.directed == directed
+        np.testing.assert_array_equal(back.src, el.src)
+        np.testing.assert_array_equal(back.ints, el.ints)
+    # USAir97 holds real weights below 1: their int attribute is 0
+    assert not nwgr_amd.read_nw_edge_list(str(dpath)).ints.any()
+
+
+def test_nw_edge_list_skips_wider_attributes(tmp_path):
+    """A third array of another element size (e.g. double weights) is skipped."""
+    el = nwgr_amd.read_mm(os.path.join(MTX, "karate.mtx"))
+    p = tmp_path / "w.el"
+    nwgr_amd.write_nw_edge_list(str(p), el)
+    raw = bytearray(p.read_bytes())
+    hdr = 27 + struct.calcsize("<iQ?")
+    struct.pack_into("<Q", raw, hdr + 8, 3)  # three arrays
+    raw += struct.pack("<QQ", el.m, 8) + np.arange(el.m, dtype=np.float64).tobytes()
+    p.write_bytes(bytes(raw))
+    back = nwgr_amd.read_nw_edge_list(str(p))
+    np.testing.assert_array_equal(back.src, el.src)
+    assert back.ints is None
