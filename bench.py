@@ -94,9 +94,10 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
-def load_traffic(scale, ef):
+def load_traffic(scale, ef, workload="pagerank"):
     """HBM bytes per sweep launch from committed rocprofv3 --pmc summaries."""
-    path = os.path.join(ROOT, "profiles", f"k_sweep_traffic_rmat_s{scale}_ef{ef}.json")
+    name = "k_sweep_traffic_er_2pow25.json" if workload == "er" else f"k_sweep_traffic_rmat_s{scale}_ef{ef}.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -507,7 +508,8 @@ def main():
     n, m = g.n, g.m
     b_iter = 8 * m + 16 * n + 4
     achieved = b_iter * sweeps / (sweep_ms / 1e3) / 1e9
-    traffic = load_traffic(args.scale, args.edge_factor) if args.workload == "pagerank" else None
+    default_layout = args.engine == "auto" and args.combine == "auto" and not args.tile_bits
+    traffic = load_traffic(args.scale, args.edge_factor, args.workload) if default_layout else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": {"tiled": "k_tile+k_combine", "tiled-accumulate": "k_tile+k_accum"}.get(engine, "k_sweep"),
