@@ -396,10 +396,11 @@ def bench_multi(args):
     for _ in range(args.steps):
         res = mp.run_device(out.data_ptr(), **kw)
         dev_ms += res.ms
-        iters, _ = mp.graph_results()
-        edge_sweeps += int(np.sum(m_g * iters.astype(np.int64)))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # every run of the same input takes the same sweeps per graph
+    iters, _ = mp.graph_results()
+    edge_sweeps = args.steps * int(np.sum(m_g * iters.astype(np.int64)))
     n_g = np.diff(vbase.astype(np.int64))
     b_sweeps = int(np.sum((8 * m_g + 16 * n_g) * iters.astype(np.int64)))  # last run's sweeps per graph
     achieved = b_sweeps / (dev_ms / args.steps / 1e3) / 1e9

@@ -87,11 +87,23 @@ __device__ __forceinline__ void bsum2(double& a, double& b, double (*s)[kMT / 64
   b = tb;
 }
 
+// A heavy row by one wavefront: lane l adds entries l, l + 64, ... in order,
+// four index loads per lane in flight (loads past the row read the padding
+// of lidx and are masked to 0), then a butterfly.
 template <bool LDS>
 __device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __restrict__ cl,
                                              const float* __restrict__ cg, uint32_t b, uint32_t e, uint32_t wl) {
   float z = 0.f;
-  for (uint32_t k = b + wl; k < e; k += 64) z += LDS ? cl[a.lidx[k]] : cg[a.lidx[k]];
+  for (uint32_t k = b + wl; k < e; k += 4 * 64) {
+    uint32_t j[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) j[q] = a.lidx[k + q * 64 < e ? k + q * 64 : a.m];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = LDS ? cl[j[q]] : cg[j[q]];
+      z += k + q * 64 < e ? x : 0.f;
+    }
+  }
   return wsumf(z);
 }
 
@@ -134,47 +146,62 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
       err += fabs(static_cast<double>(rn) - static_cast<double>(ro));
       if (dg == 0.f) dang += static_cast<double>(rn);
     };
-    // Rows of ordinary length, a thread each.  The next row's bounds and
-    // epilogue operands are loaded while the current row is gathered; a row's
-    // entries are loaded kEntriesPerStep at a time with no bounds branch
-    // (loads past its end read the padding of lidx and are masked to 0).
-    // Each row adds its entries in order.
+    // Rows of ordinary length, a thread each, in a two-stage pipeline: the
+    // bounds and epilogue operands of the row after next and the first
+    // kEntriesPerStep entries of the next row are loaded while the current
+    // row is gathered.  Entries are loaded kEntriesPerStep at a time with no
+    // bounds branch (loads past a row's end read the padding of lidx and are
+    // masked to 0).  Each row adds its entries in order.
     const uint32_t pad = a.m;  // an index into the padding of lidx
-    auto row_in = [&](uint32_t v, uint32_t& rb, uint32_t& re, float& t, float& dg, float& ro) {
+    struct RowOps {
+      uint32_t b, e;
+      float t, dg, ro;
+    };
+    auto row_in = [&](uint32_t v) {
+      RowOps o;
       const bool ok = v < ng;
       const uint32_t vc = ok ? v : 0u;
-      rb = a.offs[v0 + vc];
-      re = ok ? a.offs[v0 + vc + 1] : rb;
-      t = a.tel ? a.tel[v0 + vc] : un;
-      dg = a.degf[v0 + vc];
-      ro = rin[vc];
+      o.b = a.offs[v0 + vc];
+      o.e = ok ? a.offs[v0 + vc + 1] : o.b;
+      o.t = a.tel ? a.tel[v0 + vc] : un;
+      o.dg = a.degf[v0 + vc];
+      o.ro = rin[vc];
+      return o;
     };
-    uint32_t rb, re;
-    float t, dg, ro;
-    row_in(tid, rb, re, t, dg, ro);
+    auto first_entries = [&](const RowOps& o, uint32_t (&j)[kEntriesPerStep]) {
+#pragma unroll
+      for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[o.b + q < o.e ? o.b + q : pad];
+    };
+    RowOps cur = row_in(tid), nxt = row_in(tid + kMT);
+    uint32_t jc[kEntriesPerStep];
+    first_entries(cur, jc);
     for (uint32_t v = tid; v < ng; v += kMT) {
-      uint32_t nb, ne;
-      float nt, ndg, nro;
-      row_in(v + kMT, nb, ne, nt, ndg, nro);
-      if (re - rb <= a.heavy_min) {
+      const RowOps nn = row_in(v + 2 * kMT);
+      uint32_t jn[kEntriesPerStep];
+      first_entries(nxt, jn);
+      if (cur.e - cur.b <= a.heavy_min) {
         float z = 0.f;
-        for (uint32_t k = rb; k < re; k += kEntriesPerStep) {
+#pragma unroll
+        for (int q = 0; q < kEntriesPerStep; ++q) {
+          const float x = LDS ? cl[jc[q]] : cg[jc[q]];
+          z += cur.b + q < cur.e ? x : 0.f;
+        }
+        for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
           uint32_t j[kEntriesPerStep];
 #pragma unroll
-          for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < re ? k + q : pad];
+          for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
 #pragma unroll
           for (int q = 0; q < kEntriesPerStep; ++q) {
             const float x = LDS ? cl[j[q]] : cg[j[q]];
-            z += k + q < re ? x : 0.f;
+            z += k + q < cur.e ? x : 0.f;
           }
         }
-        finish(v, z, t, dg, ro);
+        finish(v, z, cur.t, cur.dg, cur.ro);
       }
-      rb = nb;
-      re = ne;
-      t = nt;
-      dg = ndg;
-      ro = nro;
+      cur = nxt;
+      nxt = nn;
+#pragma unroll
+      for (int q = 0; q < kEntriesPerStep; ++q) jc[q] = jn[q];
     }
     for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
       const uint32_t v = a.heavy[h] - v0;
@@ -273,12 +300,17 @@ struct MultiGraph {
   uint32_t lds_max_n = kMultiLdsMax;
   uint32_t heavy_min = 64;
   DevBuf offs, lidx, vbase, degf, order, heavy, hbase, r[2], c[2], iters, errs, queue;
-  std::vector<uint32_t> offs_h, vbase_h, iters_h;
-  std::vector<double> errs_h;
+  std::vector<uint32_t> offs_h, vbase_h;
+  // per-graph results of the last run, copied into pinned host memory
+  uint32_t* iters_h = nullptr;
+  double* errs_h = nullptr;
+  bool has_run = false;
   hipStream_t stream = nullptr;
   double setup_ms = 0;
 
   ~MultiGraph() {
+    if (iters_h) (void)hipHostFree(iters_h);
+    if (errs_h) (void)hipHostFree(errs_h);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -429,7 +461,10 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   NWGR_HIP(hipMemsetAsync(mg.queue.p, 0, sizeof(uint32_t), s));
   NWGR_HIP(hipEventRecord(e0, s));
   const size_t shm = sizeof(float) * mg.lds_max_n;
-  k_multi<<<std::min<uint32_t>(mg.num_cu, mg.count), kMT, shm, s>>>(a);
+  // two workgroups per CU when their LDS mirrors fit beside each other (the
+  // 1024-thread workgroups then fill the CU's wavefront slots)
+  const uint32_t per_cu = shm + 2048 <= 80 * 1024 ? 2u : 1u;
+  k_multi<<<std::min<uint32_t>(mg.num_cu * per_cu, mg.count), kMT, shm, s>>>(a);
   const hipError_t le = hipGetLastError();
   NWGR_HIP(hipEventRecord(e1, s));
   NWGR_HIP(hipEventSynchronize(e1));
@@ -438,10 +473,12 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   NWGR_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  mg.iters_h.resize(mg.count);
-  mg.errs_h.resize(mg.count);
-  NWGR_HIP(hipMemcpy(mg.iters_h.data(), mg.iters.p, sizeof(uint32_t) * mg.count, hipMemcpyDeviceToHost));
-  NWGR_HIP(hipMemcpy(mg.errs_h.data(), mg.errs.p, sizeof(double) * mg.count, hipMemcpyDeviceToHost));
+  if (!mg.iters_h) NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&mg.iters_h), sizeof(uint32_t) * mg.count));
+  if (!mg.errs_h) NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&mg.errs_h), sizeof(double) * mg.count));
+  NWGR_HIP(hipMemcpyAsync(mg.iters_h, mg.iters.p, sizeof(uint32_t) * mg.count, hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipMemcpyAsync(mg.errs_h, mg.errs.p, sizeof(double) * mg.count, hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  mg.has_run = true;
   uint32_t mx = 0;
   double emax = 0;
   bool all = true;
@@ -519,9 +556,9 @@ int nwgr_multi_run_f32(nwgr_multi_graph* mg, const nwgr_pr_params* p, float* r_o
 int nwgr_multi_graph_results(const nwgr_multi_graph* mg, uint32_t* iters, double* errors) {
   return nwgr::guarded([&] {
     NWGR_REQUIRE(mg != nullptr, "null argument");
-    NWGR_REQUIRE(mg->iters_h.size() == mg->count, "no run yet");
-    if (iters) std::copy(mg->iters_h.begin(), mg->iters_h.end(), iters);
-    if (errors) std::copy(mg->errs_h.begin(), mg->errs_h.end(), errors);
+    NWGR_REQUIRE(mg->has_run, "no run yet");
+    if (iters) std::copy(mg->iters_h, mg->iters_h + mg->count, iters);
+    if (errors) std::copy(mg->errs_h, mg->errs_h + mg->count, errors);
   });
 }
 
