@@ -12,12 +12,18 @@
 // mirrored in LDS for the gathers; rows with more than `heavy_min` entries are
 // reduced by a wavefront each, the others by a thread each.  Cross-workgroup
 // communication is the queue counter alone (one RMW atomic per graph).
+//
+// Inside every graph the vertices are relabelled by non-increasing in-degree
+// (stable), so the rows a wavefront takes together have about the same
+// length and its lanes stay converged; the caller's vectors (teleport,
+// initial ranks, output) are read and written through the permutation.
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "nwgr_internal.hpp"
@@ -36,6 +42,7 @@ struct MultiArgs {
   const uint32_t* __restrict__ offs;   // n + 1 (global)
   const uint16_t* __restrict__ lidx;   // m + padding, local ids
   const uint32_t* __restrict__ vbase;  // count + 1
+  const uint32_t* __restrict__ perm;   // n: internal vertex -> caller vertex (inside its graph)
   const float* __restrict__ degf;      // n: divisor of the contribution
   const uint32_t* __restrict__ order;  // processing order of the graphs
   const uint32_t* __restrict__ heavy;  // rows (global ids) with more than heavy_min entries, by graph
@@ -117,7 +124,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
   // start: r = init or 1/n_g, contributions, dangling mass
   double D = 0, unused = 0;
   for (uint32_t v = tid; v < ng; v += kMT) {
-    const float r = a.init ? a.init[v0 + v] : un;
+    const float r = a.init ? a.init[a.perm[v0 + v]] : un;
     const float dg = a.degf[v0 + v];
     a.r[0][v0 + v] = r;
     const float c = r / dg;
@@ -163,7 +170,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
       const uint32_t vc = ok ? v : 0u;
       o.b = a.offs[v0 + vc];
       o.e = ok ? a.offs[v0 + vc + 1] : o.b;
-      o.t = a.tel ? a.tel[v0 + vc] : un;
+      o.t = a.tel ? a.tel[a.perm[v0 + vc]] : un;
       o.dg = a.degf[v0 + vc];
       o.ro = rin[vc];
       return o;
@@ -207,7 +214,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
       const uint32_t v = a.heavy[h] - v0;
       const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
       const float z = gather_wave<LDS>(a, cl, cg, b, e, wl);
-      if (wl == 0) finish(v, z, a.tel ? a.tel[v0 + v] : un, a.degf[v0 + v], rin[v]);
+      if (wl == 0) finish(v, z, a.tel ? a.tel[a.perm[v0 + v]] : un, a.degf[v0 + v], rin[v]);
     }
     bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
     if constexpr (LDS) {
@@ -221,7 +228,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     if (E < a.tol) break;
   }
   const float* fin = a.r[p] + v0;
-  for (uint32_t v = tid; v < ng; v += kMT) a.out[v0 + v] = fin[v];
+  for (uint32_t v = tid; v < ng; v += kMT) a.out[a.perm[v0 + v]] = fin[v];
   if (tid == 0) {
     a.iters_out[g] = it;
     a.err_out[g] = E;
@@ -256,28 +263,37 @@ __global__ void k_vertex_graph(const uint32_t* __restrict__ vbase, uint32_t coun
     for (uint32_t v = vbase[g]; v < vbase[g + 1]; ++v) gid[v] = g;
 }
 
-// local ids, and a flag for an entry whose source lies outside its row's graph
-__global__ void k_localize(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ idx,
-                           const uint32_t* __restrict__ vbase, const uint32_t* __restrict__ gid, uint32_t n,
-                           uint16_t* __restrict__ lidx, uint32_t* __restrict__ bad) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint32_t g = gid[v];
+// Relabelled in-CSR: internal row R holds the entries of caller row perm[R],
+// in their order, as local ids of the relabelled graph; a flag for an entry
+// whose source lies outside its row's graph.
+__global__ void k_localize(const uint32_t* __restrict__ offs_in, const uint32_t* __restrict__ idx,
+                           const uint32_t* __restrict__ offs_new, const uint32_t* __restrict__ perm,
+                           const uint32_t* __restrict__ iperm, const uint32_t* __restrict__ vbase,
+                           const uint32_t* __restrict__ gid, uint32_t n, uint16_t* __restrict__ lidx,
+                           uint32_t* __restrict__ bad) {
+  for (uint32_t R = blockIdx.x * blockDim.x + threadIdx.x; R < n; R += gridDim.x * blockDim.x) {
+    const uint32_t g = gid[R];
     const uint32_t lo = vbase[g], hi = vbase[g + 1];
-    for (uint32_t k = offs[v]; k < offs[v + 1]; ++k) {
-      const uint32_t u = idx[k];
+    const uint32_t o = perm[R];
+    const uint32_t b = offs_in[o], e = offs_in[o + 1];
+    uint32_t k = offs_new[R];
+    for (uint32_t q = b; q < e; ++q, ++k) {
+      const uint32_t u = idx[q];
       if (u < lo || u >= hi) {
         atomicOr(bad, 1u);
         lidx[k] = 0;
       } else {
-        lidx[k] = static_cast<uint16_t>(u - lo);
+        lidx[k] = static_cast<uint16_t>(iperm[u] - lo);
       }
     }
   }
 }
 
-__global__ void k_degf_plain(const uint32_t* __restrict__ deg, uint32_t n, float* __restrict__ out) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    out[v] = static_cast<float>(deg[v]);
+// divisor of internal vertex R: the caller's degree of perm[R]
+__global__ void k_degf_perm(const uint32_t* __restrict__ deg, const uint32_t* __restrict__ perm, uint32_t n,
+                            float* __restrict__ out) {
+  for (uint32_t R = blockIdx.x * blockDim.x + threadIdx.x; R < n; R += gridDim.x * blockDim.x)
+    out[R] = static_cast<float>(deg[perm[R]]);
 }
 
 __global__ void k_out_degree_u32(const uint32_t* __restrict__ idx, uint64_t m, uint32_t* __restrict__ deg) {
@@ -299,7 +315,7 @@ struct MultiGraph {
   uint32_t num_cu = 0;
   uint32_t lds_max_n = kMultiLdsMax;
   uint32_t heavy_min = 64;
-  DevBuf offs, lidx, vbase, degf, order, heavy, hbase, r[2], c[2], iters, errs, queue;
+  DevBuf offs, lidx, vbase, perm, degf, order, heavy, hbase, r[2], c[2], iters, errs, queue;
   std::vector<uint32_t> offs_h, vbase_h;
   // per-graph results of the last run, copied into pinned host memory
   uint32_t* iters_h = nullptr;
@@ -363,22 +379,48 @@ MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint3
   mg->num_cu = static_cast<uint32_t>(prop.multiProcessorCount);
   NWGR_HIP(hipStreamCreateWithFlags(&mg->stream, hipStreamNonBlocking));
   hipStream_t s = mg->stream;
-  mg->offs_h.assign(offsets, offsets + n + 1);
+  // per graph, rows by non-increasing in-degree (stable): perm[new] = old
+  std::vector<uint32_t> perm(n), iperm(n);
+  {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < hw; ++t)
+      th.emplace_back([&, t] {
+        for (uint32_t g = t; g < count; g += hw) {
+          const uint32_t lo = vbase[g], hi = vbase[g + 1];
+          std::iota(perm.begin() + lo, perm.begin() + hi, lo);
+          std::stable_sort(perm.begin() + lo, perm.begin() + hi, [&](uint32_t x, uint32_t y) {
+            return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+          });
+          for (uint32_t R = lo; R < hi; ++R) iperm[perm[R]] = R;
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  mg->offs_h.assign(n + 1, 0);
+  for (uint32_t R = 0; R < n; ++R) mg->offs_h[R + 1] = mg->offs_h[R] + (offsets[perm[R] + 1] - offsets[perm[R]]);
   mg->vbase_h.assign(vbase, vbase + count + 1);
   mg->offs.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
   mg->vbase.alloc(sizeof(uint32_t) * (count + 1));
-  NWGR_HIP(hipMemcpyAsync(mg->offs.p, offsets, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  mg->perm.alloc(sizeof(uint32_t) * n);
+  NWGR_HIP(hipMemcpyAsync(mg->offs.p, mg->offs_h.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
   NWGR_HIP(hipMemcpyAsync(mg->vbase.p, vbase, sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice, s));
-  DevBuf d_idx, gid, bad;
+  NWGR_HIP(hipMemcpyAsync(mg->perm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+  DevBuf d_idx, d_offs, d_iperm, gid, bad;
   d_idx.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
+  d_offs.alloc(sizeof(uint32_t) * (static_cast<size_t>(n) + 1));
+  d_iperm.alloc(sizeof(uint32_t) * n);
   if (m) NWGR_HIP(hipMemcpyAsync(d_idx.p, indices, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipMemcpyAsync(d_offs.p, offsets, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  NWGR_HIP(hipMemcpyAsync(d_iperm.p, iperm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
   mg->lidx.alloc(sizeof(uint16_t) * (m + 64));
   NWGR_HIP(hipMemsetAsync(mg->lidx.p, 0, mg->lidx.bytes, s));
   gid.alloc(sizeof(uint32_t) * n);
   bad.alloc(sizeof(uint32_t));
   NWGR_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
   k_vertex_graph<<<nblk(count), 256, 0, s>>>(mg->vbase.as<uint32_t>(), count, gid.as<uint32_t>());
-  k_localize<<<nblk(n), 256, 0, s>>>(mg->offs.as<uint32_t>(), d_idx.as<uint32_t>(), mg->vbase.as<uint32_t>(),
+  k_localize<<<nblk(n), 256, 0, s>>>(d_offs.as<uint32_t>(), d_idx.as<uint32_t>(), mg->offs.as<uint32_t>(),
+                                     mg->perm.as<uint32_t>(), d_iperm.as<uint32_t>(), mg->vbase.as<uint32_t>(),
                                      gid.as<uint32_t>(), n, mg->lidx.as<uint16_t>(), bad.as<uint32_t>());
   NWGR_HIP(hipGetLastError());
   // divisors: the caller's degrees, or out-degrees
@@ -391,7 +433,7 @@ MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint3
     NWGR_HIP(hipMemsetAsync(deg.p, 0, deg.bytes, s));
     if (m) k_out_degree_u32<<<nblk(m), 256, 0, s>>>(d_idx.as<uint32_t>(), m, deg.as<uint32_t>());
   }
-  k_degf_plain<<<nblk(n), 256, 0, s>>>(deg.as<uint32_t>(), n, mg->degf.as<float>());
+  k_degf_perm<<<nblk(n), 256, 0, s>>>(deg.as<uint32_t>(), mg->perm.as<uint32_t>(), n, mg->degf.as<float>());
   NWGR_HIP(hipGetLastError());
   uint32_t badh = 0;
   NWGR_HIP(hipMemcpyAsync(&badh, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -430,6 +472,7 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   a.offs = mg.offs.as<uint32_t>();
   a.lidx = mg.lidx.as<uint16_t>();
   a.vbase = mg.vbase.as<uint32_t>();
+  a.perm = mg.perm.as<uint32_t>();
   a.degf = mg.degf.as<float>();
   a.order = mg.order.as<uint32_t>();
   a.heavy = mg.heavy.as<uint32_t>();
