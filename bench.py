@@ -388,11 +388,11 @@ def bench_multi(args):
         log(f"autotuned {tr['tuning']} in {tr['seconds']:.1f}s ({tr['trials']} trials)")
     for _ in range(args.warmup):
         mp.run_device(out.data_ptr(), **kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dev_ms, edge_sweeps = 0.0, 0
     offs = g.offsets.astype(np.int64)
     m_g = offs[vbase[1:].astype(np.int64)] - offs[vbase[:-1].astype(np.int64)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
     for _ in range(args.steps):
         res = mp.run_device(out.data_ptr(), **kw)
         dev_ms += res.ms
