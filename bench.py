@@ -61,7 +61,8 @@ def parse():
     ap.add_argument("--combine", choices=["auto", "gather", "accumulate"], default="auto",
                     help="tiled engine second pass (auto: the library's choice)")
     ap.add_argument("--autotune", type=int, default=1,
-                    help="multi workload: pick the decomposition with nwgr_amd.autotune before timing (0: defaults)")
+                    help="multi and shard workloads: pick the decomposition with nwgr_amd.autotune before "
+                         "timing (0: defaults)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS or all cores")
@@ -311,8 +312,9 @@ def bench_shard(args):
     destination rows cut into `--shards` contiguous slices of about equal
     entry counts, each run as the same n-vertex graph with entries only in its
     slice (the full gathered vector resident, as each GPU would hold it).
-    Per share: device time per sweep over fixed-length solves (tol 0, 20
-    sweeps, UNIFORM dangling); algorithmic bytes 8 M_k + 16 N_k.  The rows
+    Per share (and for the whole graph): the layout autotuned for its shape
+    (with --autotune 1), then device time per sweep over fixed-length solves
+    (tol 0, 20 sweeps, UNIFORM dangling); algorithmic bytes 8 M_k + 16 N_k.  The rows
     outside the slice have no entries and cost only their contribution
     store (4 B each), which a real share would not make."""
     import torch
@@ -327,6 +329,9 @@ def bench_shard(args):
 
     def per_sweep_us(graph):
         with PageRankGraph(graph, deg) as pg:
+            if args.autotune:
+                from nwgr_amd.autotune import autotune
+                autotune(pg, sweeps=8, reps=1, rounds=1)
             for _ in range(max(args.warmup, 1)):
                 pg.run_device(out.data_ptr(), **kw)
             ms, sw = 0.0, 0

@@ -34,6 +34,7 @@ ENGINE_BINNED, ENGINE_TILED = 0, 1
 SEARCH_SPACE_TILED = {
     "tile_bits": [13, 14, 15],
     "tile_fill_cost": [0, 10, 40, 160],
+    "tile_combine": [0, 1],
 }
 
 SEARCH_SPACE_BINNED = {
