@@ -489,6 +489,7 @@ def main():
     iters = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    solve_ms = []
     for i in range(args.steps):
         res = pg.run_device(out_d.data_ptr(), **kw)
         prof = pg.last_profile()
@@ -497,6 +498,7 @@ def main():
         sweep_ms += prof["sweep_ms"]
         launches += prof["launches"]
         dev_ms += res.ms
+        solve_ms.append(res.ms)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     engine = pg.last_engine()
@@ -546,6 +548,7 @@ def main():
         "config": {"workload": f"pagerank {gname}, damping 0.85, dangling uniform, tol {args.tol:g} L1, max "
                                f"{args.max_iters} sweeps, FP32",
                    "n": n, "m": m, "sweeps_per_solve": iters[0], "device_ms_per_solve": round(dev_ms / args.steps, 4),
+                   "device_ms_per_solve_median": round(float(np.median(solve_ms)), 4),
                    "setup_ms": round(pg.setup_ms, 1), "engine": engine,
                    "host_output_ms_per_solve": round(host_ms, 3)},
         "roofline": roof, "cpu_baseline": cpu,

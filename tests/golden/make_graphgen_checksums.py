@@ -1,5 +1,5 @@
-"""Write graphgen_checksums.json: edge counts and FNV-1a checksums of the
-generated graphs, so later changes to the generator are caught.  The generator
+"""Write graphgen_checksums.json: edge counts and FNV-1a (scale 10) or XXH64
+(scales 16 and 22, the configuration sizes) checksums of the generated graphs, so later changes to the generator are caught.  The generator
 itself is pinned against its specification by tests/test_graphgen.py."""
 import json
 import os
@@ -26,9 +26,12 @@ def main():
         g = nwgr_amd.rmat(scale, 16)
         out[f"rmat_s{scale}_ef16_seed27491095"] = {
             "n": g.n, "m": g.m, "fnv1a_offsets": hex(fnv1a(g.offsets)), "fnv1a_indices": hex(fnv1a(g.indices))}
+    import xxhash
     for scale in (16, 22):
         g = nwgr_amd.rmat(scale, 16)
-        out[f"rmat_s{scale}_ef16_seed27491095"] = {"n": g.n, "m": g.m}
+        out[f"rmat_s{scale}_ef16_seed27491095"] = {
+            "n": g.n, "m": g.m, "xxh64_offsets": xxhash.xxh64(g.offsets.tobytes()).hexdigest(),
+            "xxh64_indices": xxhash.xxh64(g.indices.tobytes()).hexdigest()}
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "graphgen_checksums.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)

@@ -162,3 +162,16 @@ def test_checksum_fixture_s10():
     assert g.m == e["m"]
     assert _fnv1a(g.offsets) == int(e["fnv1a_offsets"], 16)
     assert _fnv1a(g.indices) == int(e["fnv1a_indices"], 16)
+
+
+@pytest.mark.parametrize("scale", [16, 22])
+def test_checksum_fixture_configuration_sizes(scale):
+    """The R-MAT graphs of the benchmark configurations (scale 16 for the CPU
+    path, scale 22 for the headline) are pinned by XXH64 of their arrays."""
+    xxhash = pytest.importorskip("xxhash")
+    with open(os.path.join(GOLDEN, "graphgen_checksums.json")) as f:
+        e = json.load(f)[f"rmat_s{scale}_ef16_seed27491095"]
+    g = nwgr_amd.rmat(scale, 16)
+    assert (g.n, g.m) == (e["n"], e["m"])
+    assert xxhash.xxh64(g.offsets.tobytes()).hexdigest() == e["xxh64_offsets"]
+    assert xxhash.xxh64(g.indices.tobytes()).hexdigest() == e["xxh64_indices"]
