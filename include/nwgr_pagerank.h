@@ -227,6 +227,12 @@ uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap)
  * (page_rank.hpp:123,142).  Any output pointer may be NULL. */
 int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* launches, uint32_t* chunks);
 
+/* Device time (ms) of every sweep of the last run, when it was traced
+ * (params.trace = 1 captures one sweep per hipGraph launch and times each,
+ * the per-iteration time of the reference's trace, page_rank.hpp:123,142);
+ * returns the number written (up to `cap`), 0 after an untraced run. */
+uint32_t nwgr_pr_last_sweep_times(const nwgr_pr_graph* g, double* ms, uint32_t cap);
+
 /* Batched personalized PageRank: k (1 .. 256) columns solved together, column
  * j with the one-hot teleport at vertex seeds[j] (caller ids; duplicates
  * allowed).  Per column: the iteration of the plain solve with teleport =

@@ -110,6 +110,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_delta_run_device_f32.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
     L.nwgr_pr_last_errors.restype = u32
     L.nwgr_pr_last_errors.argtypes = [vp, vp, u32]
+    L.nwgr_pr_last_sweep_times.restype = u32
+    L.nwgr_pr_last_sweep_times.argtypes = [vp, vp, u32]
     L.nwgr_pr_last_profile.restype = i32
     L.nwgr_pr_last_profile.argtypes = [vp, P(f64), P(u32), P(u32)]
     L.nwgr_pr_last_engine.restype = i32

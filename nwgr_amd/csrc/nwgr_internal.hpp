@@ -165,6 +165,7 @@ struct SolveProfile {
   double chunk_ms = 0;      // summed device time of the sweep chunks
   uint32_t launches = 0;    // sweep kernel launches (sweeps + the stopping launch + idle ones)
   uint32_t chunks = 0;
+  std::vector<double> launch_ms;  // traced solves (one launch per chunk): device time of launch 1, 2, ...
 };
 
 // Engine choice.  Auto takes the tiled engine for a CSR whose entries form

@@ -1397,7 +1397,11 @@ void validate_params(const nwgr_pr_params& p) {
 }
 
 void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s) {
-  const uint32_t K = std::clamp<uint32_t>(p.chunk ? p.chunk : g.tune.chunk, 1, Graph::kMaxChunk);
+  // a traced solve captures one sweep per chunk, so the events around the
+  // chunks time every sweep on its own (the reference's per-iteration time,
+  // page_rank.hpp:123,142)
+  const bool per_sweep = p.trace != 0;
+  const uint32_t K = per_sweep ? 1u : std::clamp<uint32_t>(p.chunk ? p.chunk : g.tune.chunk, 1, Graph::kMaxChunk);
   if (g.hist_cap < p.max_iters || !g.hist.p) {
     g.hist_cap = std::max<uint32_t>(p.max_iters, 1);
     g.hist.alloc(sizeof(double) * g.hist_cap);
@@ -1509,6 +1513,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     float cms = 0;
     NWGR_HIP(hipEventElapsedTime(&cms, c.a, c.b));
     prof.chunk_ms += cms;
+    if (per_sweep) prof.launch_ms.push_back(cms);
   };
   launch(static_cast<uint32_t>(std::min<uint64_t>(K0, limit)));
   for (;;) {
@@ -1551,11 +1556,14 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   if (!g.last_errs.empty())
     NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
   if (p.trace) {
-    // the reference's trace table (page_rank.hpp:42-52, :109, :142); the
-    // per-sweep time is the sweep chunks' device time divided evenly
+    // the reference's trace table (page_rank.hpp:42-52, :109, :142), with
+    // every sweep's own device time (seconds)
     std::printf("%-5s%-20s%-20s%-20s\n", "iter", "error", "time", "outgoing");
-    for (uint32_t i = 0; i < g.last_errs.size(); ++i)
-      std::printf("%-5u%-20.10f%-20.10f%-20.10f\n", i, g.last_errs[i], prof.chunk_ms / 1e3 / prof.launches, 0.0);
+    for (uint32_t i = 0; i < g.last_errs.size(); ++i) {
+      const double t = i < prof.launch_ms.size() ? prof.launch_ms[i] / 1e3 : prof.chunk_ms / 1e3 / prof.launches;
+      std::printf("%-5u%-20.10f%-20.10f%-20.10f\n", i, g.last_errs[i], t, 0.0);
+    }
+    std::fflush(stdout);
   }
 }
 
@@ -1822,6 +1830,13 @@ int nwgr_pr_last_profile(const nwgr_pr_graph* g, double* sweep_ms, uint32_t* lau
     if (launches) *launches = g->last_profile.launches;
     if (chunks) *chunks = g->last_profile.chunks;
   });
+}
+
+uint32_t nwgr_pr_last_sweep_times(const nwgr_pr_graph* g, double* ms, uint32_t cap) {
+  if (!g || !ms) return 0;
+  const size_t k = std::min<size_t>({cap, g->last_profile.launch_ms.size(), g->last_errs.size()});
+  std::copy(g->last_profile.launch_ms.begin(), g->last_profile.launch_ms.begin() + k, ms);
+  return static_cast<uint32_t>(k);
 }
 
 int nwgr_pr_delta_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,

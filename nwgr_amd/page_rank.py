@@ -138,6 +138,12 @@ class PageRankGraph:
         e = pagerank_lib().nwgr_pr_last_engine(self._h)
         return {0: "binned", 1: "tiled", 2: "tiled-accumulate"}.get(e)
 
+    def last_sweep_times(self) -> np.ndarray:
+        """Device ms of every sweep of the last run if it was traced, else empty."""
+        buf = np.zeros(4096, np.float64)
+        k = pagerank_lib().nwgr_pr_last_sweep_times(self._h, buf.ctypes.data, len(buf))
+        return buf[:k].copy()
+
     def last_errors(self) -> np.ndarray:
         buf = np.zeros(4096, np.float64)
         k = pagerank_lib().nwgr_pr_last_errors(self._h, buf.ctypes.data, len(buf))

@@ -278,3 +278,25 @@ def test_config3_er_2pow25_full_size():
     r, res, r64, err = parity(g, deg, "uniform", 1e-6, 100, nthreads=16)
     say(f"er 2^25 sweeps={res.iters} L1={err:.3e} ms={res.ms:.2f}")
     assert abs(float(r.sum(dtype=np.float64)) - 1.0) < 1e-5
+
+
+def test_trace_table_times_every_sweep(karate, capfd):
+    """params.trace prints the reference's table (page_rank.hpp:42-52) with each
+    sweep's own device time: one sweep per captured launch, timed by events."""
+    _, g, deg = karate
+    pg = PageRankGraph(g, deg)
+    capfd.readouterr()
+    r, res = pg.run(tol=1e-7, max_iters=100, dangling="leak", trace=True)
+    out = capfd.readouterr().out.splitlines()
+    head = out.index(next(ln for ln in out if ln.split()[:4] == ["iter", "error", "time", "outgoing"]))
+    rows = [ln.split() for ln in out[head + 1:head + 1 + res.iters]]
+    assert [int(x[0]) for x in rows] == list(range(res.iters))
+    errs, times = pg.last_errors(), pg.last_sweep_times()
+    assert len(times) == res.iters and np.all(times > 0)
+    np.testing.assert_allclose([float(x[1]) for x in rows], errs, rtol=1e-6, atol=1e-10)
+    np.testing.assert_allclose([float(x[2]) for x in rows], times / 1e3, rtol=1e-5, atol=1e-10)
+    # the traced solve computes what the untraced one does
+    r2, res2 = pg.run(tol=1e-7, max_iters=100, dangling="leak")
+    assert res2.iters == res.iters
+    np.testing.assert_array_equal(r, r2)
+    assert len(pg.last_sweep_times()) == 0
