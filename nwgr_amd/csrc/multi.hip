@@ -8,9 +8,10 @@
 // page_rank.hpp:450-505, per graph: init 1/n_g, teleport (1-d)/n_g or the
 // caller's vector, dangling LEAK / UNIFORM / TELEPORT, stop on the graph's
 // own L1 change) with workgroup barriers between sweeps, then takes the next
-// graph.  The contributions of a graph with at most `lds_max_n` vertices are
-// mirrored in LDS for the gathers; rows with more than `heavy_min` entries are
-// reduced by a wavefront each, the others by a thread each.  Cross-workgroup
+// graph.  The contributions of a graph's first `lds_max_n` vertices (all of
+// a smaller graph) are mirrored in LDS for the gathers; rows with more than
+// `heavy_min` entries are reduced by a wavefront each, the others by a thread
+// each.  Cross-workgroup
 // communication is the queue counter alone (one RMW atomic per graph).
 //
 // Inside every graph the vertices are relabelled by non-increasing in-degree
@@ -94,12 +95,27 @@ __device__ __forceinline__ void bsum2(double& a, double& b, double (*s)[kMT / 64
   b = tb;
 }
 
+// Where a graph's gathered contributions live: all in the LDS mirror (the
+// graph has at most lds_max_n vertices), its first lds_max_n vertices there
+// and the rest in global memory (after the in-degree relabelling the first
+// vertices are the most gathered ones), or all in global memory.
+constexpr int kMirrorAll = 0, kMirrorPrefix = 1, kMirrorNone = 2;
+
+template <int M>
+__device__ __forceinline__ float cget(const float* __restrict__ cl, const float* __restrict__ cg, uint32_t K,
+                                      uint32_t j) {
+  if constexpr (M == kMirrorAll) return cl[j];
+  else if constexpr (M == kMirrorPrefix) return j < K ? cl[j] : cg[j];
+  else return cg[j];
+}
+
 // A heavy row by one wavefront: lane l adds entries l, l + 64, ... in order,
 // four index loads per lane in flight (loads past the row read the padding
 // of lidx and are masked to 0), then a butterfly.
-template <bool LDS>
+template <int M>
 __device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __restrict__ cl,
-                                             const float* __restrict__ cg, uint32_t b, uint32_t e, uint32_t wl) {
+                                             const float* __restrict__ cg, uint32_t K, uint32_t b, uint32_t e,
+                                             uint32_t wl) {
   float z = 0.f;
   for (uint32_t k = b + wl; k < e; k += 4 * 64) {
     uint32_t j[4];
@@ -107,7 +123,7 @@ __device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __
     for (int q = 0; q < 4; ++q) j[q] = a.lidx[k + q * 64 < e ? k + q * 64 : a.m];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float x = LDS ? cl[j[q]] : cg[j[q]];
+      const float x = cget<M>(cl, cg, K, j[q]);
       z += k + q * 64 < e ? x : 0.f;
     }
   }
@@ -115,10 +131,12 @@ __device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __
 }
 
 // One graph, start to finish, by the whole workgroup.
-template <bool LDS>
+template <int M>
 __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*s_red)[kMT / 64]) {
   const uint32_t tid = threadIdx.x, wl = tid & 63, wv = tid >> 6;
   const uint32_t v0 = a.vbase[g], ng = a.vbase[g + 1] - v0;
+  // vertices [0, K) are mirrored in LDS
+  const uint32_t K = M == kMirrorAll ? ng : (M == kMirrorPrefix ? a.lds_max_n : 0u);
   const float un = 1.0f / static_cast<float>(ng);
   const float base_u = static_cast<float>((1.0 - (static_cast<double>(a.d_hi) + a.d_lo)) / ng);
   // start: r = init or 1/n_g, contributions, dangling mass
@@ -129,7 +147,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     a.r[0][v0 + v] = r;
     const float c = r / dg;
     a.c[0][v0 + v] = c;
-    if constexpr (LDS) cl[v] = c;
+    if (v < K) cl[v] = c;
     if (dg == 0.f) D += static_cast<double>(r);
   }
   bsum2(D, unused, s_red);  // barrier: LDS and the start vectors are complete
@@ -190,7 +208,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
         float z = 0.f;
 #pragma unroll
         for (int q = 0; q < kEntriesPerStep; ++q) {
-          const float x = LDS ? cl[jc[q]] : cg[jc[q]];
+          const float x = cget<M>(cl, cg, K, jc[q]);
           z += cur.b + q < cur.e ? x : 0.f;
         }
         for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
@@ -199,7 +217,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
           for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
 #pragma unroll
           for (int q = 0; q < kEntriesPerStep; ++q) {
-            const float x = LDS ? cl[j[q]] : cg[j[q]];
+            const float x = cget<M>(cl, cg, K, j[q]);
             z += k + q < cur.e ? x : 0.f;
           }
         }
@@ -213,12 +231,12 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
       const uint32_t v = a.heavy[h] - v0;
       const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
-      const float z = gather_wave<LDS>(a, cl, cg, b, e, wl);
+      const float z = gather_wave<M>(a, cl, cg, K, b, e, wl);
       if (wl == 0) finish(v, z, a.tel ? a.tel[a.perm[v0 + v]] : un, a.degf[v0 + v], rin[v]);
     }
     bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
-    if constexpr (LDS) {
-      for (uint32_t v = tid; v < ng; v += kMT) cl[v] = cout[v];
+    if constexpr (M != kMirrorNone) {
+      for (uint32_t v = tid; v < K; v += kMT) cl[v] = cout[v];
       __syncthreads();
     }
     p ^= 1;
@@ -250,9 +268,11 @@ __global__ __launch_bounds__(kMT) void k_multi(MultiArgs a) {
     if (g == 0xffffffffu) return;
     const uint32_t ng = a.vbase[g + 1] - a.vbase[g];
     if (ng <= a.lds_max_n)
-      solve_graph<true>(a, g, cl, s_red);
+      solve_graph<kMirrorAll>(a, g, cl, s_red);
+    else if (a.lds_max_n > 0)
+      solve_graph<kMirrorPrefix>(a, g, cl, s_red);
     else
-      solve_graph<false>(a, g, cl, s_red);
+      solve_graph<kMirrorNone>(a, g, cl, s_red);
     __syncthreads();
   }
 }
