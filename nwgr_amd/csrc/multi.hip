@@ -99,12 +99,14 @@ __device__ __forceinline__ void bsum2(double& a, double& b, double (*s)[kMT / 64
 // graph has at most lds_max_n vertices), its first lds_max_n vertices there
 // and the rest in global memory (after the in-degree relabelling the first
 // vertices are the most gathered ones), or all in global memory.
-constexpr int kMirrorAll = 0, kMirrorPrefix = 1, kMirrorNone = 2;
+// A graph of at most lds_max_n / 2 vertices keeps two LDS copies (this
+// sweep's and the next), so its contributions never go to global memory.
+constexpr int kMirrorAll = 0, kMirrorPrefix = 1, kMirrorNone = 2, kMirrorDouble = 3;
 
 template <int M>
 __device__ __forceinline__ float cget(const float* __restrict__ cl, const float* __restrict__ cg, uint32_t K,
                                       uint32_t j) {
-  if constexpr (M == kMirrorAll) return cl[j];
+  if constexpr (M == kMirrorAll || M == kMirrorDouble) return cl[j];
   else if constexpr (M == kMirrorPrefix) return j < K ? cl[j] : cg[j];
   else return cg[j];
 }
@@ -136,7 +138,8 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
   const uint32_t tid = threadIdx.x, wl = tid & 63, wv = tid >> 6;
   const uint32_t v0 = a.vbase[g], ng = a.vbase[g + 1] - v0;
   // vertices [0, K) are mirrored in LDS
-  const uint32_t K = M == kMirrorAll ? ng : (M == kMirrorPrefix ? a.lds_max_n : 0u);
+  const uint32_t K = (M == kMirrorAll || M == kMirrorDouble) ? ng : (M == kMirrorPrefix ? a.lds_max_n : 0u);
+  const uint32_t half = a.lds_max_n / 2;  // second LDS copy (kMirrorDouble)
   const float un = 1.0f / static_cast<float>(ng);
   const float base_u = static_cast<float>((1.0 - (static_cast<double>(a.d_hi) + a.d_lo)) / ng);
   // start: r = init or 1/n_g, contributions, dangling mass
@@ -146,7 +149,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     const float dg = a.degf[v0 + v];
     a.r[0][v0 + v] = r;
     const float c = r / dg;
-    a.c[0][v0 + v] = c;
+    if constexpr (M != kMirrorDouble) a.c[0][v0 + v] = c;
     if (v < K) cl[v] = c;
     if (dg == 0.f) D += static_cast<double>(r);
   }
@@ -159,6 +162,9 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     const float* __restrict__ rin = a.r[p] + v0;
     float* __restrict__ rout = a.r[p ^ 1] + v0;
     float* __restrict__ cout = a.c[p ^ 1] + v0;
+    // LDS copy gathered in this sweep, and (kMirrorDouble) the one written
+    const float* __restrict__ lcur = M == kMirrorDouble ? cl + (p ? half : 0u) : cl;
+    float* __restrict__ lnext = cl + (p ? 0u : half);
     const float Df = static_cast<float>(D);
     double err = 0, dang = 0;
     auto finish = [&](uint32_t v, float z, float t, float dg, float ro) {
@@ -167,7 +173,10 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
       const float s = fmaf(Df, u, z);
       const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
       rout[v] = rn;
-      cout[v] = rn / dg;
+      if constexpr (M == kMirrorDouble)
+        lnext[v] = rn / dg;
+      else
+        cout[v] = rn / dg;
       err += fabs(static_cast<double>(rn) - static_cast<double>(ro));
       if (dg == 0.f) dang += static_cast<double>(rn);
     };
@@ -208,7 +217,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
         float z = 0.f;
 #pragma unroll
         for (int q = 0; q < kEntriesPerStep; ++q) {
-          const float x = cget<M>(cl, cg, K, jc[q]);
+          const float x = cget<M>(lcur, cg, K, jc[q]);
           z += cur.b + q < cur.e ? x : 0.f;
         }
         for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
@@ -217,7 +226,7 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
           for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
 #pragma unroll
           for (int q = 0; q < kEntriesPerStep; ++q) {
-            const float x = cget<M>(cl, cg, K, j[q]);
+            const float x = cget<M>(lcur, cg, K, j[q]);
             z += k + q < cur.e ? x : 0.f;
           }
         }
@@ -231,11 +240,11 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
       const uint32_t v = a.heavy[h] - v0;
       const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
-      const float z = gather_wave<M>(a, cl, cg, K, b, e, wl);
+      const float z = gather_wave<M>(a, lcur, cg, K, b, e, wl);
       if (wl == 0) finish(v, z, a.tel ? a.tel[a.perm[v0 + v]] : un, a.degf[v0 + v], rin[v]);
     }
     bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
-    if constexpr (M != kMirrorNone) {
+    if constexpr (M == kMirrorAll || M == kMirrorPrefix) {
       for (uint32_t v = tid; v < K; v += kMT) cl[v] = cout[v];
       __syncthreads();
     }
@@ -267,7 +276,9 @@ __global__ __launch_bounds__(kMT) void k_multi(MultiArgs a) {
     __syncthreads();
     if (g == 0xffffffffu) return;
     const uint32_t ng = a.vbase[g + 1] - a.vbase[g];
-    if (ng <= a.lds_max_n)
+    if (ng <= a.lds_max_n / 2)
+      solve_graph<kMirrorDouble>(a, g, cl, s_red);
+    else if (ng <= a.lds_max_n)
       solve_graph<kMirrorAll>(a, g, cl, s_red);
     else if (a.lds_max_n > 0)
       solve_graph<kMirrorPrefix>(a, g, cl, s_red);
