@@ -106,16 +106,24 @@ def test_hub_rows_split_in_chunks():
     parity(g, deg, "leak", 1e-6, 100)
 
 
-@pytest.mark.parametrize("n", [1, 2, 5])
-def test_edgeless_graphs(n):
+@pytest.mark.parametrize("tuning", [{}, dict(engine=0), dict(engine=1, tile_combine=0),
+                                    dict(engine=1, tile_combine=1)])
+@pytest.mark.parametrize("n", [1, 2, 5, 70000])
+def test_edgeless_graphs(n, tuning):
     g = nwgr_amd.build_adjacency([], [], n=n)
     deg = np.zeros(n, np.uint32)
-    r, res = PageRankGraph(g, deg).run(tol=1e-9, max_iters=10, dangling="leak")
-    np.testing.assert_allclose(r, 0.15 / n, rtol=1e-6)
-    assert res.iters == 2
-    r, res = PageRankGraph(g, deg).run(tol=1e-9, max_iters=10, dangling="uniform")
-    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-6)
-    assert res.iters == 1
+    for mode, want, sweeps in (("leak", 0.15 / n, 2), ("uniform", 1.0 / n, 1)):
+        pg = PageRankGraph(g, deg)
+        pg.tuning = tuning
+        r, res = pg.run(tol=1e-9, max_iters=10, dangling=mode)
+        np.testing.assert_allclose(r, want, rtol=1e-6)
+        assert res.iters == sweeps
+    # the residual solve: every vertex pushes its 1 - d once, nobody sends
+    pg = PageRankGraph(g, deg)
+    pg.tuning = tuning
+    r, res = pg.run_delta(threshold=1e-4, max_iters=10)
+    assert res.iters == 1 and res.converged
+    np.testing.assert_array_equal(r, np.float32(1.0 - np.float64(np.float32(0.85))))
 
 
 def test_iteration_cap_and_zero_sweeps():
