@@ -228,29 +228,69 @@ struct PprReduceArgs {
   double tol;
 };
 
-__global__ __launch_bounds__(kPB) void k_ppr_reduce(PprReduceArgs a) {
+// Per-column totals of a sweep's block partials.  Four groups of kPB threads
+// each add a quarter of the blocks (then of the hub blocks) for their column,
+// eight blocks' loads in flight per thread; group 0 adds the four group sums
+// in group order, so the totals do not depend on the schedule.
+constexpr uint32_t kReduceGroups = 4;
+
+__global__ __launch_bounds__(kPB * kReduceGroups) void k_ppr_reduce(PprReduceArgs a) {
   __shared__ double s_max[kPB / 64];
+  __shared__ double s_sum[2][kReduceGroups][kPB];
   if (a.st->done) return;
-  const uint32_t t = threadIdx.x;
+  const uint32_t gq = threadIdx.x / kPB;  // group
+  const uint32_t t = threadIdx.x % kPB;   // column
+  auto range_sum = [&](const double* part, uint32_t nblocks, double& e, double& D) {
+    const uint32_t per = (nblocks + kReduceGroups - 1) / kReduceGroups;
+    const uint32_t b0 = min(nblocks, gq * per), b1 = min(nblocks, b0 + per);
+    uint32_t b = b0;
+    for (; b + 8 <= b1; b += 8) {
+      double ev[8], dv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        ev[j] = part[(static_cast<size_t>(b + j) * 2) * a.K4 + t];
+        dv[j] = part[(static_cast<size_t>(b + j) * 2 + 1) * a.K4 + t];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        e += ev[j];
+        D += dv[j];
+      }
+    }
+    for (; b < b1; ++b) {
+      e += part[(static_cast<size_t>(b) * 2) * a.K4 + t];
+      D += part[(static_cast<size_t>(b) * 2 + 1) * a.K4 + t];
+    }
+  };
   double e = 0, D = 0;
   if (t < a.K4) {
-    for (uint32_t b = 0; b < a.nb; ++b) {
-      e += a.part[(static_cast<size_t>(b) * 2) * a.K4 + t];
-      D += a.part[(static_cast<size_t>(b) * 2 + 1) * a.K4 + t];
-    }
-    for (uint32_t b = 0; b < a.nh; ++b) {
-      e += a.part_hub[(static_cast<size_t>(b) * 2) * a.K4 + t];
-      D += a.part_hub[(static_cast<size_t>(b) * 2 + 1) * a.K4 + t];
-    }
-    a.dang_out[t] = D;
-    a.colerr[t] = e;
+    range_sum(a.part, a.nb, e, D);
+    range_sum(a.part_hub, a.nh, e, D);
   }
-  double m = t < a.k ? e : 0.0;
+  s_sum[0][gq][t] = e;
+  s_sum[1][gq][t] = D;
+  __syncthreads();
+  // every thread stays to the last barrier; group 0 finishes the columns
+  double m = 0.0;
+  if (gq == 0) {
+    e = 0;
+    D = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kReduceGroups; ++q) {
+      e += s_sum[0][q][t];
+      D += s_sum[1][q][t];
+    }
+    if (t < a.K4) {
+      a.dang_out[t] = D;
+      a.colerr[t] = e;
+    }
+    m = t < a.k ? e : 0.0;
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-  if ((t & 63) == 0) s_max[t / 64] = m;
+  if (gq == 0 && (t & 63) == 0) s_max[t / 64] = m;
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     double mx = 0;
     for (int i = 0; i < kPB / 64; ++i) mx = fmax(mx, s_max[i]);
     const uint32_t sweeps = a.st->ctr[0] + 1;
@@ -481,7 +521,7 @@ void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_se
       ra.dang_out = ps.dang[par].as<double>();
       k_ppr_sweep<<<ps.nb, kPB, 0, s>>>(a);
       if (ps.nhubs) k_ppr_hub<<<ps.nh, kPB, 0, s>>>(a, ps.part_hub.as<double>());
-      k_ppr_reduce<<<1, kPB, 0, s>>>(ra);
+      k_ppr_reduce<<<1, kPB * kReduceGroups, 0, s>>>(ra);
     }
     NWGR_HIP(hipGetLastError());
     hipEvent_t e;
