@@ -60,21 +60,30 @@ def parse(argv):
 def main(argv=None) -> int:
     import nwgr_amd
     from nwgr_amd import PageRankGraph
-    from nwgr_amd.report import Times, log_rows
+    from nwgr_amd.report import Times, life_timer, log_rows
     a = parse(sys.argv[1:] if argv is None else argv)
     times = Times()
     for file in a.files:
-        el = nwgr_amd.load_graph(file)
+        # the phases bench/common.hpp:82-107 times with life_timer
+        with open(file, "rb") as f:
+            binary = f.read(2) == b"NW"
+        if not binary:
+            print(f"Reading matrix market input {file}")
+        with life_timer("deserialize" if binary else "read mm"):
+            el = nwgr_amd.load_graph(file)
         if a.verbose:
             print(f"edge list: n={el.n} m={el.m} directed={el.directed}")
-        graph = nwgr_amd.adjacency_from_edge_list(el)
+        with life_timer("build adjacency"):
+            graph = nwgr_amd.adjacency_from_edge_list(el)
         if a.verbose:
             print(f"adjacency: n={graph.n} m={graph.m}")
         if a.debug:
             print("indices:", " ".join(map(str, graph.offsets[:min(len(graph.offsets), 64)])))
         degrees = nwgr_amd.build_degrees(graph)
         rankings = np.zeros(graph.n, np.float32)
-        with PageRankGraph(graph, degrees) as pg:
+        with life_timer("device setup"):
+            pg = PageRankGraph(graph, degrees)
+        with pg:
             for thread in a.threads:
                 for vid in a.ids:
                     for _ in range(a.trials):

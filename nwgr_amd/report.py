@@ -1,5 +1,7 @@
 """Result reporting in the reference benchmark's formats.
 
+``life_timer`` prints the lifetime of a block as nw::util::life_timer does.
+
 ``Times`` collects per-trial seconds keyed by (file, version, threads) and
 prints the min/avg/max table of ``Times::print`` (bench/common.hpp:175-231).
 ``log_rows`` appends fixed-width result rows laid out as ``Log``
@@ -22,6 +24,26 @@ from typing import IO
 
 LIBRARY = "NWGR_AMD"
 CXX, CXX_ID, CXX_VER, BUILD, TBBMALLOC = "hipcc", "Clang", "rocm-7.2", "Release", "OFF"
+
+
+class life_timer:
+    """Prints "# [ name ]: <ms> ms" when the block ends, whole milliseconds, as
+    nw::util::life_timer does (util/timer.hpp:60-70)."""
+
+    def __init__(self, name: str, out: IO[str] | None = None):
+        self.name, self.out = name, out
+
+    def __enter__(self):
+        import time
+        self._t0 = time.perf_counter()
+        return self
+
+    def __exit__(self, *exc):
+        import time
+        if self.name:
+            ms = int((time.perf_counter() - self._t0) * 1e3)
+            (self.out or sys.stdout).write(f"# [ {self.name} ]: {ms} ms\n")
+        return False
 
 
 class Times:

@@ -3,7 +3,7 @@ the Times table of bench/common.hpp:205-231 and the Log rows of
 bench/Log.hpp:102-160, checked column by column."""
 import io
 
-from nwgr_amd.report import LIBRARY, Times, log_rows
+from nwgr_amd.report import LIBRARY, Times, life_timer, log_rows
 
 
 def test_times_table_layout():
@@ -53,3 +53,16 @@ def test_log_rows_columns(tmp_path):
     finally:
         sys.stdout = old
     assert len(out.getvalue().splitlines()) == 2
+
+
+def test_life_timer_line():
+    out = io.StringIO()
+    with life_timer("build adjacency", out):
+        pass
+    line = out.getvalue()
+    assert line.startswith("# [ build adjacency ]: ") and line.endswith(" ms\n")
+    assert int(line.split(": ")[1].split()[0]) >= 0
+    out = io.StringIO()
+    with life_timer("", out):
+        pass
+    assert out.getvalue() == ""
