@@ -16,7 +16,9 @@ host (pr.cpp:95 times the page_rank call; building the graph is outside it,
 and here the device upload and relabelling are too).  THREADS only label
 the rows: the device does not use host threads.  Prints the Times table
 (bench/common.hpp:205-231); --verify prints the ten best-ranked vertices as
-pr.cpp:47-52; --log appends Log.hpp rows ("Time(s)", "Tolerance").
+pr.cpp:47-52; --log appends Log.hpp rows ("Time(s)", "Tolerance").  The
+loading phases print life_timer lines ("# [ read mm ]: 12 ms") under the
+names bench/common.hpp gives them, plus "device setup" for the upload.
 """
 from __future__ import annotations
 
@@ -64,11 +66,11 @@ def main(argv=None) -> int:
     a = parse(sys.argv[1:] if argv is None else argv)
     times = Times()
     for file in a.files:
-        # the phases bench/common.hpp:82-107 times with life_timer
+        # the phases bench/common.hpp:83-114 times with life_timer
         with open(file, "rb") as f:
             binary = f.read(2) == b"NW"
         if not binary:
-            print(f"Reading matrix market input {file}")
+            print(f"Reading matrix market input {file} (slow)")
         with life_timer("deserialize" if binary else "read mm"):
             el = nwgr_amd.load_graph(file)
         if a.verbose:
@@ -79,7 +81,8 @@ def main(argv=None) -> int:
             print(f"adjacency: n={graph.n} m={graph.m}")
         if a.debug:
             print("indices:", " ".join(map(str, graph.offsets[:min(len(graph.offsets), 64)])))
-        degrees = nwgr_amd.build_degrees(graph)
+        with life_timer("degrees"):
+            degrees = nwgr_amd.build_degrees(graph)
         rankings = np.zeros(graph.n, np.float32)
         with life_timer("device setup"):
             pg = PageRankGraph(graph, degrees)

@@ -44,7 +44,7 @@ def test_pr_driver_versions_verify_and_log(tmp_path):
     assert sorted(r.split()[12] for r in rows[1:]) == ["v11", "v11", "v13", "v13"]
     assert all(r.split()[14] == "karate" for r in rows[1:])
     # life_timer lines of the phases (util/timer.hpp:60-70)
-    for phase in ("read mm", "build adjacency", "device setup"):
+    for phase in ("read mm", "build adjacency", "degrees", "device setup"):
         assert any(ln.startswith(f"# [ {phase} ]: ") and ln.endswith(" ms") for ln in text.splitlines()), phase
 
 
