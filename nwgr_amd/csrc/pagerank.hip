@@ -572,12 +572,13 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // The loads of one chunk for one lane: its kLaneEdges local ids (and
-// weights) plus the ordinal of its first fragment.  Issued ahead of use (see
-// k_tile).
+// weights), plus the ordinal of the chunk's first fragment (the same in every
+// lane; the lanes' own first ordinals follow from the start bits).  Issued
+// ahead of use (see k_tile).
 struct ChunkIn {
   u32x4 e0, e1;
   f32x4 w0, w1, w2, w3;
-  uint32_t sp0;
+  uint32_t base;
 };
 
 template <bool W>
@@ -593,7 +594,7 @@ __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uin
     in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 8));
     in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 12));
   }
-  in.sp0 = __builtin_nontemporal_load(a.lane_sp0 + lane);
+  in.base = a.lane_sp0[static_cast<uint64_t>(c) * kWave];  // lane 0's entry: a scalar load
   return in;
 }
 
@@ -621,104 +622,92 @@ constexpr uint64_t kRowLastLanes = 0x8000800080008000ull;
 // coalesced stores.  With a 2^15-entry tile the 16 windows fill the rest of
 // the 160 KiB of LDS; a chunk with more fragments passes through it in turns.
 constexpr uint32_t kStageW = 496;
+// After the windows, one slot per lane that takes the stores of running sums
+// that end no fragment, so every entry's store is unconditional (no lane-mask
+// switch per entry).  Never read.
+constexpr uint32_t kJunk = kWave;
+// Dynamic LDS of k_tile for a tile of 2^bits contributions.
+constexpr size_t tile_lds_bytes(uint32_t bits) {
+  return sizeof(float) * ((size_t{1} << bits) + kSweepThreads / kWave * kStageW + kJunk);
+}
+static_assert(tile_lds_bytes(kMaxTileBits) <= 160 * 1024, "k_tile: LDS of one CU");
 
 // Orders one wavefront's LDS accesses around it (no workgroup barrier).  The
 // LDS unit executes a wavefront's DS instructions in issue order, so only the
 // compiler must be kept from moving accesses across; a memory fence here would
 // also wait for the wavefront's outstanding global loads and stores (one
-// in-order counter on this target), draining the chunk prefetch.
+// counter for both on this target), draining the chunk prefetch.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __asm__ volatile("" ::: "memory");
 }
 
-// The per-entry pass of tile_lane over one LDS window: the running fragment
-// sums, and each sum that ends a fragment written at its window offset `qa`
-// (bytes; fragment ordinal relative to the window, times 4) -- for the
-// fragment running in from the previous lane, only its in-lane part.
-// WINDOWED: only offsets inside the window are written (the chunk has more
-// fragments than the window holds).
-template <bool WINDOWED>
-__device__ __forceinline__ void fragment_pass(float* stage, uint32_t q0, const float (&v)[16], const bool (&st)[16],
-                                              bool end15, float& tail, bool& any_start, bool& fix) {
-  char* const dst = reinterpret_cast<char*>(stage);
-  uint32_t qa = q0;
-  float acc = v[0];
-  bool seen = st[0];  // a fragment started in this lane at entries 0..i
-  fix = false;        // the fragment running in from the previous lane ends in this lane
+// LDS accesses at absolute LDS byte addresses (32-bit pointers of the LDS
+// address space), so a precomputed address costs no further add.
+typedef __attribute__((address_space(3))) float lds_f32;
+
+__device__ __forceinline__ float lds_ld(uint32_t b) { return *(const lds_f32*)static_cast<uintptr_t>(b); }
+
+__device__ __forceinline__ void lds_st(uint32_t b, float v) { *(lds_f32*)static_cast<uintptr_t>(b) = v; }
+
+// Word 3 of a raw buffer resource on this target (32-bit elements, no format
+// conversion).
+constexpr int kRawBufferWord3 = 0x00020000;
+
+// One window of fragment sums held in registers on its way out: lane l holds
+// slots l, l + 64, ..., l + 448 of partial[base, base + nw).
+struct WindowOut {
+  float o[8];
+  uint32_t base;
+  uint32_t nw;
+};
+
+// Reads the window at LDS address stage_b into registers.
+__device__ __forceinline__ void read_window(WindowOut& out, uint32_t stage_b, uint32_t wl) {
+  static_assert(kStageW <= 8 * kWave, "read_window: eight slots per lane");
+  // (opaque per call, so the per-slot offsets fold into the instructions'
+  // immediate offsets instead of living in registers across the chunk loop)
+  uint32_t l4 = wl * 4u;
+  __asm__("" : "+v"(l4));
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    if (i > 0) {
-      qa += st[i] ? 4u : 0u;
-      acc = st[i] ? v[i] : acc + v[i];
-      seen = seen || st[i];
-    }
-    const bool end = i < 15 ? st[i + 1] : end15;
-    if (end && (!WINDOWED || qa < kStageW * 4u)) *reinterpret_cast<float*>(dst + qa) = acc;
-    fix = fix || (end && !seen);
-  }
-  tail = acc;
-  any_start = seen;
+  for (uint32_t j = 0; j < 7; ++j) out.o[j] = lds_ld(stage_b + l4 + j * kWave * 4u);
+  out.o[7] = lds_ld(stage_b + min(7 * kWave * 4u + l4, (kStageW - 1) * 4u));
 }
 
-// One lane's share of a chunk: 16 entries, gathered from the LDS tile.
-//
-// Every per-entry predicate is a lane mask in SGPRs, so the per-entry work is
-// a handful of VALU instructions: the start flag (bit 15 of the entry) comes
-// from a sign compare; the running fragment sum, the fragment ordinal and the
-// "first fragment still open" tracker are selects on those masks; a
-// fragment's sum is written (at its ordinal) into the wavefront's LDS window
-// under the mask of the entries that end it, and the window leaves in
-// coalesced stores.  A fragment that runs in from the previous lane is
-// completed after a segmented scan over the lane's DPP row (groups are cut at
-// row boundaries): the carry of the lanes before is added to its in-lane sum
-// in the window, once per lane.
-//
-// Summation order, fixed: entries of a fragment in order within each lane;
-// a fragment spanning lanes adds the lanes' sums left to right.
-template <bool W>
-__device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __restrict__ s_tile, float* stage,
-                                          uint32_t mask, const ChunkIn& in, uint32_t wl) {
-  static_assert(kLaneEdges == 16 && kGroupEdges == 16 * kLaneEdges, "tile_lane: 16 entries, 16-lane rows");
-  const uint32_t words[8] = {in.e0.x, in.e0.y, in.e0.z, in.e0.w, in.e1.x, in.e1.y, in.e1.z, in.e1.w};
-  const float wt[16] = {in.w0.x, in.w0.y, in.w0.z, in.w0.w, in.w1.x, in.w1.y, in.w1.z, in.w1.w,
-                        in.w2.x, in.w2.y, in.w2.z, in.w2.w, in.w3.x, in.w3.y, in.w3.z, in.w3.w};
-  float v[16];
-  bool st[16];  // entry i starts a fragment
+// Stores a window to partial[base, base + nw) with a fixed count of stores:
+// raw buffer stores bounded to nw elements, so the lanes past nw are dropped
+// by the bounds check instead of branched around (nw = 0 stores nothing).
+__device__ __forceinline__ void store_window(float* partial, const WindowOut& out, uint32_t wl) {
+  // (base and nw are wave-uniform; the compiler is told so, or it would
+  // build the resource per lane and loop over its distinct values)
+  const uint32_t base = __builtin_amdgcn_readfirstlane(out.base);
+  const uint32_t nw = __builtin_amdgcn_readfirstlane(out.nw);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(partial + base, 0, static_cast<int>(nw * 4u), kRawBufferWord3);
+  uint32_t l4 = wl * 4u;
+  __asm__("" : "+v"(l4));
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t w = words[i >> 1];
-    const uint32_t e = (i & 1) ? (w >> 16) : (w & 0xffffu);
-    st[i] = (i & 1) ? static_cast<int32_t>(w) < 0 : static_cast<int16_t>(w & 0xffffu) < 0;
-    v[i] = s_tile[e & mask];
-  }
-  if constexpr (W) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] *= wt[i];
-  }
-  // the chunk's first ordinal (lane 0, entry 0 starts a fragment) and its
-  // fragment count, from lane 63's first ordinal and its starts
-  const uint32_t base = __builtin_amdgcn_readfirstlane(in.sp0);
-  uint32_t cnt = 1u + static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(in.sp0), kWave - 1)) - base;
-#pragma unroll
-  for (int i = 1; i < 16; ++i) cnt += static_cast<uint32_t>(__builtin_amdgcn_ballot_w64(st[i]) >> 63);
-  uint32_t q0 = (in.sp0 - base) * 4u;  // byte offset of the fragment holding entry 0
-  // (opaque to the compiler: otherwise it re-derives every entry's offset from
-  // sp0 and base, one extra subtraction per entry)
-  __asm__("" : "+v"(q0));
-  // entry 15 ends a fragment when the next lane's entry 0 starts one, or at a row end
-  const bool end15 = lane_bit((__builtin_amdgcn_ballot_w64(st[0]) >> 1) | kRowLastLanes);
-  float tail, carry = 0.f;
-  bool seen, fix;
-  for (uint32_t w0 = 0; w0 < cnt; w0 += kStageW) {
-    if (cnt <= kStageW)
-      fragment_pass<false>(stage, q0, v, st, end15, tail, seen, fix);
-    else
-      fragment_pass<true>(stage, q0 - w0 * 4u, v, st, end15, tail, seen, fix);
-    if (w0 == 0) {
-      // inclusive segmented scan of the open tail sums over the 16 lanes of the row
-      float tv = tail;
-      uint32_t tf = seen ? 1u : 0u;
+  for (uint32_t j = 0; j < 8; ++j)
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(out.o[j]), rs, static_cast<int>(l4 + j * kWave * 4u), 0, 0);
+}
+
+// Waits for every outstanding global load and store of the wavefront.  Loads
+// and stores share one counter and may complete out of order, so a wait for
+// one load after a store is a wait for both: the chunk loop waits once per
+// batch, before it issues the next batch's loads and the previous batch's
+// stores, so the wait covers only operations issued a whole batch earlier.
+__device__ __forceinline__ void wait_global() {
+  // s_waitcnt vmcnt(0), expcnt and lgkmcnt at their maxima (no wait)
+  __builtin_amdgcn_s_waitcnt(0x0f70);
+}
+
+// Carry of a fragment that runs into a lane from the lanes before it in the
+// lane's DPP row: an inclusive segmented scan of the lanes' open tail sums
+// (`seen`: a fragment starts in the lane), shifted by one lane (0 in the
+// row's first lane).
+__device__ __forceinline__ float row_carry(float tail, bool seen) {
+  float tv = tail;
+  uint32_t tf = seen ? 1u : 0u;
 #define NWGR_SCAN_STEP(D)                                 \
   {                                                       \
     const float ov = dpp_f32<kDppRowShr | (D)>(tv);       \
@@ -726,55 +715,196 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, const float* __rest
     tv = tf ? tv : tv + ov;                               \
     tf |= of;                                             \
   }
-      NWGR_SCAN_STEP(1)
-      NWGR_SCAN_STEP(2)
-      NWGR_SCAN_STEP(4)
-      NWGR_SCAN_STEP(8)
+  NWGR_SCAN_STEP(1)
+  NWGR_SCAN_STEP(2)
+  NWGR_SCAN_STEP(4)
+  NWGR_SCAN_STEP(8)
 #undef NWGR_SCAN_STEP
-      carry = dpp_f32<kDppRowShr | 1>(tv);  // 0 in the row's first lane
-    }
-    // the fragment running in from the previous lane: its in-lane sum was
-    // written at its end; add the carry of the lanes before
-    const uint32_t qf = q0 - w0 * 4u;
-    if (fix && qf < kStageW * 4u) stage[qf / 4u] += carry;
-    wave_sync();
-    const uint32_t nw = min(kStageW, cnt - w0);
+  return dpp_f32<kDppRowShr | 1>(tv);
+}
+
+// Inclusive prefix sum of x over the 64 lanes: Hillis-Steele steps inside
+// each 16-lane DPP row, then the totals of the rows before added from lanes
+// 15, 31 and 47.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += dpp_u32<kDppRowShr | 1>(x);
+  x += dpp_u32<kDppRowShr | 2>(x);
+  x += dpp_u32<kDppRowShr | 4>(x);
+  x += dpp_u32<kDppRowShr | 8>(x);
+  const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), 15));
+  const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), 31));
+  const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), 47));
+  return x + (lane_bit(0xffffffffffff0000ull) ? r0 : 0u) + (lane_bit(0xffffffff00000000ull) ? r1 : 0u) +
+         (lane_bit(0xffff000000000000ull) ? r2 : 0u);
+}
+
+// One lane's share of a chunk: 16 entries, gathered from the LDS tile at LDS
+// address lds0.
+//
+// Every per-entry predicate is a wave mask in SGPRs, taken once by a ballot
+// (entry i starts a fragment: bit 15 of its id), so the per-entry work is a
+// handful of VALU instructions: the tile address, the running fragment sum
+// and the fragment's slot are selects and adds on those masks.  Each entry's
+// running sum is stored: at its fragment's slot in the wavefront's LDS window
+// when the entry ends the fragment, else at the lane's junk slot, so no store
+// needs a lane-mask switch.  The window leaves in coalesced stores.  A
+// fragment that runs in from the previous lane is completed after a segmented
+// scan over the lane's DPP row (groups are cut at row boundaries): the carry
+// of the lanes before is added to its in-lane sum in the window, once per lane.
+//
+// Summation order, fixed: entries of a fragment in order within each lane;
+// a fragment spanning lanes adds the lanes' sums left to right.
+template <bool W>
+__device__ __forceinline__ void tile_lane(const TileArgs& a, uint32_t lds0, uint32_t stage_b, uint32_t junk_b,
+                                          uint32_t mask, const ChunkIn& in, uint32_t wl, WindowOut& out) {
+  static_assert(kLaneEdges == 16 && kGroupEdges == 16 * kLaneEdges, "tile_lane: 16 entries, 16-lane rows");
+  const uint32_t words[8] = {in.e0.x, in.e0.y, in.e0.z, in.e0.w, in.e1.x, in.e1.y, in.e1.z, in.e1.w};
+  const float wt[16] = {in.w0.x, in.w0.y, in.w0.z, in.w0.w, in.w1.x, in.w1.y, in.w1.z, in.w1.w,
+                        in.w2.x, in.w2.y, in.w2.z, in.w2.w, in.w3.x, in.w3.y, in.w3.z, in.w3.w};
+  float v[16];
+  uint64_t sm[16];  // lanes whose entry i starts a fragment
 #pragma unroll
-    for (uint32_t j = 0; j < kStageW; j += kWave)
-      if (j + wl < nw) a.partial[base + w0 + j + wl] = stage[j + wl];
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t w = words[k];
+    v[2 * k] = lds_ld(lds0 + ((w & mask) << 2));
+    v[2 * k + 1] = lds_ld(lds0 + (((w >> 16) & mask) << 2));
+    sm[2 * k] = __builtin_amdgcn_ballot_w64((w & 0x8000u) != 0);
+    sm[2 * k + 1] = __builtin_amdgcn_ballot_w64(static_cast<int32_t>(w) < 0);
+  }
+  if constexpr (W) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] *= wt[i];
+  }
+  // the chunk's first ordinal (lane 0, entry 0 starts a fragment); each
+  // lane's count of fragment starts and their inclusive prefix over the wave:
+  // the chunk's fragment count, and the lane's first fragment's ordinal
+  const uint32_t base = __builtin_amdgcn_readfirstlane(in.base);
+  uint32_t nst = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) nst += static_cast<uint32_t>(__builtin_popcount(words[k] & 0x80008000u));
+  const uint32_t incl = wave_incl_scan(nst);
+  const uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+  uint64_t seen_m = sm[0];  // lanes where a fragment starts
+#pragma unroll
+  for (int i = 1; i < 16; ++i) seen_m |= sm[i];
+  // entry 15 ends a fragment when the next lane's entry 0 starts one, or at a row end
+  const uint64_t end15_m = (sm[0] >> 1) | kRowLastLanes;
+  // lanes where the fragment running in from the previous lane ends
+  const bool fix = lane_bit(~sm[0] & (seen_m | end15_m));
+  const bool seen = lane_bit(seen_m);
+  // LDS address of the slot of entry 0's fragment: the starts before this
+  // lane, plus one if entry 0 starts a fragment, minus one
+  uint32_t q0 = stage_b + (incl - nst) * 4u + ((words[0] >> 13) & 4u) - 4u;
+  // (opaque to the compiler: otherwise it re-derives every entry's address
+  // from sp0 and base)
+  __asm__("" : "+v"(q0));
+  if (cnt <= kStageW) {
+    uint32_t qa = q0;
+    float acc = v[0];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i > 0) {
+        qa += lane_bit(sm[i]) ? 4u : 0u;
+        acc = lane_bit(sm[i]) ? v[i] : acc + v[i];
+      }
+      lds_st(lane_bit(i < 15 ? sm[i + 1] : end15_m) ? qa : junk_b, acc);
+    }
+    const float carry = row_carry(acc, seen);
+    if (fix) lds_st(q0, lds_ld(q0) + carry);
     wave_sync();
+    read_window(out, stage_b, wl);
+    out.base = base;
+    out.nw = cnt;
+    wave_sync();
+  } else {
+    // more fragments than the window holds: one pass per window, each
+    // keeping the sums whose slots fall inside it; all windows but the last
+    // are stored at once
+    float tail = 0.f;
+    for (uint32_t w0 = 0;; w0 += kStageW) {
+      const uint32_t lo = stage_b + w0 * 4u, hi = lo + kStageW * 4u;  // slot addresses of this window
+      uint32_t qa = q0;
+      float acc = v[0];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        if (i > 0) {
+          qa += lane_bit(sm[i]) ? 4u : 0u;
+          acc = lane_bit(sm[i]) ? v[i] : acc + v[i];
+        }
+        const bool end = lane_bit(i < 15 ? sm[i + 1] : end15_m);
+        lds_st(end && qa >= lo && qa < hi ? qa - w0 * 4u : junk_b, acc);
+      }
+      if (w0 == 0) tail = row_carry(acc, seen);
+      if (fix && q0 >= lo && q0 < hi) lds_st(q0 - w0 * 4u, lds_ld(q0 - w0 * 4u) + tail);
+      wave_sync();
+      read_window(out, stage_b, wl);
+      out.base = base + w0;
+      out.nw = min(kStageW, cnt - w0);
+      wave_sync();
+      if (w0 + kStageW >= cnt) break;
+      store_window(a.partial, out, wl);
+    }
   }
 }
 
 // The chunks of one task taken by one wavefront (lo + wv, lo + wv + 16, ...),
-// NB per batch, double-buffered: the loads of the next batch are issued --
-// unconditionally, with clamped chunk ids -- before the current batch is
-// reduced, so the waits for them are exact.
+// NB per batch, double-buffered.  Each batch starts with one wait for all of
+// the wavefront's global operations (its own loads, issued a batch earlier,
+// and the previous batch's stores), then issues the next batch's loads --
+// unconditionally, with clamped chunk ids -- and the previous batch's stores
+// of fragment sums, which waited in registers, and then reduces its chunks.
+// A batch past the end of the range is reduced by a branch that leaves the
+// loop; its missing chunks store nothing.
 template <bool W, int NB>
-__device__ __forceinline__ void run_chunks(const TileArgs& a, const float* __restrict__ s_tile, float* stage,
+__device__ __forceinline__ void run_chunks(const TileArgs& a, uint32_t lds0, uint32_t stage_b, uint32_t junk_b,
                                            uint32_t mask, uint32_t first, uint32_t hi, uint32_t wl) {
   constexpr uint32_t S = kSweepThreads / kWave;
   if (first >= hi) return;
   const uint32_t last = hi - 1;
   ChunkIn A[NB], B[NB];
+  WindowOut P[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) P[j] = WindowOut{{}, 0u, 0u};
 #pragma unroll
   for (int j = 0; j < NB; ++j) A[j] = load_chunk<W>(a, min(first + j * S, last), wl);
   for (uint32_t c = first;;) {
+    wait_global();
 #pragma unroll
     for (int j = 0; j < NB; ++j) B[j] = load_chunk<W>(a, min(c + (NB + j) * S, last), wl);
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (c + j * S < hi) tile_lane<W>(a, s_tile, stage, mask, A[j], wl);
+    for (int j = 0; j < NB; ++j) store_window(a.partial, P[j], wl);
+    if (c + (NB - 1) * S >= hi) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        P[j].nw = 0;
+        if (c + j * S < hi) tile_lane<W>(a, lds0, stage_b, junk_b, mask, A[j], wl, P[j]);
+      }
+      break;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) tile_lane<W>(a, lds0, stage_b, junk_b, mask, A[j], wl, P[j]);
     c += NB * S;
     if (c >= hi) break;
+    wait_global();
 #pragma unroll
     for (int j = 0; j < NB; ++j) A[j] = load_chunk<W>(a, min(c + (NB + j) * S, last), wl);
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (c + j * S < hi) tile_lane<W>(a, s_tile, stage, mask, B[j], wl);
+    for (int j = 0; j < NB; ++j) store_window(a.partial, P[j], wl);
+    if (c + (NB - 1) * S >= hi) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        P[j].nw = 0;
+        if (c + j * S < hi) tile_lane<W>(a, lds0, stage_b, junk_b, mask, B[j], wl, P[j]);
+      }
+      break;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) tile_lane<W>(a, lds0, stage_b, junk_b, mask, B[j], wl, P[j]);
     c += NB * S;
     if (c >= hi) break;
   }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) store_window(a.partial, P[j], wl);
 }
 
 // Tile copy into LDS: all J loads of a thread are issued before its LDS
@@ -794,24 +924,32 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   constexpr int kNW = kSweepThreads / kWave;
   // All LDS is dynamic, so the tile starts at address 0 (its gathers need no
   // base) and is 16-byte aligned for the float4 fill: 2^bits floats of tile,
-  // then kStageW floats per wavefront.  The convergence reduction runs before
-  // the first tile and borrows the start of the stage area.
+  // then kStageW floats per wavefront, then the kJunk slots.  The convergence
+  // reduction runs before the first tile and borrows the start of the stage
+  // area.
   extern __shared__ __align__(16) float s_tile[];
   double(*s_red)[kNW] = reinterpret_cast<double(*)[kNW]>(s_tile + (1u << a.bits));
   uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_red + 2);
   const uint32_t tid = threadIdx.x;
   const uint32_t wl = tid & (kWave - 1);
   const uint32_t wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+  // consecutive task ranges (mostly the same tile) go to workgroups of one
+  // XCD, so a tile's contributions are fetched into that XCD's L2 once
+  const uint32_t G = gridDim.x;
+  const uint32_t vb = G % kXcds == 0 ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+  // the task range, the stop flag and the previous sweep's pairs are loaded
+  // together: one memory round trip before the first tile, not three
+  const uint32_t t_lo = a.wg_task[vb], t_hi = a.wg_task[vb + 1];
   if constexpr (KIND != kCoef) {
-    if (tid == 0) *s_flag = a.st->done;
-    __syncthreads();
-    if (*s_flag) return;
     // totals of the previous sweep, identical in every workgroup
     double E = 0, D = 0;
     for (uint32_t b = tid; b < a.nparts; b += kSweepThreads) {
       E += a.part_in[2 * b];
       D += a.part_in[2 * b + 1];
     }
+    if (tid == 0) *s_flag = a.st->done;
+    __syncthreads();
+    if (*s_flag) return;
     E = wave_sum(E);
     D = wave_sum(D);
     if (wl == 0) {
@@ -856,11 +994,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
 
   const uint32_t mask = (1u << a.bits) - 1;
   const uint32_t nq = (mask + 1) / 4;  // float4 per tile
-  // consecutive task ranges (mostly the same tile) go to workgroups of one
-  // XCD, so a tile's contributions are fetched into that XCD's L2 once
-  const uint32_t G = gridDim.x;
-  const uint32_t vb = G % kXcds == 0 ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds : blockIdx.x;
-  const uint32_t t_lo = a.wg_task[vb], t_hi = a.wg_task[vb + 1];
+  // LDS address of the tile (held in a register: a constant LDS base would be
+  // added again at every access)
+  uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_f32*)s_tile));
+  __asm__("" : "+s"(lds0));
   for (uint32_t ti = t_lo; ti < t_hi; ++ti) {
     const TiledTask task = a.tasks[ti];
     if (ti > t_lo) __syncthreads();  // every wave is done with the previous tile
@@ -875,8 +1012,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
         fill_tile<kMaxTile / 16 / kSweepThreads>(src, dst, tid);
     }
     __syncthreads();
-    run_chunks<kW, kW ? 1 : 2>(a, s_tile, s_tile + mask + 1 + wv * kStageW, mask, task.chunk_lo + wv, task.chunk_hi,
-                               wl);
+    // one chunk per batch: two per batch held twice the registers and ran
+    // slower (R-MAT s22: 61.0 against 58.5 us per launch)
+    run_chunks<kW, 1>(a, lds0, lds0 + (mask + 1 + wv * kStageW) * 4u, lds0 + (mask + 1 + kNW * kStageW + wl) * 4u,
+                      mask, task.chunk_lo + wv, task.chunk_hi, wl);
   }
 }
 
@@ -1248,7 +1387,7 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
 // One tiled sweep: k_tile over the workgroup task ranges, then k_combine.
 void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
   static const bool attrs = [] {
-    const int bytes = static_cast<int>((kMaxTile + kSweepThreads / kWave * kStageW) * sizeof(float));
+    const int bytes = static_cast<int>(tile_lds_bytes(kMaxTileBits));
     for (const void* f : {reinterpret_cast<const void*>(&k_tile<kPlain>),
                           reinterpret_cast<const void*>(&k_tile<kWeighted>),
                           reinterpret_cast<const void*>(&k_tile<kAdjoint>),
@@ -1285,7 +1424,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ca.wave_rows = t.wave_rows.as<uint32_t>();
   ca.nwave_rows = t.nwave_rows;
   ca.row0 = t.row0;
-  const size_t shm = sizeof(float) * ((size_t{1} << t.bits) + kSweepThreads / kWave * kStageW);
+  const size_t shm = tile_lds_bytes(t.bits);
   const uint32_t nb = t.nblocks;
   if (t.acc_on) {
     static const bool acc_attrs = [] {
