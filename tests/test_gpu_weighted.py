@@ -113,6 +113,43 @@ def test_autograd_matches_fp64_adjoint_and_fd(mode):
     assert efd <= 1e-4
 
 
+@pytest.mark.parametrize("combine", [0, 1])
+def test_weighted_gradients_low_locality_tiled(combine):
+    """Forward and gradients through the tiled engine on a graph whose chunks
+    hold more fragments than one LDS window (about one entry per row and
+    2^13-wide tile), with the gathering and the accumulating second pass."""
+    say(f"weighted low-locality tiled combine={combine}")
+    from nwgr_amd.autograd import weighted_pagerank
+    n = 1 << 16
+    rng = np.random.default_rng(11)
+    g = nwgr_amd.build_adjacency(rng.integers(0, n, 8 * n), rng.integers(0, n, 8 * n), n=n, dedup=True,
+                                 drop_self_loops=True)
+    w = (1.0 - rng.random(g.m)).astype(np.float32)
+    t = rng.dirichlet(np.ones(g.n)).astype(np.float32)
+    wl = rng.standard_normal(g.n).astype(np.float32)
+    pg = PageRankGraph(g)
+    pg.tuning = dict(engine=1, tile_bits=13, tile_fill_cost=0, tile_combine=combine)
+    w64, t64 = w.astype(np.float64), t.astype(np.float64)
+    # forward at the matched sweep count
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    res = pg.wpr_forward(dev(w).data_ptr(), out.data_ptr(), tol=1e-6, max_iters=200, dangling="uniform",
+                         teleport_ptr=dev(t).data_ptr())
+    torch.cuda.synchronize()
+    assert pg.last_engine().startswith("tiled")
+    rk, _ = oracle.wpr_forward(g.offsets, g.indices, w64, t64, D, 0.0, res.iters, oracle.UNIFORM, nthreads=8)
+    assert l1(out.cpu().numpy(), rk) <= 1e-6
+    # gradients
+    wd = dev(w).requires_grad_(True)
+    td = dev(t).requires_grad_(True)
+    r = weighted_pagerank(pg, wd, td, tol=1e-7, max_iters=300, dangling="uniform", adj_tol=1e-7)
+    (r * dev(wl)).sum().backward()
+    r64, _ = oracle.wpr_forward(g.offsets, g.indices, w64, t64, D, 1e-15, 5000, oracle.UNIFORM, nthreads=8)
+    _, dw64, dt64, _ = oracle.wpr_adjoint(g.offsets, g.indices, w64, t64, r64, wl.astype(np.float64), D, 1e-13,
+                                          5000, oracle.UNIFORM, nthreads=8)
+    assert rel2(wd.grad.cpu().numpy(), dw64) <= 1e-4
+    assert rel2(td.grad.cpu().numpy(), dt64) <= 1e-4
+
+
 def test_adjoint_without_teleport_grad():
     from nwgr_amd.autograd import weighted_pagerank
     g, w, _, wl = case(10, seed=3)
