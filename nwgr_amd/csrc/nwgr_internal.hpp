@@ -223,9 +223,9 @@ struct Plan {
 // copies the tile's contributions into LDS and gathers from there.  Each lane
 // reduces its entries; runs crossing lanes are joined by a segmented scan
 // over the 16 lanes of a DPP row (no LDS traffic), and every fragment's sum
-// is stored at the fragment's ordinal (`lane_sp0`: ordinal of the fragment
-// holding each lane's first entry), so a chunk's partials are written
-// contiguously.  The combine pass adds each row's partials in tile order
+// is stored at the fragment's ordinal (the chunk's first ordinal, `lane_sp0`
+// at its lane 0, plus the fragment starts before it in the chunk), so a
+// chunk's partials are written contiguously.  The combine pass adds each row's partials in tile order
 // (`rslot` lists them row by row) and applies the row epilogue.
 constexpr uint32_t kMaxTileBits = 15;          // local ids use 15 bits of a u16
 constexpr uint32_t kMaxTile = 1u << kMaxTileBits;

@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
 struct TileArgs {
   const uint16_t* __restrict__ eloc;
   const float* __restrict__ w;  // weights in tiled order (weighted kinds)
-  const uint32_t* __restrict__ lane_sp0;
+  const uint32_t* __restrict__ lane_sp0;  // read at lane 0 of each chunk: the chunk's first ordinal
   const TiledTask* __restrict__ tasks;
   const uint32_t* __restrict__ wg_task;
   const float* __restrict__ c_in;  // padded to whole tiles
@@ -594,7 +594,7 @@ __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uin
     in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 8));
     in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 12));
   }
-  in.base = a.lane_sp0[static_cast<uint64_t>(c) * kWave];  // lane 0's entry: a scalar load
+  in.base = a.lane_sp0[static_cast<uint64_t>(c) * kWave];  // lane 0's entry, one address for the wave
   return in;
 }
 
