@@ -38,6 +38,7 @@ constexpr uint32_t kMultiMaxN = 65536;    // vertices per graph (16-bit local id
 constexpr uint32_t kMultiLdsMax = 36864;  // contributions mirrored in LDS at most (144 KiB)
 constexpr uint32_t kDLeak = 0, kDUniform = 1, kDTeleport = 2;
 constexpr int kEntriesPerStep = 4;  // entries of a row loaded together
+constexpr int kHold = 4;            // rows of a thread whose results are stored together
 
 struct MultiArgs {
   const uint32_t* __restrict__ offs;   // n + 1 (global)
@@ -167,25 +168,34 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     float* __restrict__ lnext = cl + (p ? 0u : half);
     const float Df = static_cast<float>(D);
     double err = 0, dang = 0;
-    auto finish = [&](uint32_t v, float z, float t, float dg, float ro) {
+    // row epilogue: the new rank (returned), its contribution (returned, and
+    // kept in LDS for kMirrorDouble), residual and dangling mass
+    auto epilogue = [&](uint32_t v, float z, float t, float dg, float ro, float& c) {
       const float base = a.tel ? a.one_minus_d * t : base_u;
       const float u = a.dangling == kDUniform ? un : (a.dangling == kDTeleport ? t : 0.f);
       const float s = fmaf(Df, u, z);
       const float rn = fmaf(a.d_hi, s, fmaf(a.d_lo, s, base));
-      rout[v] = rn;
-      if constexpr (M == kMirrorDouble)
-        lnext[v] = rn / dg;
-      else
-        cout[v] = rn / dg;
+      c = rn / dg;
+      if constexpr (M == kMirrorDouble) lnext[v] = c;
       err += fabs(static_cast<double>(rn) - static_cast<double>(ro));
       if (dg == 0.f) dang += static_cast<double>(rn);
+      return rn;
+    };
+    auto finish = [&](uint32_t v, float z, float t, float dg, float ro) {
+      float c;
+      rout[v] = epilogue(v, z, t, dg, ro, c);
+      if constexpr (M != kMirrorDouble) cout[v] = c;
     };
     // Rows of ordinary length, a thread each, in a two-stage pipeline: the
     // bounds and epilogue operands of the row after next and the first
     // kEntriesPerStep entries of the next row are loaded while the current
     // row is gathered.  Entries are loaded kEntriesPerStep at a time with no
     // bounds branch (loads past a row's end read the padding of lidx and are
-    // masked to 0).  Each row adds its entries in order.
+    // masked to 0).  Each row adds its entries in order.  The results of
+    // kHold consecutive rows of a thread wait in registers and are stored
+    // together: loads and stores share one counter and may complete out of
+    // order, so the wait for a prefetched load after a store waits for the
+    // store too -- once per kHold rows instead of once per row.
     const uint32_t pad = a.m;  // an index into the padding of lidx
     struct RowOps {
       uint32_t b, e;
@@ -209,33 +219,50 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     RowOps cur = row_in(tid), nxt = row_in(tid + kMT);
     uint32_t jc[kEntriesPerStep];
     first_entries(cur, jc);
-    for (uint32_t v = tid; v < ng; v += kMT) {
-      const RowOps nn = row_in(v + 2 * kMT);
-      uint32_t jn[kEntriesPerStep];
-      first_entries(nxt, jn);
-      if (cur.e - cur.b <= a.heavy_min) {
-        float z = 0.f;
+    for (uint32_t vh = tid; vh < ng; vh += kHold * kMT) {
+      float hr[kHold], hc[kHold];
+      bool hk[kHold];
 #pragma unroll
-        for (int q = 0; q < kEntriesPerStep; ++q) {
-          const float x = cget<M>(lcur, cg, K, jc[q]);
-          z += cur.b + q < cur.e ? x : 0.f;
-        }
-        for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
-          uint32_t j[kEntriesPerStep];
+      for (int h = 0; h < kHold; ++h) {
+        const uint32_t v = vh + h * kMT;
+        hk[h] = false;
+        if (v < ng) {
+          const RowOps nn = row_in(v + 2 * kMT);
+          uint32_t jn[kEntriesPerStep];
+          first_entries(nxt, jn);
+          if (cur.e - cur.b <= a.heavy_min) {
+            float z = 0.f;
 #pragma unroll
-          for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
+            for (int q = 0; q < kEntriesPerStep; ++q) {
+              const float x = cget<M>(lcur, cg, K, jc[q]);
+              z += cur.b + q < cur.e ? x : 0.f;
+            }
+            for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
+              uint32_t j[kEntriesPerStep];
 #pragma unroll
-          for (int q = 0; q < kEntriesPerStep; ++q) {
-            const float x = cget<M>(lcur, cg, K, j[q]);
-            z += k + q < cur.e ? x : 0.f;
+              for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
+#pragma unroll
+              for (int q = 0; q < kEntriesPerStep; ++q) {
+                const float x = cget<M>(lcur, cg, K, j[q]);
+                z += k + q < cur.e ? x : 0.f;
+              }
+            }
+            hr[h] = epilogue(v, z, cur.t, cur.dg, cur.ro, hc[h]);
+            hk[h] = true;
           }
-        }
-        finish(v, z, cur.t, cur.dg, cur.ro);
-      }
-      cur = nxt;
-      nxt = nn;
+          cur = nxt;
+          nxt = nn;
 #pragma unroll
-      for (int q = 0; q < kEntriesPerStep; ++q) jc[q] = jn[q];
+          for (int q = 0; q < kEntriesPerStep; ++q) jc[q] = jn[q];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < kHold; ++h) {
+        if (hk[h]) {
+          rout[vh + h * kMT] = hr[h];
+          if constexpr (M != kMirrorDouble) cout[vh + h * kMT] = hc[h];
+        }
+      }
     }
     for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
       const uint32_t v = a.heavy[h] - v0;
