@@ -562,10 +562,12 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   NWGR_HIP(hipMemsetAsync(mg.queue.p, 0, sizeof(uint32_t), s));
   NWGR_HIP(hipEventRecord(e0, s));
   const size_t shm = sizeof(float) * mg.lds_max_n;
-  // two workgroups per CU when their LDS mirrors fit beside each other (the
-  // 1024-thread workgroups then fill the CU's wavefront slots)
-  const uint32_t per_cu = shm + 2048 <= 80 * 1024 ? 2u : 1u;
-  k_multi<<<std::min<uint32_t>(mg.num_cu * per_cu, mg.count), kMT, shm, s>>>(a);
+  // one persistent workgroup per CU: a 1024-thread workgroup at this kernel's
+  // register count (over 64 VGPRs) takes all of a CU's wavefront slots, so a
+  // second one could not run beside it even when two LDS mirrors would fit
+  // (capping the kernel at 64 VGPRs spills; 512-thread workgroups two per CU
+  // measured 317 against 380 GTEPS on 2,048 R-MAT graphs)
+  k_multi<<<std::min<uint32_t>(mg.num_cu, mg.count), kMT, shm, s>>>(a);
   const hipError_t le = hipGetLastError();
   NWGR_HIP(hipEventRecord(e1, s));
   NWGR_HIP(hipEventSynchronize(e1));
