@@ -234,6 +234,22 @@ constexpr uint32_t kGroupEdges = 16 * kLaneEdges;
 constexpr uint32_t kChunkEdges = 64 * kLaneEdges;
 constexpr uint32_t kWaveRowMin = 32;           // rows with more partials are combined by a wavefront
 
+// Storage order inside a chunk.  The layout is built in logical positions
+// (entry i of lane l of chunk c at c*1024 + 16 l + i); the local ids and the
+// tiled weights are stored permuted so that every 16-byte load of a
+// wavefront reads one contiguous 1 KiB: ids as two halves (entries 0-7 of
+// every lane, then 8-15), weights as four quarters (entries 4q .. 4q+3).
+__host__ __device__ constexpr uint64_t eloc_pos(uint64_t P) {
+  const uint32_t r = static_cast<uint32_t>(P % kChunkEdges), lane = r / kLaneEdges, i = r % kLaneEdges;
+  return P - r + (i / 8) * (kChunkEdges / 2) + lane * 8 + i % 8;
+}
+__host__ __device__ constexpr uint64_t wgt_pos(uint64_t P) {
+  const uint32_t r = static_cast<uint32_t>(P % kChunkEdges), lane = r / kLaneEdges, i = r % kLaneEdges;
+  return P - r + (i / 4) * (kChunkEdges / 4) + lane * 4 + i % 4;
+}
+static_assert(eloc_pos(16 + 9) == 512 + 8 + 1 && wgt_pos(16 + 9) == 512 + 4 + 1 && eloc_pos(1024 + 7) == 1024 + 7,
+              "chunk storage order");
+
 struct TiledTask {
   uint32_t tile;
   uint32_t chunk_lo, chunk_hi;  // chunks [chunk_lo, chunk_hi) of this tile

@@ -584,15 +584,16 @@ struct ChunkIn {
 template <bool W>
 __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uint32_t wl) {
   ChunkIn in;
-  const uint64_t lane = static_cast<uint64_t>(c) * kWave + wl;
-  const uint64_t base = lane * kLaneEdges;
-  in.e0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base));
-  in.e1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + base + 8));
+  // chunk storage order (eloc_pos, wgt_pos): each load below reads 1 KiB
+  // contiguous over the wavefront
+  const uint64_t cb = static_cast<uint64_t>(c) * kChunkEdges;
+  in.e0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + cb + wl * 8));
+  in.e1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.eloc + cb + kChunkEdges / 2 + wl * 8));
   if constexpr (W) {
-    in.w0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base));
-    in.w1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 4));
-    in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 8));
-    in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + base + 12));
+    in.w0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + wl * 4));
+    in.w1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + kChunkEdges / 4 + wl * 4));
+    in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + kChunkEdges / 2 + wl * 4));
+    in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + 3 * kChunkEdges / 4 + wl * 4));
   }
   in.base = a.lane_sp0[static_cast<uint64_t>(c) * kWave];  // lane 0's entry, one address for the wave
   return in;

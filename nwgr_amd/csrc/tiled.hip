@@ -87,7 +87,7 @@ __global__ void k_tile_place(const uint32_t* __restrict__ tkey, const uint32_t* 
     const uint32_t e = tperm[k];
     const uint32_t row = erow[e];
     const bool first = i == 0 || (P % kGroupEdges) == 0 || erow[tperm[k - 1]] != row;
-    eloc[P] = static_cast<uint16_t>((idx[e] & mask) | (first ? 0x8000u : 0u));
+    eloc[eloc_pos(P)] = static_cast<uint16_t>((idx[e] & mask) | (first ? 0x8000u : 0u));
     epos[P] = e;
     flagv[P] = first ? 1u : 0u;
   }
@@ -102,7 +102,7 @@ __global__ void k_pad_flags(const uint32_t* __restrict__ tile_cnt, const uint32_
     const uint64_t hi = static_cast<uint64_t>(tile_chunk0[t + 1]) * kChunkEdges;
     for (uint64_t P = lo; P < hi; ++P)
       if (P == lo || P % kGroupEdges == 0) {
-        eloc[P] = 0x8000u;
+        eloc[eloc_pos(P)] = 0x8000u;
         flagv[P] = 1u;
       }
   }
@@ -150,9 +150,10 @@ __global__ __launch_bounds__(kT) void k_count_runs(const uint32_t* __restrict__ 
   if (wl == 0 && cnt) atomicAdd(runs, static_cast<unsigned long long>(cnt));
 }
 
+// Weights in tiled order (stored in the chunk storage order of wgt_pos).
 __global__ void k_gather_w(const float* __restrict__ src, const uint32_t* __restrict__ pos, uint64_t count,
                            float* __restrict__ dst) {
-  GRID_STRIDE(i, count) dst[i] = src[pos[i]];
+  GRID_STRIDE(i, count) dst[wgt_pos(i)] = src[pos[i]];
 }
 
 // Accumulate form of the second pass.  Row of every fragment from the
