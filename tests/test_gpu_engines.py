@@ -50,6 +50,24 @@ def test_engine_parity(gname, ename):
         assert pg.last_engine() == "binned" if ename == "binned" else pg.last_engine().startswith("tiled")
 
 
+@pytest.mark.parametrize("n,bits", [(32767, 15), (32768, 15), (32769, 15), (65537, 15), (8193, 13), (16385, 14)])
+def test_tiled_vertex_counts_at_tile_boundaries(n, bits):
+    """Vertex counts one below, at and one above a tile width: the last tile
+    (partly padding, or one vertex) and the quiet rows behind it."""
+    say(f"tiled bits {bits} n {n}")
+    rng = np.random.default_rng(n)
+    m = 12 * n
+    src = rng.integers(0, n, m)
+    dst = np.minimum(rng.geometric(8.0 / n, m) - 1, n - 1)  # skewed rows: about 40 % without entries
+    g = nwgr_amd.build_adjacency(src, dst, n=n, dedup=True, drop_self_loops=True)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    pg.tuning = dict(engine=1, tile_bits=bits, tile_fill_cost=0)
+    for mode in ("uniform", "leak"):
+        parity(g, deg, mode, 1e-6, 100, device_graph=pg)
+    assert pg.last_engine().startswith("tiled")
+
+
 def test_engines_agree_and_switch_back_and_forth():
     """Switching engines on one handle re-plans without stale state."""
     g = nwgr_amd.rmat(15, 16)
