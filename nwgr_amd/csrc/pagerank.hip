@@ -821,21 +821,24 @@ __device__ __forceinline__ void tile_lane(const TileArgs& a, uint32_t lds0, uint
     // more fragments than the window holds: one pass per window, each
     // keeping the sums whose slots fall inside it; all windows but the last
     // are stored at once
-    float tail = 0.f;
+    // (each entry's running sum and slot are formed once, in registers)
+    float acc[16];
+    uint32_t qs[16];
+    acc[0] = v[0];
+    qs[0] = q0;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+      qs[i] = qs[i - 1] + (lane_bit(sm[i]) ? 4u : 0u);
+      acc[i] = lane_bit(sm[i]) ? v[i] : acc[i - 1] + v[i];
+    }
+    const float tail = row_carry(acc[15], seen);
     for (uint32_t w0 = 0;; w0 += kStageW) {
       const uint32_t lo = stage_b + w0 * 4u, hi = lo + kStageW * 4u;  // slot addresses of this window
-      uint32_t qa = q0;
-      float acc = v[0];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        if (i > 0) {
-          qa += lane_bit(sm[i]) ? 4u : 0u;
-          acc = lane_bit(sm[i]) ? v[i] : acc + v[i];
-        }
         const bool end = lane_bit(i < 15 ? sm[i + 1] : end15_m);
-        lds_st(end && qa >= lo && qa < hi ? qa - w0 * 4u : junk_b, acc);
+        lds_st(end && qs[i] >= lo && qs[i] < hi ? qs[i] - w0 * 4u : junk_b, acc[i]);
       }
-      if (w0 == 0) tail = row_carry(acc, seen);
       if (fix && q0 >= lo && q0 < hi) lds_st(q0 - w0 * 4u, lds_ld(q0 - w0 * 4u) + tail);
       wave_sync();
       read_window(out, stage_b, wl);
