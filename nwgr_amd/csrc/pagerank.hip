@@ -117,7 +117,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 // Sum of (a, b) over the workgroup; every thread receives the totals.
 // Deterministic: fixed butterfly inside each wave, waves added in order.
-__device__ __forceinline__ void block_sum2(double& a, double& b, double (*s)[kBlock / 64]) {
+template <int NT = kBlock>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double (*s)[NT / 64]) {
   a = wave_sum(a);
   b = wave_sum(b);
   const int w = threadIdx.x >> 6;
@@ -129,7 +130,7 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double (*s)[kBl
   __syncthreads();
   double ta = 0, tb = 0;
 #pragma unroll
-  for (int i = 0; i < kBlock / 64; ++i) {
+  for (int i = 0; i < NT / 64; ++i) {
     ta += s[0][i];
     tb += s[1][i];
   }
@@ -1047,14 +1048,18 @@ __device__ __forceinline__ void quiet_rows(const SweepArgs& a, float Df, double&
   }
 }
 
-// Partials staged in LDS per workgroup step of kBlock rows.
-constexpr uint32_t kCombineStage = 4096;
+// Threads (and rows per window) of a k_combine workgroup, and the partials
+// staged in LDS per step: 512-row windows with an 8192-partial stage measured
+// fastest at R-MAT s22 (39.9 us against 41.6 for 256 / 4096, 41.0 for
+// 512 / 4096, 48.8 for 1024 / 16384).
+constexpr int kCombineThreads = 512;
+constexpr uint32_t kCombineStage = 8192;
 // Consecutive combine steps kept on one XCD.
 constexpr uint32_t kCombineRun = 8;
 
 template <uint32_t KIND>
-__global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
-  __shared__ double s_red[2][kBlock / 64];
+__global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
+  __shared__ double s_red[2][kCombineThreads / 64];
   __shared__ float s_buf[kCombineStage];
   __shared__ uint32_t s_flag;
   const uint32_t tid = threadIdx.x;
@@ -1080,8 +1085,8 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
   }
   // rows with many partials: one wavefront each (lane-strided, then a
   // butterfly)
-  const uint32_t nwaves = G * (kBlock / kWave);
-  for (uint32_t i = vb * (kBlock / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
+  const uint32_t nwaves = G * (kCombineThreads / kWave);
+  for (uint32_t i = vb * (kCombineThreads / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
     const uint32_t row = a.wave_rows[i];
     const uint32_t b = pp[row], e = pp[row + 1];
     const RowIn in = load_row<KIND>(a.s, row);
@@ -1101,14 +1106,14 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
     if (wl == 0) finish_row<KIND>(a.s, row, z, Df, in, err, dang);
   }
   // The other rows, from the first row after the leading run of wave rows: a
-  // thread each, in windows of kBlock consecutive rows.  A window is cut into
+  // thread each, in windows of kCombineThreads consecutive rows.  A window is cut into
   // steps whose partials fit the LDS stage (the longest prefix of its rows
   // that fits, found with one counting barrier); each step's partials are
   // gathered into LDS by the whole workgroup, with four loads in flight per
   // thread, then every thread adds its row's partials in tile order.
   const uint32_t n = a.s.nrows;  // quiet rows [nrows, n) are not stored
-  for (uint32_t w0 = a.row0 + vb * kBlock; w0 < n; w0 += G * kBlock) {
-    const uint32_t w1 = min(w0 + kBlock, n);
+  for (uint32_t w0 = a.row0 + vb * kCombineThreads; w0 < n; w0 += G * kCombineThreads) {
+    const uint32_t w1 = min(w0 + kCombineThreads, n);
     for (uint32_t r0 = w0; r0 < w1;) {
       const uint32_t row = r0 + tid;
       const bool valid = row < w1;
@@ -1125,19 +1130,19 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
       const bool staged = nfit > 0;  // else row r0 alone overflows: a wave row, skipped here
       nfit = max(nfit, 1u);
       const uint32_t end = staged ? pp[r0 + nfit] : base;
-      for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kBlock) {
+      for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kCombineThreads) {
         uint32_t sl[4];
         float v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const uint32_t k = k0 + j * kBlock;
+          const uint32_t k = k0 + j * kCombineThreads;
           sl[j] = k < end ? a.rslot[k] : 0u;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = k0 + j * kBlock < end ? a.partial[sl[j]] : 0.f;
+        for (int j = 0; j < 4; ++j) v[j] = k0 + j * kCombineThreads < end ? a.partial[sl[j]] : 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (k0 + j * kBlock < end) s_buf[k0 + j * kBlock - base] = v[j];
+          if (k0 + j * kCombineThreads < end) s_buf[k0 + j * kCombineThreads - base] = v[j];
       }
       __syncthreads();
       if (staged && tid < nfit && e - b <= kWaveRowMin) {
@@ -1149,9 +1154,9 @@ __global__ __launch_bounds__(kBlock) void k_combine(CombineArgs a) {
       r0 += nfit;
     }
   }
-  quiet_rows<KIND, kBlock>(a.s, Df, err, dang);
+  quiet_rows<KIND, kCombineThreads>(a.s, Df, err, dang);
   if constexpr (KIND != kCoef) {
-    block_sum2(err, dang, s_red);
+    block_sum2<kCombineThreads>(err, dang, s_red);
     if (tid == 0) {
       a.s.part_out[2 * blockIdx.x] = err;
       a.s.part_out[2 * blockIdx.x + 1] = dang;
@@ -1472,23 +1477,23 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   switch (sp.kind) {
     case kPlain:
       k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
-      k_combine<kPlain><<<g.grid_c, kBlock, 0, s>>>(ca);
+      k_combine<kPlain><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
       break;
     case kWeighted:
       k_tile<kWeighted><<<nb, kSweepThreads, shm, s>>>(ta);
-      k_combine<kWeighted><<<g.grid_c, kBlock, 0, s>>>(ca);
+      k_combine<kWeighted><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
       break;
     case kAdjoint:
       k_tile<kAdjoint><<<nb, kSweepThreads, shm, s>>>(ta);
-      k_combine<kAdjoint><<<g.grid_c, kBlock, 0, s>>>(ca);
+      k_combine<kAdjoint><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
       break;
     case kDelta:
       k_tile<kDelta><<<nb, kSweepThreads, shm, s>>>(ta);
-      k_combine<kDelta><<<g.grid_c, kBlock, 0, s>>>(ca);
+      k_combine<kDelta><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
       break;
     default:
       k_tile<kCoef><<<nb, kSweepThreads, shm, s>>>(ta);
-      k_combine<kCoef><<<g.grid_c, kBlock, 0, s>>>(ca);
+      k_combine<kCoef><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
       break;
   }
   NWGR_HIP(hipGetLastError());
