@@ -18,13 +18,22 @@
  * (experimental/algorithms/page_rank.hpp:118-120, apb/spmv.cpp:48-50).  A
  * plain `nw::graph::csr_view` overload serves callers without NWGraph.
  *
- * Semantics: Jacobi sweeps in FP32 on the GPU (the reference's v11 sweeps in
- * place, Gauss-Seidel; both reach the same fixed point, the sweep count
- * differs); dangling mass leaks as in the reference; the result is written
- * into the caller's vector, which must already hold graph.size() entries
- * (page_rank.hpp:102).  num_threads is ignored (unused in the reference too,
- * page_rank.hpp:91).  Failures (no GPU, malformed CSR) throw
- * std::runtime_error; not converging within max_iters returns normally.
+ * Semantics: Jacobi sweeps on the GPU (the reference's v11 sweeps in place,
+ * Gauss-Seidel; both reach the same fixed point, the sweep count differs),
+ * in the caller's Real: FP32 for float, FP64 for double (long double and
+ * other wider types are computed in FP64); dangling mass leaks as in the
+ * reference; the result is written into the caller's vector, which must
+ * already hold graph.size() entries (page_rank.hpp:102).  An empty graph
+ * returns at once, as the reference does for N = 0.  num_threads is ignored
+ * (unused in the reference too, page_rank.hpp:91).  Failures (no GPU,
+ * malformed CSR) throw std::runtime_error; not converging within max_iters
+ * returns normally.
+ *
+ * The device graph of a call is kept for the next call on the same arrays
+ * (nwgr_pr_graph_cached: same addresses, sizes and contents), as the
+ * reference's graph outlives its page_rank calls; pr.exe's trials
+ * (bench/pr.cpp:94-95) then pay for the solve and the copy of the ranks only.
+ * After editing a graph's arrays in place, call page_rank_cache_clear().
  */
 #ifndef NWGR_PAGE_RANK_HPP
 #define NWGR_PAGE_RANK_HPP
@@ -77,18 +86,15 @@ inline int& device() {
 enum class solver { sweeps, residual };
 
 template <typename Real>
-void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
-         const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank, Real damping_factor,
-         Real threshold, std::size_t max_iters, solver kind = solver::sweeps) {
+void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m, const uint32_t* degrees,
+         std::size_t degrees_size, std::vector<Real>& page_rank, Real damping_factor, Real threshold,
+         std::size_t max_iters, solver kind = solver::sweeps) {
   static_assert(std::is_floating_point_v<Real>, "Real must be a floating-point type");
   if (page_rank.size() < n) throw std::runtime_error("nwgr: page_rank must hold graph.size() entries");
-  if (degrees.size() < n) throw std::runtime_error("nwgr: degrees must hold graph.size() entries");
-  nwgr_pr_graph* g = nullptr;
-  check(nwgr_pr_graph_create(offsets, indices, n, m, degrees.data(), device(), &g));
-  struct Guard {
-    nwgr_pr_graph* g;
-    ~Guard() { nwgr_pr_graph_destroy(g); }
-  } guard{g};
+  if (degrees_size < n) throw std::runtime_error("nwgr: degrees must hold graph.size() entries");
+  if (n == 0) return;  // page_rank.hpp:92-147 over an empty graph writes nothing
+  nwgr_pr_graph* g = nullptr;  // owned by the library's resident-graph cache
+  check(nwgr_pr_graph_cached(offsets, indices, n, m, degrees, device(), &g, nullptr));
   nwgr_pr_params p;
   nwgr_pr_params_default(&p);
   p.damping = static_cast<double>(damping_factor);
@@ -96,16 +102,32 @@ void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t 
   p.max_iters = max_iters > 0xffffffffu ? 0xffffffffu : static_cast<uint32_t>(max_iters);
   p.dangling = NWGR_DANGLING_LEAK;
   nwgr_pr_result res;
-  auto solve = [&](float* out) {
-    check(kind == solver::residual ? nwgr_pr_delta_run_f32(g, &p, out, &res) : nwgr_pr_run_f32(g, &p, out, &res));
-  };
-  if constexpr (std::is_same_v<Real, float>) {
-    solve(page_rank.data());
+  if (kind == solver::residual) {
+    // the residual push is FP32 on the device
+    if constexpr (std::is_same_v<Real, float>) {
+      check(nwgr_pr_delta_run_f32(g, &p, page_rank.data(), &res));
+    } else {
+      std::vector<float> tmp(n);
+      check(nwgr_pr_delta_run_f32(g, &p, tmp.data(), &res));
+      std::copy(tmp.begin(), tmp.end(), page_rank.begin());
+    }
+  } else if constexpr (std::is_same_v<Real, float>) {
+    check(nwgr_pr_run_f32(g, &p, page_rank.data(), &res));
+  } else if constexpr (std::is_same_v<Real, double>) {
+    check(nwgr_pr_run_f64(g, &p, page_rank.data(), &res));
   } else {
-    std::vector<float> tmp(n);
-    solve(tmp.data());
-    for (uint32_t i = 0; i < n; ++i) page_rank[i] = static_cast<Real>(tmp[i]);
+    std::vector<double> tmp(n);
+    check(nwgr_pr_run_f64(g, &p, tmp.data(), &res));
+    std::copy(tmp.begin(), tmp.end(), page_rank.begin());
   }
+}
+
+template <class Graph>
+const uint32_t* as_u32(const std::vector<typename Graph::vertex_id_type>& v) {
+  static_assert(sizeof(typename Graph::vertex_id_type) == sizeof(uint32_t) &&
+                    std::is_integral_v<typename Graph::vertex_id_type>,
+                "the device path takes uint32 vertex ids (util/defaults.hpp:23-24)");
+  return reinterpret_cast<const uint32_t*>(v.data());
 }
 
 }  // namespace nwgr_detail
@@ -113,55 +135,54 @@ void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t 
 /* Select the GPU used by page_rank (default 0). */
 inline void page_rank_set_device(int device) { nwgr_detail::device() = device; }
 
+/* Release the device graphs kept between calls (needed after a graph's arrays
+ * are edited in place). */
+inline void page_rank_cache_clear() { nwgr_pr_cache_clear(); }
+
 template <nwgr_csr_graph Graph, typename Real>
 void page_rank(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
                std::vector<Real>& page_rank, Real damping_factor, Real threshold, std::size_t max_iters,
                std::size_t /*num_threads*/) {
-  static_assert(sizeof(typename Graph::vertex_id_type) == sizeof(uint32_t),
-                "the device path takes uint32 vertex ids (util/defaults.hpp:23-24)");
   const auto& offs = graph.indices_;
   const auto& nbrs = std::get<0>(graph.to_be_indexed_);
   const uint32_t n = static_cast<uint32_t>(graph.size());
-  std::vector<uint32_t> deg(degrees.begin(), degrees.end());
   nwgr_detail::run<Real>(reinterpret_cast<const uint32_t*>(offs.data()),
                          reinterpret_cast<const uint32_t*>(nbrs.data()), n,
-                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), deg, page_rank, damping_factor,
-                         threshold, max_iters);
+                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), nwgr_detail::as_u32<Graph>(degrees),
+                         degrees.size(), page_rank, damping_factor, threshold, max_iters);
 }
 
 template <typename Real>
 void page_rank(const csr_view& graph, const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank,
                Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
-  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
-                         threshold, max_iters);
+  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees.data(), degrees.size(), page_rank,
+                         damping_factor, threshold, max_iters);
 }
 
 /* Residual ("delta") PageRank, the signature of page_rank_v13
  * (experimental/algorithms/page_rank.hpp:632-634, pr.exe version 13):
  * ranks start at 0, residuals at 1 - d; residuals above `threshold` are
  * pushed into the rank and sent on as residual * d / degree; unnormalised
- * result (nwgr_pr_delta_run_f32). */
+ * result (nwgr_pr_delta_run_f32, FP32 for every Real). */
 template <nwgr_csr_graph Graph, typename Real>
 void page_rank_v13(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
                    std::vector<Real>& page_rank, Real damping_factor, Real threshold, std::size_t max_iters,
                    std::size_t /*num_threads*/) {
-  static_assert(sizeof(typename Graph::vertex_id_type) == sizeof(uint32_t),
-                "the device path takes uint32 vertex ids (util/defaults.hpp:23-24)");
   const auto& offs = graph.indices_;
   const auto& nbrs = std::get<0>(graph.to_be_indexed_);
   const uint32_t n = static_cast<uint32_t>(graph.size());
-  std::vector<uint32_t> deg(degrees.begin(), degrees.end());
   nwgr_detail::run<Real>(reinterpret_cast<const uint32_t*>(offs.data()),
                          reinterpret_cast<const uint32_t*>(nbrs.data()), n,
-                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), deg, page_rank, damping_factor,
-                         threshold, max_iters, nwgr_detail::solver::residual);
+                         static_cast<uint64_t>(offs.empty() ? 0 : offs[n]), nwgr_detail::as_u32<Graph>(degrees),
+                         degrees.size(), page_rank, damping_factor, threshold, max_iters,
+                         nwgr_detail::solver::residual);
 }
 
 template <typename Real>
 void page_rank_v13(const csr_view& graph, const std::vector<uint32_t>& degrees, std::vector<Real>& page_rank,
                    Real damping_factor, Real threshold, std::size_t max_iters, std::size_t /*num_threads*/) {
-  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees, page_rank, damping_factor,
-                         threshold, max_iters, nwgr_detail::solver::residual);
+  nwgr_detail::run<Real>(graph.offsets, graph.indices, graph.n, graph.m, degrees.data(), degrees.size(), page_rank,
+                         damping_factor, threshold, max_iters, nwgr_detail::solver::residual);
 }
 
 /* page_rank_v3 (experimental/algorithms/page_rank.hpp:566-568), the serial

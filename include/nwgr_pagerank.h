@@ -90,10 +90,25 @@ typedef struct nwgr_pr_result {
  * (pr.cpp:27), max_iters 20 (pr.cpp:26), LEAK, no teleport/init. */
 void nwgr_pr_params_default(nwgr_pr_params* p);
 
-/* Upload an in-CSR (host arrays) to `device` and prepare it for sweeps.
+/* Checks p as every solve does before it starts -- damping in [0, 1], tol not
+ * NaN, a known dangling mode (TELEPORT with a teleport vector) -- and, taking
+ * p->teleport and p->init as HOST arrays of n entries, that every entry is
+ * finite and non-negative.  NWGR_EINVAL otherwise.  The solves check device
+ * vectors the same way on the device; the reference checks nothing
+ * (page_rank.hpp:89-91 takes only damping and threshold). */
+int nwgr_pr_params_check(const nwgr_pr_params* p, uint32_t n);
+
+/* Upload an in-CSR (host arrays) to `device` and prepare it for sweeps:
+ * relabelling, the tiled layout the default tuning's sweeps use, the work
+ * plan, all built here so that the first solve costs what later ones do.
  * Replaces the graph/degrees arguments of page_rank (page_rank.hpp:89-91):
  * the caller keeps ownership of its arrays; the handle owns device memory.
- * Validates offsets (monotone, offsets[0]=0, offsets[n]=m) and indices < n. */
+ * Validates offsets (monotone, offsets[0]=0, offsets[n]=m) and indices < n.
+ * n = 0 (with m = 0) gives an empty handle that holds no device memory and
+ * needs no device; every solve on it returns NWGR_OK at once, as the
+ * reference returns for N = 0 (page_rank.hpp:92-147): result.iters = 1
+ * (max_iters when tol <= 0, 0 when max_iters = 0), error 0.
+ * n and m must be < 2^31 (the device sorts count in int). */
 int nwgr_pr_graph_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
                          const uint32_t* degrees, int device, nwgr_pr_graph** out);
 
@@ -103,7 +118,35 @@ int nwgr_pr_graph_create_device(const uint32_t* d_offsets, const uint32_t* d_ind
                                 uint64_t m, const uint32_t* d_degrees, int device, void* stream,
                                 nwgr_pr_graph** out);
 
+/* Destroys a handle; a handle of the resident-graph cache (below) is also
+ * dropped from the cache. */
 void nwgr_pr_graph_destroy(nwgr_pr_graph* g);
+
+/* Resident-graph cache for repeated drop-in calls.  The reference's
+ * page_rank runs on a graph the caller built once (page_rank.hpp:92-119 sets
+ * up only the rank vectors) and pr.exe times that call over trials on one
+ * graph (bench/pr.cpp:94-95); nw::graph::page_rank (include/nwgr/page_rank.hpp)
+ * and nwgr_amd.page_rank get their device graph from here instead of
+ * creating and destroying one per call.
+ * nwgr_pr_graph_cached returns the handle made for an earlier call with the
+ * same offsets, indices and degrees addresses, n, m and device whose
+ * contents still match -- the degrees word for word, offsets and indices at
+ * 4096 fixed positions and their last word -- (*hit = 1), else creates one
+ * as nwgr_pr_graph_create does (*hit = 0; `hit` may be NULL).  The cache owns
+ * its handles: keep one only until the next cache call (the least recently
+ * used is destroyed when a new graph would pass the capacity, default 2), or
+ * destroy it with nwgr_pr_graph_destroy.  A graph edited in place so that
+ * the compared words stay equal is not noticed: clear the cache after such
+ * an edit.  The cache functions are serialised by a lock; a handle itself is
+ * used by one thread at a time. */
+int nwgr_pr_graph_cached(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                         const uint32_t* degrees, int device, nwgr_pr_graph** out, uint32_t* hit);
+/* Destroys every cached handle. */
+void nwgr_pr_cache_clear(void);
+/* Most handles kept (1 .. 64); the least recently used beyond it are destroyed. */
+int nwgr_pr_cache_set_capacity(uint32_t entries);
+/* Lookups that found / did not find a resident graph, and handles held. */
+void nwgr_pr_cache_stats(uint64_t* hits, uint64_t* misses, uint32_t* entries);
 
 uint32_t nwgr_pr_graph_n(const nwgr_pr_graph* g);
 uint64_t nwgr_pr_graph_m(const nwgr_pr_graph* g);
@@ -113,7 +156,10 @@ double nwgr_pr_graph_setup_ms(const nwgr_pr_graph* g);
 
 /* Plain PageRank; the ranks are written to the host array r_out[n] in the
  * caller's vertex order.  Replaces the body of page_rank
- * (page_rank.hpp:92-147); r_out plays the role of the page_rank vector. */
+ * (page_rank.hpp:92-147); r_out plays the role of the page_rank vector.
+ * A teleport or initial vector with a NaN, infinite or negative entry gives
+ * NWGR_EINVAL.  The handle keeps the device buffers of this form, so a
+ * repeated call allocates no device memory. */
 int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res);
 
 /* Same with the output in device memory (d_r_out[n]) and all work enqueued
@@ -121,6 +167,18 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
  * returning, since the sweep count is decided on the device. */
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
                            nwgr_pr_result* res, void* stream);
+
+/* FP64 PageRank: the iteration and stop rule of nwgr_pr_run_f32 with double
+ * ranks, contributions and sums and the integer degrees converted exactly --
+ * the reference's Real = double (page_rank.hpp:89-91; its PageRank test runs
+ * in double, test/page_rank_test.cpp:66,93).  r_out: n doubles in caller
+ * order (host array; device array for the _device form, on `stream`).
+ * Teleport and initial vectors are float arrays as for the FP32 solve (host
+ * arrays here, device arrays for the _device form), widened to double.  The
+ * sweeps run on the degree-binned work plan of the handle. */
+int nwgr_pr_run_f64(nwgr_pr_graph* g, const nwgr_pr_params* p, double* r_out, nwgr_pr_result* res);
+int nwgr_pr_run_device_f64(nwgr_pr_graph* g, const nwgr_pr_params* p, double* d_r_out, nwgr_pr_result* res,
+                           void* stream);
 
 /* Residual ("delta") PageRank.  Replaces page_rank_v13
  * (experimental/algorithms/page_rank.hpp:632-698, pr.exe version 13,
@@ -216,8 +274,10 @@ int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t);
  * host side; the relabelled CSR is kept). */
 int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t);
 
-/* Per-sweep L1 errors of the last run (up to `cap` entries); returns the
- * number written.  Backs the reference's trace table (page_rank.hpp:42-52). */
+/* Per-sweep L1 errors of the last run (up to `cap` entries, and of at most
+ * its first 4096 sweeps: the device history is that long, whatever
+ * max_iters is); returns the number written.  Backs the reference's trace
+ * table (page_rank.hpp:42-52). */
 uint32_t nwgr_pr_last_errors(const nwgr_pr_graph* g, double* errs, uint32_t cap);
 
 /* Device timing of the last run's sweep kernels: summed event time of the

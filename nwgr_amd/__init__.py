@@ -9,8 +9,8 @@ the torch autograd bridge.
 from .graph import (CSRGraph, EdgeList, adjacency_from_edge_list, build_adjacency, build_degrees,  # noqa: F401
                     erdos_renyi, load_graph, read_mm, read_nw_adjacency, read_nw_edge_list, rmat, rmat_many,
                     row_cuts, row_shard, transpose, write_nw_adjacency, write_nw_edge_list)
-from .page_rank import (PageRankGraph, RunResult, device_count, page_rank, page_rank_delta, top_k,  # noqa: F401
-                        top_k_device)
+from .page_rank import (PageRankGraph, RunResult, cache_clear, cache_set_capacity, cache_stats,  # noqa: F401
+                        device_count, page_rank, page_rank_delta, top_k, top_k_device)
 from .multi import MultiPageRank  # noqa: F401
 from ._lib import NwgrError  # noqa: F401
 
@@ -18,4 +18,4 @@ __all__ = ["CSRGraph", "EdgeList", "build_adjacency", "build_degrees", "rmat", "
            "read_mm", "read_nw_edge_list", "write_nw_edge_list", "read_nw_adjacency", "write_nw_adjacency",
            "load_graph", "adjacency_from_edge_list", "row_cuts", "row_shard",
            "PageRankGraph", "MultiPageRank", "RunResult", "page_rank", "page_rank_delta", "device_count", "NwgrError", "top_k",
-           "top_k_device"]
+           "top_k_device", "cache_clear", "cache_stats", "cache_set_capacity"]

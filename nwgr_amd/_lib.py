@@ -88,6 +88,16 @@ def _declare_pagerank(L):
     L.nwgr_pr_graph_create_device.argtypes = [vp, vp, u32, u64, vp, i32, vp, P(vp)]
     L.nwgr_pr_graph_destroy.restype = None
     L.nwgr_pr_graph_destroy.argtypes = [vp]
+    L.nwgr_pr_graph_cached.restype = i32
+    L.nwgr_pr_graph_cached.argtypes = [vp, vp, u32, u64, vp, i32, P(vp), P(u32)]
+    L.nwgr_pr_cache_clear.restype = None
+    L.nwgr_pr_cache_clear.argtypes = []
+    L.nwgr_pr_cache_set_capacity.restype = i32
+    L.nwgr_pr_cache_set_capacity.argtypes = [u32]
+    L.nwgr_pr_cache_stats.restype = None
+    L.nwgr_pr_cache_stats.argtypes = [P(u64), P(u64), P(u32)]
+    L.nwgr_pr_params_check.restype = i32
+    L.nwgr_pr_params_check.argtypes = [P(PrParams), u32]
     L.nwgr_pr_graph_n.restype = u32
     L.nwgr_pr_graph_n.argtypes = [vp]
     L.nwgr_pr_graph_m.restype = u64
@@ -104,6 +114,10 @@ def _declare_pagerank(L):
     L.nwgr_pr_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
     L.nwgr_pr_run_device_f32.restype = i32
     L.nwgr_pr_run_device_f32.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
+    L.nwgr_pr_run_f64.restype = i32
+    L.nwgr_pr_run_f64.argtypes = [vp, P(PrParams), vp, P(PrResult)]
+    L.nwgr_pr_run_device_f64.restype = i32
+    L.nwgr_pr_run_device_f64.argtypes = [vp, P(PrParams), vp, P(PrResult), vp]
     L.nwgr_pr_delta_run_f32.restype = i32
     L.nwgr_pr_delta_run_f32.argtypes = [vp, P(PrParams), vp, P(PrResult)]
     L.nwgr_pr_delta_run_device_f32.restype = i32

@@ -520,6 +520,11 @@ MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint3
 
 void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
   validate_params(p);
+  {
+    DevBuf flag;
+    flag.alloc(sizeof(uint32_t));
+    check_vectors(p, mg.n, flag.as<uint32_t>(), s);
+  }
   static const bool attrs = [] {
     NWGR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_multi), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(kMultiLdsMax * sizeof(float))));

@@ -144,8 +144,13 @@ struct State {
   double err;         // error of the last sweep (valid once done)
   double dang;        // dangling mass (or adjoint Q) of the final iterate (valid once done)
   double dsum;        // tiled engine: dangling mass (or Q) of the sweep's input iterate
+  double last_e;      // error of the latest completed sweep (the decay estimate past the history)
   float rq[2];        // quiet rows (no entries): their common rank in the iterate of parity p
 };
+
+// Per-sweep errors kept on the device for the trace (the first kHistCap
+// sweeps of a solve); the solve itself needs O(1) memory for any max_iters.
+constexpr uint32_t kHistCap = 4096;
 
 // Host-mapped mirror the solver publishes into: progress every sweep, the
 // result when it stops.
@@ -329,6 +334,7 @@ struct PprDeleter {
 
 struct Graph {
   int device = 0;
+  bool cached = false;            // owned by the resident-graph cache (cache.hip)
   uint32_t n = 0;
   uint64_t m = 0;
   uint32_t num_cu = 0;
@@ -343,6 +349,7 @@ struct Graph {
   DevBuf idx;       // u32[m]  new source ids
   DevBuf epos;      // u32[m]  new edge position -> original edge position
   DevBuf degf;      // f32[n]  divisor of the contribution (caller degrees), new order
+  DevBuf degu;      // u32[n]  the same divisors as given (the FP64 solve)
   Plan fwd;         // plan of the in-CSR (rows already in degree order)
   std::vector<uint32_t> in_deg_h;  // host copy of in-degrees (new order)
   std::vector<uint32_t> offs_h;    // host copy of offs
@@ -382,6 +389,9 @@ struct Graph {
   DevBuf state;     // State
   DevBuf hist;      // double[hist_cap]
   uint32_t hist_cap = 0;
+  DevBuf out_buf;   // f32[n] ranks of a host-output solve before the copy back
+  DevBuf tel_h, init_h;  // f32[n] device copies of a host-output solve's teleport / initial vector
+  DevBuf vflag;     // u32 result of the device check of the caller's vectors
   HostFlag* hflag = nullptr;   // pinned, mapped
   HostFlag* hflag_dev = nullptr;
 
@@ -394,6 +404,10 @@ struct Graph {
   std::vector<double> last_errs;
   SolveProfile last_profile;
   int last_engine = -1;
+
+  // FP64 solve (fp64.hip; built on first use): ranks and contributions with
+  // the zero sentinel at index n, the permuted teleport, hub chunk partials
+  DevBuf r64[2], c64[2], tel64, hub64;
 
   // batched personalized solve (built on first use)
   std::unique_ptr<PprState, PprDeleter> ppr;
@@ -421,10 +435,18 @@ inline uint64_t bytes_of(const Graph& g) {
                           &g.fwd.hub_part, &g.fwd.hub_ticket, &g.fwd.rows, &g.t_offs, &g.t_idx, &g.t_pos,
                           &g.adj.hub_units, &g.adj.hub_part, &g.adj.hub_ticket, &g.adj.rows, &g.w_in,
                           &g.w_out, &g.s, &g.gsrc, &g.rsave, &g.coef_a, &g.coef_b, &g.r[0], &g.r[1], &g.c[0],
-                          &g.c[1], &g.tel, &g.partials, &g.state, &g.hist})
+                          &g.c[1], &g.tel, &g.partials, &g.state, &g.hist, &g.degu, &g.out_buf, &g.tel_h,
+                          &g.init_h, &g.r64[0], &g.r64[1], &g.c64[0], &g.c64[1], &g.tel64, &g.hub64})
     b += d->bytes;
   return b;
 }
+
+// resident-graph cache (cache.hip): the handle of a graph the caller passed
+// before (same arrays and contents), or a new one the cache then owns
+Graph* cache_acquire(const uint32_t* offs, const uint32_t* idx, uint32_t n, uint64_t m, const uint32_t* deg,
+                     int device, bool& hit);
+// drop `g` from the cache without destroying it; false if it is not cached
+bool cache_forget(Graph* g);
 
 // graph construction (graph.hip)
 Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
@@ -435,6 +457,8 @@ void build_plan(Graph& g, Plan& plan, const std::vector<uint32_t>& deg_by_pos,
                 const std::vector<uint32_t>& offs_h, const std::vector<uint32_t>& row_of, hipStream_t s);
 // transposed CSR + its plan (graph.hip); no-op once built
 void ensure_transpose(Graph& g, hipStream_t s);
+// throws NWGR_EINVAL for a tuning out of range (graph.hip)
+void validate_tuning(const Tuning& t);
 // validates g.tune, rebuilds the plans and the sweep grid (graph.hip)
 void replan(Graph& g, hipStream_t s);
 // tiled layout of a CSR over rows [0, n) (tiled.hip); no-op once built
@@ -463,6 +487,8 @@ inline size_t iter_len(uint32_t n) {
 // g.r[res.iters & 1] (new labels).
 void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
+// FP64 Jacobi solve (fp64.hip); d_out: n doubles in caller order
+void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_result& res, hipStream_t s);
 // residual ("delta") solve, page_rank_v13; p.tol is the push threshold and
 // res.iters counts iterations (pushes), res.error the last one's push count
 void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
@@ -472,9 +498,20 @@ void run_coef_pass(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, con
 // d_out is n x k row-major in caller order (ppr.hip)
 void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_seeds, float* d_out,
              nwgr_pr_result& res, hipStream_t s);
+// count the quiet rows whose divisor is 0 (once per handle)
+void prepare_quiet(Graph& g, hipStream_t s);
 // out[v] = src[iperm[v]] * scale (new labels -> caller order)
 void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s);
 void validate_params(const nwgr_pr_params& p);
+// validate_params plus, when `host_vectors`, the teleport and initial vectors
+// (host arrays of n entries): every entry finite and non-negative
+void check_params(const nwgr_pr_params& p, uint32_t n, bool host_vectors);
+// the same check of device vectors, by one pass on the device (d_flag: one
+// u32 of scratch); synchronises `s` when there is a vector to check
+void check_vectors(const nwgr_pr_params& p, uint32_t n, uint32_t* d_flag, hipStream_t s);
+// result of a solve on a graph without vertices
+void empty_result(const nwgr_pr_params& p, nwgr_pr_result& res);
+void empty_delta_result(const nwgr_pr_params& p, nwgr_pr_result& res);
 
 struct DeviceGuard {
   int prev = 0;

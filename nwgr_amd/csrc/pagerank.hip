@@ -23,6 +23,7 @@
 // chunks; the host sizes the chunks from the observed error decay.
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -373,6 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       a.st->iters = 0;
       a.st->err = INFINITY;
       a.st->dang = 0;
+      a.st->last_e = 0;
       a.st->rq[0] = a.un;  // quiet rows of the initial iterate (used without an init vector only)
       a.st->rq[1] = a.un;
     }
@@ -445,7 +447,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
       if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
       // progress for the host's chunk planning
       a.hflag->last_err = E;
-      a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
+      a.hflag->prev_err = done_sweeps >= 2 ? a.st->last_e : 0.0;
+      a.st->last_e = E;
       a.hflag->progress = done_sweeps;
       if (done_sweeps == 0) a.hflag->first_err = E;
     }
@@ -991,7 +994,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
       a.st->dsum = D;  // read by this sweep's combine pass
       if (done_sweeps > 0 && done_sweeps - 1 < a.hist_cap) a.hist[done_sweeps - 1] = E;
       a.hflag->last_err = E;
-      a.hflag->prev_err = (done_sweeps >= 2 && done_sweeps - 2 < a.hist_cap) ? a.hist[done_sweeps - 2] : 0.0;
+      a.hflag->prev_err = done_sweeps >= 2 ? a.st->last_e : 0.0;
+      a.st->last_e = E;
       a.hflag->progress = done_sweeps;
       if (done_sweeps == 0) a.hflag->first_err = E;
     }
@@ -1303,6 +1307,15 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t* __restrict
   }
 }
 
+// Flags `bit` in *bad when an entry of v is NaN, infinite or negative.
+__global__ __launch_bounds__(kBlock) void k_check_vec(const float* __restrict__ v, uint32_t n, uint32_t bit,
+                                                      uint32_t* bad) {
+  bool ok = true;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) ok &= (v[i] >= 0.f) & (v[i] <= FLT_MAX);
+  if (!ok) atomicOr(bad, bit);
+}
+
 // Quiet rows: the count of zero divisors among them.
 __global__ __launch_bounds__(kBlock) void k_quiet_setup(const float* __restrict__ degf, uint32_t nz, uint32_t n,
                                                         uint32_t* zeros) {
@@ -1544,14 +1557,61 @@ void validate_params(const nwgr_pr_params& p) {
                "dangling = TELEPORT needs a teleport vector");
 }
 
+void check_params(const nwgr_pr_params& p, uint32_t n, bool host_vectors) {
+  validate_params(p);
+  if (!host_vectors) return;
+  for (const float* v : {p.teleport, p.init}) {
+    if (!v) continue;
+    // NaN fails both comparisons, +inf the second
+    bool ok = true;
+    for (uint32_t i = 0; i < n; ++i) ok &= (v[i] >= 0.f) & (v[i] <= FLT_MAX);
+    NWGR_REQUIRE(ok, v == p.teleport ? "teleport has a NaN, infinite or negative entry"
+                                     : "init has a NaN, infinite or negative entry");
+  }
+}
+
+void check_vectors(const nwgr_pr_params& p, uint32_t n, uint32_t* d_flag, hipStream_t s) {
+  if (n == 0 || (!p.teleport && !p.init)) return;
+  NWGR_HIP(hipMemsetAsync(d_flag, 0, sizeof(uint32_t), s));
+  const uint32_t blocks = std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096);
+  if (p.teleport) k_check_vec<<<blocks, kBlock, 0, s>>>(p.teleport, n, 1u, d_flag);
+  if (p.init) k_check_vec<<<blocks, kBlock, 0, s>>>(p.init, n, 2u, d_flag);
+  NWGR_HIP(hipGetLastError());
+  uint32_t bad = 0;
+  NWGR_HIP(hipMemcpyAsync(&bad, d_flag, sizeof(bad), hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  NWGR_REQUIRE(!(bad & 1u), "teleport has a NaN, infinite or negative entry");
+  NWGR_REQUIRE(!(bad & 2u), "init has a NaN, infinite or negative entry");
+}
+
+void empty_delta_result(const nwgr_pr_params& p, nwgr_pr_result& res) {
+  // an empty graph: iteration 1 pushes from no vertex, so the residual solve
+  // stops there (experimental/algorithms/page_rank.hpp:693-696)
+  validate_params(p);
+  NWGR_REQUIRE(!p.teleport && !p.init && p.dangling == kDanglingLeak,
+               "the residual solve takes no teleport, initial vector or dangling mode");
+  res = nwgr_pr_result{};
+  res.iters = std::min<uint32_t>(p.max_iters, 1);
+  res.converged = res.iters > 0 ? 1u : 0u;
+}
+
+void empty_result(const nwgr_pr_params& p, nwgr_pr_result& res) {
+  // an empty graph: every sweep changes nothing (error 0), so the first one
+  // converges when tol > 0 (page_rank.hpp:121-147 over an empty range)
+  res = nwgr_pr_result{};
+  res.iters = p.tol > 0 ? std::min<uint32_t>(p.max_iters, 1) : p.max_iters;
+  res.error = 0.0;
+  res.converged = (res.iters > 0 && p.tol > 0) ? 1u : 0u;
+}
+
 void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s) {
   // a traced solve captures one sweep per chunk, so the events around the
   // chunks time every sweep on its own (the reference's per-iteration time,
   // page_rank.hpp:123,142)
   const bool per_sweep = p.trace != 0;
   const uint32_t K = per_sweep ? 1u : std::clamp<uint32_t>(p.chunk ? p.chunk : g.tune.chunk, 1, Graph::kMaxChunk);
-  if (g.hist_cap < p.max_iters || !g.hist.p) {
-    g.hist_cap = std::max<uint32_t>(p.max_iters, 1);
+  if (!g.hist.p) {
+    g.hist_cap = kHistCap;
     g.hist.alloc(sizeof(double) * g.hist_cap);
     g.drop_graphs();
   }
@@ -1731,6 +1791,18 @@ void run_coef_pass(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, const
   NWGR_HIP(hipGetLastError());
 }
 
+void prepare_quiet(Graph& g, hipStream_t s) {
+  if (g.quiet_ready || g.nz >= g.n) return;
+  if (!g.vflag.p) g.vflag.alloc(sizeof(uint32_t));
+  NWGR_HIP(hipMemsetAsync(g.vflag.p, 0, sizeof(uint32_t), s));
+  k_quiet_setup<<<std::min<uint32_t>((g.n - g.nz + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+      g.degf.as<float>(), g.nz, g.n, g.vflag.as<uint32_t>());
+  NWGR_HIP(hipGetLastError());
+  NWGR_HIP(hipMemcpyAsync(&g.nq_dangling, g.vflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  g.quiet_ready = true;
+}
+
 void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s) {
   k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
       g.iperm.as<uint32_t>(), src, src, nullptr, g.n, g.n, scale, out);
@@ -1739,6 +1811,8 @@ void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t 
 
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
   validate_params(p);
+  if (!g.vflag.p) g.vflag.alloc(sizeof(uint32_t));
+  check_vectors(p, g.n, g.vflag.as<uint32_t>(), s);
   SolveSpec sp;
   sp.kind = kPlain;
   sp.plan = &g.fwd;
@@ -1751,17 +1825,7 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   // quiet rows share one rank when the teleport is uniform, the initial
   // iterate uniform and the dangling mass leaked or spread uniformly
   sp.quiet = sp.tiled && !p.teleport && !p.init && p.dangling != kDanglingTeleport && g.nz < g.n;
-  if (sp.quiet && !g.quiet_ready) {
-    DevBuf zeros;
-    zeros.alloc(sizeof(uint32_t));
-    NWGR_HIP(hipMemsetAsync(zeros.p, 0, sizeof(uint32_t), s));
-    k_quiet_setup<<<std::min<uint32_t>((g.n - g.nz + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
-        g.degf.as<float>(), g.nz, g.n, zeros.as<uint32_t>());
-    NWGR_HIP(hipGetLastError());
-    NWGR_HIP(hipMemcpyAsync(&g.nq_dangling, zeros.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    NWGR_HIP(hipStreamSynchronize(s));
-    g.quiet_ready = true;
-  }
+  if (sp.quiet) prepare_quiet(g, s);
   solve(g, sp, p, res, s);
   hipEvent_t e0, e1;
   NWGR_HIP(hipEventCreate(&e0));
@@ -1850,6 +1914,13 @@ void nwgr_pr_params_default(nwgr_pr_params* p) {
 
 const char* nwgr_last_error(void) { return nwgr::t_last_error.c_str(); }
 
+int nwgr_pr_params_check(const nwgr_pr_params* p, uint32_t n) {
+  return guarded([&] {
+    NWGR_REQUIRE(p != nullptr, "null argument");
+    nwgr::check_params(*p, n, true);
+  });
+}
+
 int nwgr_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -1879,6 +1950,7 @@ int nwgr_pr_graph_create_device(const uint32_t* d_offsets, const uint32_t* d_ind
 
 void nwgr_pr_graph_destroy(nwgr_pr_graph* g) {
   if (!g) return;
+  if (g->cached) (void)nwgr::cache_forget(g);
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(g->device);
@@ -1913,7 +1985,6 @@ int nwgr_pr_graph_get_tuning(const nwgr_pr_graph* g, nwgr_pr_tuning* t) {
 int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
   return guarded([&] {
     NWGR_REQUIRE(g && t, "null argument");
-    nwgr::DeviceGuard dg(g->device);
     const nwgr::Tuning old = g->tune;
     g->tune.lds_cache = t->lds_cache;
     g->tune.hub_threshold = t->hub_threshold;
@@ -1926,6 +1997,16 @@ int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
     g->tune.tile_bits = t->tile_bits;
     g->tune.tile_fill_cost = t->tile_fill_cost;
     g->tune.tile_combine = t->tile_combine;
+    if (g->n == 0) {  // no layout to re-plan
+      try {
+        nwgr::validate_tuning(g->tune);
+      } catch (...) {
+        g->tune = old;
+        throw;
+      }
+      return;
+    }
+    nwgr::DeviceGuard dg(g->device);
     try {
       nwgr::replan(*g, g->own_stream);
     } catch (...) {
@@ -1939,7 +2020,10 @@ int nwgr_pr_graph_set_tuning(nwgr_pr_graph* g, const nwgr_pr_tuning* t) {
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,
                            void* stream) {
   return guarded([&] {
-    NWGR_REQUIRE(g && p && d_r_out && res, "null argument");
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::check_params(*p, g->n, false);
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(d_r_out != nullptr, "null argument");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     nwgr::run_plain(*g, *p, d_r_out, *res, s);
@@ -1948,25 +2032,30 @@ int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r
 
 int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res) {
   return guarded([&] {
-    NWGR_REQUIRE(g && p && r_out && res, "null argument");
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(r_out != nullptr, "null argument");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = g->own_stream;
     const size_t nb = sizeof(float) * g->n;
-    nwgr::DevBuf out, tel, init;
-    out.alloc(nb);
+    // the handle keeps its staging buffers, so repeated calls allocate nothing
+    auto staged = [&](nwgr::DevBuf& b) -> float* {
+      if (b.bytes < nb) b.alloc(nb);
+      return b.as<float>();
+    };
+    float* out = staged(g->out_buf);
     nwgr_pr_params q = *p;
     if (p->teleport) {
-      tel.alloc(nb);
-      NWGR_HIP(hipMemcpyAsync(tel.p, p->teleport, nb, hipMemcpyHostToDevice, s));
-      q.teleport = tel.as<float>();
+      q.teleport = staged(g->tel_h);
+      NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.teleport), p->teleport, nb, hipMemcpyHostToDevice, s));
     }
     if (p->init) {
-      init.alloc(nb);
-      NWGR_HIP(hipMemcpyAsync(init.p, p->init, nb, hipMemcpyHostToDevice, s));
-      q.init = init.as<float>();
+      q.init = staged(g->init_h);
+      NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.init), p->init, nb, hipMemcpyHostToDevice, s));
     }
-    nwgr::run_plain(*g, q, out.as<float>(), *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, out.p, nb, hipMemcpyDeviceToHost, s));
+    nwgr::run_plain(*g, q, out, *res, s);
+    NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
   });
 }
@@ -1990,7 +2079,10 @@ uint32_t nwgr_pr_last_sweep_times(const nwgr_pr_graph* g, double* ms, uint32_t c
 int nwgr_pr_delta_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out, nwgr_pr_result* res,
                                  void* stream) {
   return guarded([&] {
-    NWGR_REQUIRE(g && p && d_r_out && res, "null argument");
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    if (g->n == 0) return nwgr::empty_delta_result(*p, *res);
+    NWGR_REQUIRE(d_r_out != nullptr, "null argument");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     nwgr::run_delta(*g, *p, d_r_out, *res, s);
@@ -1999,14 +2091,16 @@ int nwgr_pr_delta_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, floa
 
 int nwgr_pr_delta_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwgr_pr_result* res) {
   return guarded([&] {
-    NWGR_REQUIRE(g && p && r_out && res, "null argument");
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    if (g->n == 0) return nwgr::empty_delta_result(*p, *res);
+    NWGR_REQUIRE(r_out != nullptr, "null argument");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = g->own_stream;
     const size_t nb = sizeof(float) * g->n;
-    nwgr::DevBuf out;
-    out.alloc(nb);
-    nwgr::run_delta(*g, *p, out.as<float>(), *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, out.p, nb, hipMemcpyDeviceToHost, s));
+    if (g->out_buf.bytes < nb) g->out_buf.alloc(nb);
+    nwgr::run_delta(*g, *p, g->out_buf.as<float>(), *res, s);
+    NWGR_HIP(hipMemcpyAsync(r_out, g->out_buf.p, nb, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
   });
 }

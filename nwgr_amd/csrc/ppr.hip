@@ -446,8 +446,8 @@ void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_se
   const uint32_t K4 = (k + 3) & ~3u;
   ppr_alloc(g, ps, K4);
   const uint32_t n = g.n;
-  if (g.hist_cap < p.max_iters || !g.hist.p) {
-    g.hist_cap = std::max<uint32_t>(p.max_iters, 1);
+  if (!g.hist.p) {
+    g.hist_cap = kHistCap;
     g.hist.alloc(sizeof(double) * g.hist_cap);
     g.drop_graphs();
   }
@@ -580,6 +580,7 @@ int nwgr_ppr_batch_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_
                               float* d_R_out, nwgr_pr_result* res, void* stream) {
   return nwgr::guarded([&] {
     NWGR_REQUIRE(g && p && res, "null argument");
+    NWGR_REQUIRE(g->n > 0, "a graph without vertices has no seed vertices");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     nwgr::run_ppr(*g, *p, k, d_seeds, d_R_out, *res, s);
@@ -591,6 +592,7 @@ int nwgr_ppr_batch_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t k, co
   return nwgr::guarded([&] {
     NWGR_REQUIRE(g && p && res && seeds && R_out, "null argument");
     NWGR_REQUIRE(k >= 1 && k <= nwgr::kPprMaxCols, "k must be in [1, 256]");
+    NWGR_REQUIRE(g->n > 0, "a graph without vertices has no seed vertices");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = g->own_stream;
     nwgr::DevBuf ds, dr;

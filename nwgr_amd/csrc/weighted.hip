@@ -178,6 +178,8 @@ void run_weighted_forward(Graph& g, const nwgr_pr_params& p, const float* d_w, f
                           hipStream_t s) {
   validate_params(p);
   NWGR_REQUIRE(d_w != nullptr || g.m == 0, "weights are null");
+  if (!g.vflag.p) g.vflag.alloc(sizeof(uint32_t));
+  check_vectors(p, g.n, g.vflag.as<uint32_t>(), s);
   prepare_weights(g, d_w, s);
   SolveSpec sp = forward_spec(g, p);
   attach_tiled(g, sp, false, g.w_in.as<float>(), s);
@@ -191,6 +193,8 @@ void run_weighted_adjoint(Graph& g, const nwgr_pr_params& p, const float* d_w, c
                           const float* d_grad_r, float* d_dw, float* d_dt, nwgr_pr_result& res, hipStream_t s) {
   validate_params(p);
   NWGR_REQUIRE(d_r && d_grad_r && d_dt && (d_dw || g.m == 0) && (d_w || g.m == 0), "null argument");
+  if (!g.vflag.p) g.vflag.alloc(sizeof(uint32_t));
+  check_vectors(p, g.n, g.vflag.as<uint32_t>(), s);
   prepare_weights(g, d_w, s);
   const uint32_t n = g.n;
   if (!g.gsrc.p) {
@@ -252,7 +256,10 @@ extern "C" {
 int nwgr_wpr_forward_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, const float* d_w, float* d_r_out,
                                 nwgr_pr_result* res, void* stream) {
   return nwgr::guarded([&] {
-    NWGR_REQUIRE(g && p && d_r_out && res, "null argument");
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(d_r_out != nullptr, "null argument");
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     nwgr::run_weighted_forward(*g, *p, d_w, d_r_out, *res, s);
@@ -264,6 +271,8 @@ int nwgr_wpr_adjoint_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, const
                                 void* stream) {
   return nwgr::guarded([&] {
     NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
     nwgr::DeviceGuard dg(g->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     nwgr::run_weighted_adjoint(*g, *p, d_w, d_r, d_grad_r, d_grad_w, d_grad_t, *res, s);

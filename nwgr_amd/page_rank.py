@@ -54,9 +54,11 @@ def device_count() -> int:
 class PageRankGraph:
     """A graph uploaded to one GPU, ready for PageRank sweeps."""
 
-    def __init__(self, graph: CSRGraph | None = None, degrees=None, device: int = 0, *, _handle=None):
+    def __init__(self, graph: CSRGraph | None = None, degrees=None, device: int = 0, *, _handle=None,
+                 _owned: bool = True):
         L = pagerank_lib()
         self._h = ctypes.c_void_p()
+        self._owned = _owned
         if _handle is not None:
             self._h = _handle
         else:
@@ -81,9 +83,35 @@ class PageRankGraph:
         check(L.nwgr_pr_graph_create_device(offsets, indices, n, m, degrees, device, stream, ctypes.byref(h)))
         return cls(_handle=h, device=device)
 
+    @classmethod
+    def cached(cls, graph: CSRGraph, degrees=None, device: int = 0):
+        """The resident graph of the library's cache for (graph, degrees,
+        device) -- the handle an earlier call with the same arrays made, when
+        their contents still match, else a new one (nwgr_pr_graph_cached).
+        Returns (PageRankGraph, hit).  The cache owns the handle: closing the
+        returned object does not destroy it; keep it only until the next
+        cached() call."""
+        L = pagerank_lib()
+        off = np.ascontiguousarray(graph.offsets, dtype=np.uint32)
+        ind = np.ascontiguousarray(graph.indices, dtype=np.uint32)
+        if degrees is not None:
+            deg = np.ascontiguousarray(degrees, dtype=np.uint32)
+            if len(deg) != graph.n:
+                raise ValueError("degrees must have one entry per vertex")
+        if off is not graph.offsets or ind is not graph.indices or (degrees is not None and deg is not degrees):
+            # the cache keys on array addresses: a converted copy would miss every time
+            raise ValueError("cached() needs contiguous uint32 offsets, indices and degrees")
+        h = ctypes.c_void_p()
+        hit = ctypes.c_uint32()
+        check(L.nwgr_pr_graph_cached(off.ctypes.data if len(off) else None, ind.ctypes.data if len(ind) else None,
+                                     graph.n, graph.m, None if degrees is None else deg.ctypes.data, device,
+                                     ctypes.byref(h), ctypes.byref(hit)))
+        return cls(_handle=h, device=device, _owned=False), bool(hit.value)
+
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
-            pagerank_lib().nwgr_pr_graph_destroy(self._h)
+            if self._owned:
+                pagerank_lib().nwgr_pr_graph_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -168,6 +196,38 @@ class PageRankGraph:
         res = PrResult()
         check(pagerank_lib().nwgr_pr_run_f32(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
         return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run64(self, damping=0.85, tol=1e-4, max_iters=20, dangling="leak", teleport=None, init=None, chunk=0,
+              out=None):
+        """FP64 solve (nwgr_pr_run_f64): the same iteration in double, the
+        reference's Real = double; returns (ranks float64[n] in caller order,
+        RunResult).  Teleport and initial vectors are taken as float32."""
+        p = _params(damping, tol, max_iters, dangling, chunk, False)
+        keep = []
+        for name, vec in (("teleport", teleport), ("init", init)):
+            if vec is not None:
+                v = np.ascontiguousarray(vec, dtype=np.float32)
+                if v.shape != (self.n,):
+                    raise ValueError(f"{name} must have shape ({self.n},)")
+                keep.append(v)
+                setattr(p, name, v.ctypes.data)
+        if out is None:
+            out = np.empty(self.n, np.float64)
+        elif out.dtype != np.float64 or out.shape != (self.n,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float64 array of n entries")
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_run_f64(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
+        return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run_device64(self, out_ptr: int, damping=0.85, tol=1e-4, max_iters=20, dangling="leak",
+                     teleport_ptr=None, init_ptr=None, chunk=0, stream=None) -> RunResult:
+        """FP64 solve with the ranks (n doubles) written to device memory."""
+        p = _params(damping, tol, max_iters, dangling, chunk, False)
+        p.teleport = teleport_ptr
+        p.init = init_ptr
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_run_device_f64(self._h, ctypes.byref(p), out_ptr, ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
     def run_delta(self, damping=0.85, threshold=1e-4, max_iters=20, chunk=0, out=None):
         """Residual ("delta") solve as page_rank_v13
@@ -255,19 +315,50 @@ class PageRankGraph:
         return buf[:k].copy()
 
 
+def _resident(graph: CSRGraph, degrees, device: int) -> PageRankGraph:
+    """The device graph of a drop-in call: the cache's resident one when the
+    degrees are a contiguous uint32 array (as build_degrees returns them;
+    the cache keys on array addresses), else a handle for this call only."""
+    if degrees is not None:
+        deg = np.ascontiguousarray(degrees, dtype=np.uint32)
+        if deg is not degrees:  # a converted copy would never be found again
+            return PageRankGraph(graph, deg, device)
+    g, _ = PageRankGraph.cached(graph, degrees, device)
+    return g
+
+
+def _output(page_rank, n: int, dtype=np.float32):
+    """(array of `dtype` the solve writes, whether it is page_rank itself)."""
+    if isinstance(page_rank, np.ndarray) and page_rank.dtype == dtype and page_rank.flags.c_contiguous \
+            and page_rank.shape == (n,):
+        return page_rank, True
+    return np.empty(n, dtype), False
+
+
 def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, max_iters=20,
               num_threads=None, *, device: int = 0) -> None:
     """Drop-in for nw::graph::page_rank (page_rank.hpp:89-91).
 
-    ``page_rank`` must be a pre-sized float array of graph.size() entries
-    (the reference writes without resizing, page_rank.hpp:102); it is filled
-    in place.  ``num_threads`` is accepted and ignored, as in the reference.
-    Dangling vertices leak their mass, as in the reference."""
+    ``page_rank`` must be a pre-sized array of graph.size() entries (the
+    reference writes without resizing, page_rank.hpp:102); it is filled in
+    place.  Its dtype plays the role of the reference's Real: a float64
+    array gets the FP64 solve, anything else the FP32 one.
+    ``num_threads`` is accepted and ignored, as in the reference.  Dangling
+    vertices leak their mass, as in the reference.  The device graph is kept
+    between calls on the same arrays (nwgr_pr_graph_cached), so a repeated
+    call pays for the solve and the copy of the ranks only; an empty graph
+    returns at once."""
     if len(page_rank) != graph.size():
         raise ValueError("page_rank must be pre-sized to graph.size()")
-    with PageRankGraph(graph, degrees, device) as g:
-        r, _ = g.run(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak")
-    page_rank[:] = r
+    if graph.n == 0:
+        return
+    fp64 = getattr(page_rank, "dtype", None) == np.float64
+    with _resident(graph, degrees, device) as g:
+        out, direct = _output(page_rank, graph.n, np.float64 if fp64 else np.float32)
+        solve = g.run64 if fp64 else g.run
+        solve(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak", out=out)
+    if not direct:
+        page_rank[:] = out
 
 
 def page_rank_delta(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, max_iters=20,
@@ -276,13 +367,36 @@ def page_rank_delta(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, th
     632-634), the residual-push PageRank of pr.exe version 13; also returns
     what the serial page_rank_v3 (:566-568) returns.
 
-    ``page_rank`` is a pre-sized float array filled in place with the
-    unnormalised ranks; ``num_threads`` is accepted and ignored."""
+    ``page_rank`` is a pre-sized array filled in place with the unnormalised
+    ranks; ``num_threads`` is accepted and ignored.  The device graph is kept
+    between calls as for page_rank."""
     if len(page_rank) != graph.size():
         raise ValueError("page_rank must be pre-sized to graph.size()")
-    with PageRankGraph(graph, degrees, device) as g:
-        r, _ = g.run_delta(damping=damping_factor, threshold=threshold, max_iters=max_iters)
-    page_rank[:] = r
+    if graph.n == 0:
+        return
+    with _resident(graph, degrees, device) as g:
+        out, direct = _output(page_rank, graph.n)
+        g.run_delta(damping=damping_factor, threshold=threshold, max_iters=max_iters, out=out)
+    if not direct:
+        page_rank[:] = out
+
+
+def cache_clear() -> None:
+    """Destroy the resident graphs the drop-in calls keep (nwgr_pr_cache_clear):
+    needed after a graph's arrays are edited in place."""
+    pagerank_lib().nwgr_pr_cache_clear()
+
+
+def cache_stats() -> dict:
+    """Hits and misses of the resident-graph cache, and handles held."""
+    h, m, e = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+    pagerank_lib().nwgr_pr_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(e))
+    return {"hits": h.value, "misses": m.value, "entries": e.value}
+
+
+def cache_set_capacity(entries: int) -> None:
+    """Most resident graphs kept (1 .. 64)."""
+    check(pagerank_lib().nwgr_pr_cache_set_capacity(int(entries)))
 
 
 def build_csr_device(src, dst, n: int, dedup: bool = True, drop_self_loops: bool = True, stream=None):

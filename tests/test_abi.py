@@ -4,6 +4,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from nwgr_amd import _lib
@@ -125,3 +126,146 @@ void call(const fake_adjacency& g, const std::vector<unsigned>& d, std::vector<f
     r = subprocess.run([gxx, "-std=c++20", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+# ---- inputs the reference accepts: an empty graph; checks of caller vectors
+
+
+def _empty_handle(L):
+    off = np.zeros(1, np.uint32)
+    h = ctypes.c_void_p()
+    assert L.nwgr_pr_graph_create(off.ctypes.data, None, 0, 0, None, 0, ctypes.byref(h)) == _lib.NWGR_OK
+    assert h.value
+    return h
+
+
+def test_empty_graph_returns_normally_without_a_device():
+    """N = 0: the reference's page_rank initialises nothing, reduces an empty
+    range (error 0) and returns after its first iteration
+    (page_rank.hpp:92-147); the C ABI does the same and needs no GPU."""
+    L = _lib.pagerank_lib()
+    h = _empty_handle(L)
+    try:
+        assert L.nwgr_pr_graph_n(h) == 0 and L.nwgr_pr_graph_m(h) == 0
+        assert L.nwgr_pr_graph_device_bytes(h) == 0
+        p = _lib.PrParams()
+        L.nwgr_pr_params_default(ctypes.byref(p))
+        for fn in (L.nwgr_pr_run_f32, L.nwgr_pr_run_f64, L.nwgr_pr_delta_run_f32):
+            res = _lib.PrResult()
+            assert fn(h, ctypes.byref(p), None, ctypes.byref(res)) == _lib.NWGR_OK, L.nwgr_last_error()
+            assert (res.iters, res.converged, res.error) == (1, 1, 0.0)
+        res = _lib.PrResult()
+        assert L.nwgr_pr_run_device_f32(h, ctypes.byref(p), None, ctypes.byref(res), None) == _lib.NWGR_OK
+        assert res.iters == 1
+        # no sweeps asked for: none performed; tol 0 is never met
+        p.max_iters = 0
+        assert L.nwgr_pr_run_f32(h, ctypes.byref(p), None, ctypes.byref(res)) == _lib.NWGR_OK
+        assert (res.iters, res.converged) == (0, 0)
+        p.max_iters, p.tol = 7, 0.0
+        assert L.nwgr_pr_run_f32(h, ctypes.byref(p), None, ctypes.byref(res)) == _lib.NWGR_OK
+        assert (res.iters, res.converged) == (7, 0)
+        # tuning is kept and checked without a layout to re-plan
+        t = _lib.PrTuning()
+        assert L.nwgr_pr_graph_get_tuning(h, ctypes.byref(t)) == _lib.NWGR_OK
+        t.chunk = 0
+        assert L.nwgr_pr_graph_set_tuning(h, ctypes.byref(t)) == _lib.NWGR_EINVAL
+        # a bad parameter is still an error on an empty graph
+        p.damping = float("nan")
+        assert L.nwgr_pr_run_f32(h, ctypes.byref(p), None, ctypes.byref(res)) == _lib.NWGR_EINVAL
+    finally:
+        L.nwgr_pr_graph_destroy(h)
+
+
+def test_empty_graph_with_entries_is_einval():
+    L = _lib.pagerank_lib()
+    off = np.zeros(1, np.uint32)
+    h = ctypes.c_void_p()
+    assert L.nwgr_pr_graph_create(off.ctypes.data, off.ctypes.data, 0, 1, None, 0, ctypes.byref(h)) == _lib.NWGR_EINVAL
+
+
+def test_empty_graph_through_the_cache_and_the_mirror():
+    import nwgr_amd
+    g = nwgr_amd.CSRGraph(np.zeros(1, np.uint32), np.zeros(0, np.uint32))
+    deg = np.zeros(0, np.uint32)
+    before = nwgr_amd.cache_stats()
+    pg, hit = nwgr_amd.PageRankGraph.cached(g, deg)
+    assert pg.n == 0 and not hit
+    r, res = pg.run(tol=1e-4, max_iters=20)
+    assert len(r) == 0 and res.iters == 1 and res.converged
+    out = np.zeros(0, np.float64)
+    assert nwgr_amd.page_rank(g, deg, out, 0.85, 1e-4, 20) is None
+    assert nwgr_amd.cache_stats()["misses"] == before["misses"] + 1
+    nwgr_amd.cache_clear()
+    assert nwgr_amd.cache_stats()["entries"] == 0
+
+
+@pytest.mark.parametrize("bad", [float("nan"), -1e-9, float("inf"), -float("inf")])
+@pytest.mark.parametrize("field", ["teleport", "init"])
+def test_params_check_rejects_bad_vectors(field, bad):
+    L = _lib.pagerank_lib()
+    n = 1000
+    v = np.full(n, 1.0 / n, np.float32)
+    p = _lib.PrParams()
+    L.nwgr_pr_params_default(ctypes.byref(p))
+    setattr(p, field, v.ctypes.data)
+    assert L.nwgr_pr_params_check(ctypes.byref(p), n) == _lib.NWGR_OK
+    v[n - 1 if bad != bad else 17] = bad
+    assert L.nwgr_pr_params_check(ctypes.byref(p), n) == _lib.NWGR_EINVAL
+    assert field.encode() in L.nwgr_last_error()
+
+
+def test_params_check_scalars():
+    L = _lib.pagerank_lib()
+    p = _lib.PrParams()
+    L.nwgr_pr_params_default(ctypes.byref(p))
+    assert L.nwgr_pr_params_check(ctypes.byref(p), 0) == _lib.NWGR_OK
+    for field, val in (("damping", 1.5), ("damping", float("nan")), ("tol", float("nan")), ("dangling", 3)):
+        q = _lib.PrParams()
+        L.nwgr_pr_params_default(ctypes.byref(q))
+        setattr(q, field, val)
+        assert L.nwgr_pr_params_check(ctypes.byref(q), 0) == _lib.NWGR_EINVAL, field
+    q = _lib.PrParams()
+    L.nwgr_pr_params_default(ctypes.byref(q))
+    q.dangling = 2  # TELEPORT needs a vector
+    assert L.nwgr_pr_params_check(ctypes.byref(q), 0) == _lib.NWGR_EINVAL
+    assert L.nwgr_pr_params_check(None, 0) == _lib.NWGR_EINVAL
+
+
+def test_cache_capacity_bounds():
+    L = _lib.pagerank_lib()
+    assert L.nwgr_pr_cache_set_capacity(0) == _lib.NWGR_EINVAL
+    assert L.nwgr_pr_cache_set_capacity(65) == _lib.NWGR_EINVAL
+    assert L.nwgr_pr_cache_set_capacity(2) == _lib.NWGR_OK
+
+
+def test_cpp_dropin_empty_graph_and_double(tmp_path):
+    """page_rank<Graph, float / double> on an empty graph returns at once, as
+    the reference does for N = 0 (no device is touched)."""
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ missing")
+    src = tmp_path / "empty.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "nwgr/page_rank.hpp"
+int main() {
+  std::vector<uint32_t> offs{0}, idx, deg;
+  nw::graph::csr_view g{offs.data(), idx.data(), 0, 0};
+  std::vector<float> rf;
+  std::vector<double> rd;
+  nw::graph::page_rank(g, deg, rf, 0.85f, 1e-4f, 20, 1);
+  nw::graph::page_rank(g, deg, rd, 0.85, 1e-7, 100, 1);
+  nw::graph::page_rank_v13(g, deg, rf, 0.85f, 1e-4f, 20, 1);
+  std::puts("ok");
+  return 0;
+}
+''')
+    exe = tmp_path / "empty"
+    r = subprocess.run([gxx, "-std=c++20", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", os.path.dirname(_lib.PAGERANK_SO), "-lnwgr_pagerank",
+                        "-Wl,-rpath," + os.path.dirname(_lib.PAGERANK_SO), "-Wl,--allow-shlib-undefined"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr
