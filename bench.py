@@ -95,6 +95,27 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
+def copy_rate_gbs(mib=1024, reps=3):
+    """Bytes read + written per second by a plain device-to-device copy of
+    `mib` MiB (torch's copy kernel, the best of `reps`): the box's practical
+    streaming rate, against which measured traffic gives a ceiling."""
+    import torch
+    n = mib * 2**20 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2 * n * 4 / (best / 1e3) / 1e9
+
+
 def load_traffic(scale, ef, workload="pagerank"):
     """HBM bytes per sweep launch from committed rocprofv3 --pmc summaries."""
     name = "k_sweep_traffic_er_2pow25.json" if workload == "er" else f"k_sweep_traffic_rmat_s{scale}_ef{ef}.json"
@@ -104,6 +125,91 @@ def load_traffic(scale, ef, workload="pagerank"):
     with open(path) as f:
         d = json.load(f)
     return d.get("hbm_bytes_per_sweep")
+
+
+def host_info(threads):
+    """The host the CPU baseline runs on: CPU model, sockets, physical cores,
+    logical CPUs, the CPUs this process may use (affinity, cgroup quota) and
+    the threads the baseline used."""
+    info = {"threads_used": threads}
+    try:
+        model, phys, logical = None, set(), 0
+        sock = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    sock = v
+                elif k == "core id":
+                    phys.add((sock, v))
+        info.update({"cpu_model": model, "sockets": len({s for s, _ in phys}) or None,
+                     "physical_cores": len(phys) or None, "logical_cpus": logical})
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+        info["cgroup_cpu_quota"] = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    info["omp_num_threads_env"] = os.environ.get("OMP_NUM_THREADS")
+    pc = info.get("physical_cores")
+    info["all_physical_cores_used"] = bool(pc) and threads >= pc
+    return info
+
+
+def dropin_calls(g, deg, tol, max_iters, reps, pg, out_d):
+    """The drop-in as pr.exe times it (bench/pr.cpp:94-95 brackets the whole
+    page_rank call): nwgr_amd.page_rank on host arrays, LEAK dangling as the
+    reference, ranks written back into the caller's float array.  The first
+    call builds the resident device graph (upload, relabelling, tiled layout);
+    later calls find it in the cache and pay for the solve and the copy of the
+    ranks.  Beside them: the same solve on the resident handle with the ranks
+    left in device memory, and with them copied to the host."""
+    import torch
+    import nwgr_amd
+    nwgr_amd.cache_clear()
+    ranks = np.zeros(g.n, np.float32)
+    t0 = time.perf_counter()
+    nwgr_amd.page_rank(g, deg, ranks, 0.85, tol, max_iters)
+    first = (time.perf_counter() - t0) * 1e3
+    rep = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        nwgr_amd.page_rank(g, deg, ranks, 0.85, tol, max_iters)
+        rep.append((time.perf_counter() - t0) * 1e3)
+    kw = dict(damping=0.85, tol=tol, max_iters=max_iters, dangling="leak")
+    dev, host = [], []
+    out = np.empty(g.n, np.float32)
+    res = pg.run_device(out_d.data_ptr(), **kw)
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = pg.run_device(out_d.data_ptr(), **kw)
+        dev.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        pg.run(out=out, **kw)
+        host.append((time.perf_counter() - t0) * 1e3)
+    st = nwgr_amd.cache_stats()
+    same = bool(np.array_equal(out, ranks))
+    rep_ms, dev_ms = float(np.median(rep)), float(np.median(dev))
+    return {"semantics": "nwgr_amd.page_rank(graph, degrees, ranks, 0.85, tol, max_iters) wall time per call, "
+                         "dangling leaked as the reference, host vectors in and out (bench/pr.cpp:94-95)",
+            "first_call_ms": round(first, 3), "repeat_call_ms_median": round(rep_ms, 3),
+            "repeat_call_ms_min_max": [round(min(rep), 3), round(max(rep), 3)],
+            "resident_solve_device_output_ms_median": round(dev_ms, 3),
+            "resident_solve_host_output_ms_median": round(float(np.median(host)), 3),
+            "repeat_over_resident": round(rep_ms / dev_ms, 3), "sweeps_per_solve": int(res.iters),
+            "calls": reps + 1, "cache_hits": st["hits"], "ranks_equal_resident": same}
 
 
 def cpu_baseline(g, deg, target_s, threads, tol, max_iters):
@@ -527,6 +633,15 @@ def main():
             "avg_sweep_us": round(sweep_ms / sweeps * 1e3, 2),
             "launch": "one sweep: k_tile + k_combine or k_accum (tiled engine) or k_sweep (binned); the stopping launch "
                       "of each solve is timed with the sweeps"}
+    if traffic:
+        # what the design could reach: its measured bytes moved at the copy rate
+        cr = copy_rate_gbs()
+        t_min = traffic / (cr * 1e9)
+        roof["ceiling"] = {"copy_rate_gbs": round(cr, 1), "traffic_bytes_per_sweep": traffic,
+                           "sweep_us_at_copy_rate": round(t_min * 1e6, 2),
+                           "frac_at_copy_rate": round(b_iter / t_min / 1e9 / HBM_PEAK_GBS, 4),
+                           "traffic_rate_gbs": round(traffic * sweeps / (sweep_ms / 1e3) / 1e9, 1),
+                           "traffic_rate_over_copy_rate": round(traffic * sweeps / (sweep_ms / 1e3) / 1e9 / cr, 3)}
     value = m * sweeps / wall / 1e9
 
     cpu = None
@@ -535,6 +650,7 @@ def main():
         log(f"cpu baseline on {threads} threads (~{args.cpu_seconds:.0f}s)")
         cpu = cpu_baseline(g, deg, args.cpu_seconds, threads, args.tol, args.max_iters)
         cpu["value"] = round(cpu["value"], 4)
+        cpu["host"] = host_info(threads)
         log(f"cpu baseline {cpu['value']} GTEPS, {cpu['ms_per_solve']:.1f} ms per solve")
     fa = None
     if args.fwd_adjoint and args.workload == "pagerank":
@@ -568,6 +684,16 @@ def main():
         line["gpu_leak_solve"] = {"ms_per_solve": round(leak_ms, 4), "sweeps_per_solve": res_l.iters}
         line["gpu_vs_cpu_per_sweep"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
         line["gpu_vs_cpu_time_to_tolerance"] = round(cpu["ms_per_solve"] / leak_ms, 1)
+    if args.workload == "pagerank":
+        di = dropin_calls(g, deg, args.tol, args.max_iters, max(args.steps, 3), pg, out_d)
+        log(f"drop-in: first call {di['first_call_ms']} ms, repeated {di['repeat_call_ms_median']} ms, resident "
+            f"{di['resident_solve_device_output_ms_median']} ms")
+        line["dropin"] = di
+        if cpu:
+            # whole calls on both sides: pr.exe's timed region on the host path
+            # and on the drop-in (the GPU's includes the copy of the ranks)
+            line["dropin"]["cpu_whole_call_ms"] = round(cpu["ms_per_solve"], 2)
+            line["dropin"]["repeat_call_vs_cpu_whole_call"] = round(cpu["ms_per_solve"] / di["repeat_call_ms_median"], 1)
     line["fwd_adjoint_vs_fwd"] = fa
     print(json.dumps(line), flush=True)
 

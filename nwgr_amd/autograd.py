@@ -44,12 +44,19 @@ class _WeightedPageRank(torch.autograd.Function):
         graph = ctx.graph
         dw = torch.empty_like(w)
         dt = torch.empty(graph.n, dtype=torch.float32, device=w.device)
-        scale = float(g.abs().sum())
+        # The adjoint is linear in dL/dr: solving it for g / ||g||_1 with the
+        # absolute threshold adj_tol is the relative stop rule, and scaling the
+        # gradients back by ||g||_1 gives dL/dw, dL/dt.  The norm stays on the
+        # device (no host synchronisation before the adjoint is enqueued).
+        scale = g.abs().sum().clamp_min(1e-30)
+        gn = g / scale
         stream = torch.cuda.current_stream(w.device).cuda_stream
-        res = graph.wpr_adjoint(w.data_ptr(), r.data_ptr(), g.data_ptr(), dw.data_ptr(), dt.data_ptr(),
-                                ctx.damping, ctx.adj_tol * max(scale, 1e-30), ctx.adj_max_iters, ctx.dangling,
+        res = graph.wpr_adjoint(w.data_ptr(), r.data_ptr(), gn.data_ptr(), dw.data_ptr(), dt.data_ptr(),
+                                ctx.damping, ctx.adj_tol, ctx.adj_max_iters, ctx.dangling,
                                 tc.data_ptr() if ctx.has_t else None, stream=stream)
         ctx.graph.last_adjoint = res
+        dw.mul_(scale)
+        dt.mul_(scale)
         return dw, (dt if ctx.has_t else None), None, None, None, None, None, None, None
 
 
