@@ -8,8 +8,10 @@ of the oracle's (the batch stops when its slowest column converges), and the
 same top order per column.  At the full size (R-MAT s20, 256 columns) the
 oracle is too slow for every column, so the test uses a size-independent
 property instead: the iteration is linear in the teleport vector from a
-fixed start, so after any number of sweeps the mean of the columns equals the
-plain solve with the mean teleport (LEAK and UNIFORM dangling).
+fixed start, so after any number of sweeps a weighted combination of the
+columns (weights summing to 1) equals the plain solve with the combined
+teleport (LEAK and UNIFORM dangling), and 16 columns are checked against the
+oracle.
 """
 import numpy as np
 import pytest
@@ -98,9 +100,10 @@ def test_ppr_plain_solver_still_correct_after_batch(karate):
 @pytest.mark.slow
 @pytest.mark.parametrize("mode", ["leak", "uniform"])
 def test_config4a_rmat_s20_256_columns_linearity(mode):
-    """R-MAT s20 ef16, 256 one-hot seeds: mean of the columns after a fixed
-    number of sweeps equals the plain solve with the mean teleport; two
-    columns against the FP64 oracle."""
+    """R-MAT s20 ef16, 256 one-hot seeds, 30 sweeps: the mean of the columns
+    and four random weighted combinations of them equal the plain solve with
+    the combined teleport (which checks every column and its position); 16
+    columns against the FP64 oracle; each column's mass under UNIFORM."""
     import torch
     say(f"ppr s20 k=256 {mode}")
     g = nwgr_amd.rmat(20, 16)
@@ -122,11 +125,30 @@ def test_config4a_rmat_s20_256_columns_linearity(mode):
     err = l1(mean_cols, r_mix)
     say(f"  L1(mean of columns, mixed teleport) = {err:.2e}")
     assert err <= 1e-6
+    # Every column's identity: for random positive weights a (summing to 1,
+    # so the common start 1/n mixes to itself), sum_j a_j R[:, j] is the
+    # plain solve with teleport sum_j a_j e_{seed_j}.  A permutation of the
+    # columns, or one column wrong, changes the left side only.
+    for trial in range(4):
+        a = rng.dirichlet(np.ones(256))
+        mixed = (R.double() @ torch.from_numpy(a).to(dev)).cpu().numpy()
+        t_a = np.zeros(g.n, np.float64)
+        np.add.at(t_a, seeds, a)
+        r_a, res_a = pg.run(damping=0.85, tol=0.0, max_iters=sweeps, dangling=mode, teleport=t_a.astype(np.float32))
+        assert res_a.iters == sweeps
+        err = l1(mixed, r_a)
+        say(f"  combination {trial}: L1 {err:.2e}")
+        assert err <= 1e-6, (trial, err)
     Rh = R.cpu().numpy()
-    for j in (0, 255):
+    if mode == "uniform":  # each column is a distribution
+        mass = Rh.sum(axis=0, dtype=np.float64)
+        assert np.abs(mass - 1.0).max() < 1e-5, np.abs(mass - 1.0).max()
+    # columns at both ends, at the float4 lane boundaries and at random
+    cols = sorted({0, 1, 2, 3, 4, 127, 128, 253, 254, 255} | set(rng.choice(256, 6, replace=False).tolist()))
+    for j in cols:
         r64, _ = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 0.0, sweeps, dangling=MODES[mode],
                                teleport=onehot(g.n, seeds[j]), nthreads=16)
-        assert l1(Rh[:, j], r64) <= 1e-6
+        assert l1(Rh[:, j], r64) <= 1e-6, j
     # converged solve of the full batch
     res = pg.ppr_batch_device(sd.data_ptr(), 256, R.data_ptr(), tol=1e-6, max_iters=200, dangling=mode)
     assert res.converged and pg.ppr_column_errors().max() < 1e-6

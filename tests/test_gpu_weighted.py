@@ -166,9 +166,11 @@ def test_adjoint_without_teleport_grad():
 
 
 @pytest.mark.slow
+@pytest.mark.timeout(600)
 def test_config5_rmat_s20_gradients_and_cost():
     """R-MAT s20 ef16, w ~ U(0,1], t ~ Dirichlet(1), L = sum wl r: gradients vs
-    the FP64 adjoint and forward+adjoint time vs forward time."""
+    the FP64 adjoint and vs central finite differences (64 edges), and
+    forward+adjoint time vs forward time."""
     from nwgr_amd.autograd import weighted_pagerank
     say("config5 s20 generate")
     g, w, t, wl = case(20, seed=5)
@@ -201,3 +203,20 @@ def test_config5_rmat_s20_gradients_and_cost():
     say(f"config5 rel err dw {ew:.2e} dt {et:.2e}")
     assert ew <= 1e-4 and et <= 1e-4
     assert t_fb / t_f <= 2.5
+    # central finite differences of the FP64 forward at this size (h = 1e-3,
+    # 64 random edges), each perturbed solve warm-started from r64
+    say("config5 finite differences (64 edges)")
+    rng = np.random.default_rng(55)
+    h = 1e-3
+    ks = rng.choice(g.m, 64, replace=False)
+    fd = np.empty(len(ks))
+    for i, k in enumerate(ks):
+        wp, wm = w64.copy(), w64.copy()
+        wp[k] += h
+        wm[k] -= h
+        rp, _ = oracle.wpr_forward(g.offsets, g.indices, wp, t64, D, 1e-15, 5000, oracle.LEAK, nthreads=16, init=r64)
+        rm, _ = oracle.wpr_forward(g.offsets, g.indices, wm, t64, D, 1e-15, 5000, oracle.LEAK, nthreads=16, init=r64)
+        fd[i] = (wl64 @ rp - wl64 @ rm) / (2 * h)
+    efd = rel2(wd.grad.cpu().numpy()[ks], fd)
+    say(f"config5 rel err vs FD (64 edges) {efd:.2e}")
+    assert efd <= 1e-4
