@@ -379,6 +379,19 @@ int nwgr_top_k_device_f32(const float* d_ranks, uint32_t n, uint32_t k, uint32_t
                           void* stream);
 int nwgr_top_k_f32(const float* ranks, uint32_t n, uint32_t k, uint32_t* ids, float* vals, int device);
 
+/* nwgr_top_k_device_f32 without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The sort's scratch (keys, ids, the radix sort's temporary storage) is a
+ * fixed 68 MiB array in the library's device memory, one per GPU, held until
+ * the process ends; n <= 4194304 (2^22), else NWGR_EINVAL.  Same results as
+ * nwgr_top_k_device_f32, on the current device, enqueued on `stream` without
+ * synchronising it.  Every call on one GPU uses that GPU's one scratch: calls
+ * that overlap on one GPU (other streams, other threads) race; calls on one
+ * stream are ordered. */
+int nwgr_top_k_device_f32_noalloc(const float* d_ranks, uint32_t n, uint32_t k, uint32_t* d_ids, float* d_vals,
+                                  void* stream);
+
 /* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED, NWGR_ENGINE_TILED
  * (second pass GATHER) or 2 (tiled, second pass ACCUMULATE); -1 before the
  * first solve or for a null handle. */

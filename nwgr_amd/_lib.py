@@ -132,6 +132,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_last_engine.argtypes = [vp]
     L.nwgr_top_k_device_f32.restype = i32
     L.nwgr_top_k_device_f32.argtypes = [vp, u32, u32, vp, vp, vp]
+    L.nwgr_top_k_device_f32_noalloc.restype = i32
+    L.nwgr_top_k_device_f32_noalloc.argtypes = [vp, u32, u32, vp, vp, vp]
     L.nwgr_top_k_f32.restype = i32
     L.nwgr_top_k_f32.argtypes = [vp, u32, u32, vp, vp, i32]
     L.nwgr_ppr_batch_f32.restype = i32
