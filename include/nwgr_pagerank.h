@@ -168,6 +168,29 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
 int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r_out,
                            nwgr_pr_result* res, void* stream);
 
+/* Block Gauss-Seidel PageRank: the in-place sweep of the reference's
+ * page_rank (v11, page_rank.hpp:124-139: each row's new rank and
+ * contribution replace the old ones within the sweep) made deterministic.
+ * The rows, in the handle's relabelled order (nwgr_pr_gs_phases), are cut
+ * into `phases` consecutive phases of about equal entry counts; inside a
+ * phase every row gathers the contributions as they stood when the phase
+ * began, and a phase's results replace the old ones before the next phase
+ * gathers.  phases = 1 is the Jacobi solve of nwgr_pr_run_f32; more phases
+ * take fewer sweeps to a tolerance (one row per phase would be the serial
+ * v11 sweep) and launch two kernels per phase.  Dangling mass leaks (LEAK
+ * only, as in the reference); teleport and init as for nwgr_pr_run_f32;
+ * error = the L1 change of the whole sweep, stop rule and result as the
+ * Jacobi solve.  phases in [1, 64]. */
+int nwgr_pr_gs_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phases, float* r_out,
+                       nwgr_pr_result* res);
+int nwgr_pr_gs_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phases, float* d_r_out,
+                              nwgr_pr_result* res, void* stream);
+/* The sweep order of the block Gauss-Seidel solve: order[i] (n entries,
+ * caller ids; may be NULL) is the i-th row visited -- in-degree
+ * non-increasing, ties by ascending id -- and cuts[0 .. phases] (may be
+ * NULL) the positions where the phases begin (cuts[phases] = n). */
+int nwgr_pr_gs_phases(const nwgr_pr_graph* g, uint32_t phases, uint32_t* order, uint32_t* cuts);
+
 /* FP64 PageRank: the iteration and stop rule of nwgr_pr_run_f32 with double
  * ranks, contributions and sums and the integer degrees converted exactly --
  * the reference's Real = double (page_rank.hpp:89-91; its PageRank test runs
@@ -393,8 +416,8 @@ int nwgr_top_k_device_f32_noalloc(const float* d_ranks, uint32_t n, uint32_t k, 
                                   void* stream);
 
 /* Engine the last solve on `g` ran: NWGR_ENGINE_BINNED, NWGR_ENGINE_TILED
- * (second pass GATHER) or 2 (tiled, second pass ACCUMULATE); -1 before the
- * first solve or for a null handle. */
+ * (second pass GATHER), 2 (tiled, second pass ACCUMULATE) or 3 (tiled block
+ * Gauss-Seidel phases); -1 before the first solve or for a null handle. */
 int nwgr_pr_last_engine(const nwgr_pr_graph* g);
 
 /* Thread-local message of the last failure on this thread ("" if none). */

@@ -262,6 +262,8 @@ struct TiledTask {
 
 struct Tiled {
   bool built = false;
+  uint32_t row_lo = 0, row_hi = 0;  // the rows whose entries the layout holds; the second pass writes only these
+  bool quiet_tail = true;        // its second pass also refreshes the quiet rows (rows [nz, n) without entries)
   uint32_t bits = 0;             // tile width 2^bits the layout was built with
   uint32_t ntiles = 0, nchunks = 0;
   uint32_t nsub = 0;             // fragments including the dummy ones
@@ -269,7 +271,7 @@ struct Tiled {
   uint32_t fill_cost = 0;        // tile fill cost the tasks were planned with
   uint32_t ntasks = 0;
   uint32_t nwave_rows = 0;
-  uint32_t row0 = 0;             // rows [0, row0) all have more than kWaveRowMin partials
+  uint32_t row0 = 0;             // rows [row_lo, row0) all have more than kWaveRowMin partials
   DevBuf eloc;       // u16[nchunks * kChunkEdges]
   DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
   DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)
@@ -305,6 +307,11 @@ enum SolveKind : uint32_t {
   kNumKinds = 4,
   kCoef = 4,      // one pass on the transpose: gradient coefficients from the final lambda
 };
+// Captured-graph slot of the block Gauss-Seidel solve (plain kind, phases).
+constexpr uint32_t kExecGs = kNumKinds;
+constexpr uint32_t kExecSlots = kNumKinds + 1;
+// Most phases of a block Gauss-Seidel sweep.
+constexpr uint32_t kMaxGsPhases = 64;
 
 // Per-edge weights enter the sweep of these kinds.
 constexpr bool weighted_kind(uint32_t kind) { return kind == kWeighted || kind == kAdjoint || kind == kCoef; }
@@ -324,6 +331,9 @@ struct SolveSpec {
   const float* tw = nullptr;       // weights in the tiled order (weighted kinds, tiled engine)
   bool quiet = false;              // rows [g.nz, n) kept as the scalar State::rq (plain kind, tiled)
   float thr = 0.f;                 // push threshold (kDelta)
+  // block Gauss-Seidel: one tiled layout per phase (rows of consecutive
+  // ranges), swept in order over one contribution vector updated in place
+  const std::vector<Tiled>* phases = nullptr;
 };
 
 // Device state of the batched personalized solve (ppr.hip).
@@ -395,10 +405,18 @@ struct Graph {
   HostFlag* hflag = nullptr;   // pinned, mapped
   HostFlag* hflag_dev = nullptr;
 
-  // captured chunk graphs: [kind][parity of the first sweep][length]
+  // captured chunk graphs: [kind or kExecGs][parity of the first sweep][length]
   static constexpr uint32_t kMaxChunk = 32;
-  hipGraphExec_t exec[kNumKinds][2][kMaxChunk + 1] = {};
-  uint64_t exec_key[kNumKinds] = {};
+  hipGraphExec_t exec[kExecSlots][2][kMaxChunk + 1] = {};
+  uint64_t exec_key[kExecSlots] = {};
+
+  // block Gauss-Seidel (built on first use): the phases' layouts, their row
+  // cuts (positions in the relabelled order, phases + 1 entries) and the
+  // per-phase (error, dangling) pairs [2 parities][phases][grid_c][2]
+  std::vector<Tiled> gs;
+  std::vector<uint32_t> gs_cuts;
+  uint32_t gs_bits = 0;
+  DevBuf gs_part;
   hipStream_t own_stream = nullptr;
 
   std::vector<double> last_errs;
@@ -414,7 +432,7 @@ struct Graph {
   uint32_t ppr_cols = 0;
 
   void drop_graphs(int kind = -1) {
-    for (int k = 0; k < kNumKinds; ++k) {
+    for (int k = 0; k < static_cast<int>(kExecSlots); ++k) {
       if (kind >= 0 && k != kind) continue;
       for (auto& row : exec[k])
         for (auto& e : row)
@@ -461,8 +479,15 @@ void ensure_transpose(Graph& g, hipStream_t s);
 void validate_tuning(const Tuning& t);
 // validates g.tune, rebuilds the plans and the sweep grid (graph.hip)
 void replan(Graph& g, hipStream_t s);
-// tiled layout of a CSR over rows [0, n) (tiled.hip); no-op once built
-void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s);
+// tiled layout of a CSR over n rows with m entries, all in rows [row_lo,
+// row_hi) (tiled.hip); no-op once built
+void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, uint64_t m, uint32_t row_lo,
+                 uint32_t row_hi, hipStream_t s);
+// phase cuts of block Gauss-Seidel: `phases` consecutive ranges of the
+// relabelled rows with about equal entry counts (positions, phases + 1)
+std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases);
+// the phases' layouts for `phases` phases (tiled.hip); no-op once built
+void ensure_gs(Graph& g, uint32_t phases, hipStream_t s);
 // row blocks and segment table of the accumulate form of the second pass
 // (tiled.hip); no-op once built
 void build_accum(Graph& g, Tiled& t, hipStream_t s);
@@ -487,6 +512,8 @@ inline size_t iter_len(uint32_t n) {
 // g.r[res.iters & 1] (new labels).
 void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
+// block Gauss-Seidel solve with `phases` phases (LEAK dangling)
+void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s);
 // FP64 Jacobi solve (fp64.hip); d_out: n doubles in caller order
 void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_result& res, hipStream_t s);
 // residual ("delta") solve, page_rank_v13; p.tol is the push threshold and

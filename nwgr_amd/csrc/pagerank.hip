@@ -559,6 +559,8 @@ struct TileArgs {
   uint32_t bits;    // tile width 2^bits
   uint32_t p;
   uint32_t max_iters;
+  uint32_t check;   // 1: this launch takes the stop decision (the first tile pass of a sweep); 0: it only
+                    // returns when the sweep's first pass stopped the solve (later block Gauss-Seidel phases)
   double tol;
 };
 
@@ -569,7 +571,9 @@ struct CombineArgs {
   const float* __restrict__ partial;
   const uint32_t* __restrict__ wave_rows;
   uint32_t nwave_rows;
-  uint32_t row0;  // rows [0, row0) all have more than kWaveRowMin partials (a wave each)
+  uint32_t row0;        // rows [row_lo, row0) all have more than kWaveRowMin partials (a wave each)
+  uint32_t row_hi;      // the layout's rows end here (they start at or before row0)
+  uint32_t quiet_tail;  // 1: this pass also refreshes the quiet rows
 };
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -948,7 +952,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   // the task range, the stop flag and the previous sweep's pairs are loaded
   // together: one memory round trip before the first tile, not three
   const uint32_t t_lo = a.wg_task[vb], t_hi = a.wg_task[vb + 1];
-  if constexpr (KIND != kCoef) {
+  if (KIND != kCoef && !a.check) {
+    // a later phase of a block Gauss-Seidel sweep: the sweep's first tile
+    // pass took the stop decision
+    if (tid == 0) *s_flag = a.st->done;
+    __syncthreads();
+    if (*s_flag) return;
+  } else if constexpr (KIND != kCoef) {
     // totals of the previous sweep, identical in every workgroup
     double E = 0, D = 0;
     for (uint32_t b = tid; b < a.nparts; b += kSweepThreads) {
@@ -1115,7 +1125,7 @@ __global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
   // that fits, found with one counting barrier); each step's partials are
   // gathered into LDS by the whole workgroup, with four loads in flight per
   // thread, then every thread adds its row's partials in tile order.
-  const uint32_t n = a.s.nrows;  // quiet rows [nrows, n) are not stored
+  const uint32_t n = min(a.s.nrows, a.row_hi);  // quiet rows [nrows, n) are not stored
   for (uint32_t w0 = a.row0 + vb * kCombineThreads; w0 < n; w0 += G * kCombineThreads) {
     const uint32_t w1 = min(w0 + kCombineThreads, n);
     for (uint32_t r0 = w0; r0 < w1;) {
@@ -1158,7 +1168,7 @@ __global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
       r0 += nfit;
     }
   }
-  quiet_rows<KIND, kCombineThreads>(a.s, Df, err, dang);
+  if (a.quiet_tail) quiet_rows<KIND, kCombineThreads>(a.s, Df, err, dang);
   if constexpr (KIND != kCoef) {
     block_sum2<kCombineThreads>(err, dang, s_red);
     if (tid == 0) {
@@ -1350,9 +1360,18 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
     a.c_in = g.c[parity ^ 1].as<float>();
     a.c_out = g.c[parity].as<float>();
   }
-  double* part = g.partials.as<double>();
-  a.part_in = part + 2 * g.part_stride * (parity ^ 1);
-  a.part_out = part + 2 * g.part_stride * parity;
+  if (sp.phases) {
+    // block Gauss-Seidel: contributions updated in place, phase by phase;
+    // pairs in blocks of grid_c per phase
+    a.c_in = a.c_out = g.c[0].as<float>();
+    const size_t stride = 2 * static_cast<size_t>(g.grid_c) * sp.phases->size();
+    a.part_in = g.gs_part.as<double>() + stride * (parity ^ 1);
+    a.part_out = g.gs_part.as<double>() + stride * parity;
+  } else {
+    double* part = g.partials.as<double>();
+    a.part_in = part + 2 * g.part_stride * (parity ^ 1);
+    a.part_out = part + 2 * g.part_stride * parity;
+  }
   a.st = g.state.as<State>();
   a.hflag = g.hflag_dev;
   a.hist = g.hist.as<double>();
@@ -1406,8 +1425,12 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
   }
 }
 
-// One tiled sweep: k_tile over the workgroup task ranges, then k_combine.
-void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
+// One tiled pass pair over layout `t`: k_tile over the workgroup task ranges,
+// then the second pass (k_combine, or k_accum in the accumulate form).
+// `check`: the tile pass takes the stop decision from the `nparts` pairs
+// the previous sweep wrote.
+void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, const SweepArgs& sa, uint32_t check,
+                   uint32_t nparts, hipStream_t s) {
   static const bool attrs = [] {
     const int bytes = static_cast<int>(tile_lds_bytes(kMaxTileBits));
     for (const void* f : {reinterpret_cast<const void*>(&k_tile<kPlain>),
@@ -1419,10 +1442,9 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
     return true;
   }();
   (void)attrs;
-  const Tiled& t = *sp.tiled;
   TileArgs ta{};
   ta.eloc = t.eloc.as<uint16_t>();
-  ta.w = sp.tw;
+  ta.w = tw;
   ta.lane_sp0 = t.lane_sp0.as<uint32_t>();
   ta.tasks = t.tasks.as<TiledTask>();
   ta.wg_task = t.wg_task.as<uint32_t>();
@@ -1433,10 +1455,11 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ta.hflag = sa.hflag;
   ta.hist = sa.hist;
   ta.hist_cap = sa.hist_cap;
-  ta.nparts = g.grid_c;
+  ta.nparts = nparts;
   ta.bits = t.bits;
   ta.p = sa.p;
   ta.max_iters = sa.max_iters;
+  ta.check = check;
   ta.tol = sa.tol;
   CombineArgs ca{};
   ca.s = sa;
@@ -1446,6 +1469,8 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   ca.wave_rows = t.wave_rows.as<uint32_t>();
   ca.nwave_rows = t.nwave_rows;
   ca.row0 = t.row0;
+  ca.row_hi = t.row_hi;
+  ca.quiet_tail = t.quiet_tail ? 1u : 0u;
   const size_t shm = tile_lds_bytes(t.bits);
   const uint32_t nb = t.nblocks;
   if (t.acc_on) {
@@ -1462,7 +1487,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
     (void)acc_attrs;
     AccArgs q{t.ablo.as<uint32_t>(), t.aseg.as<uint32_t>(), t.afrow.as<uint16_t>(), t.nab, t.ntiles, t.acc_rows};
     const size_t ashm = sizeof(double) * (t.acc_rows + kWave);
-    switch (sp.kind) {
+    switch (kind) {
       case kPlain:
         k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
         k_accum<kPlain><<<g.grid_c, kSweepThreads, ashm, s>>>(ca, q);
@@ -1487,7 +1512,7 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
     NWGR_HIP(hipGetLastError());
     return;
   }
-  switch (sp.kind) {
+  switch (kind) {
     case kPlain:
       k_tile<kPlain><<<nb, kSweepThreads, shm, s>>>(ta);
       k_combine<kPlain><<<g.grid_c, kCombineThreads, 0, s>>>(ca);
@@ -1512,16 +1537,35 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
   NWGR_HIP(hipGetLastError());
 }
 
+// One tiled sweep of the whole-graph layout.
+void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
+  launch_layout(g, sp.kind, *sp.tiled, sp.tw, sa, 1, g.grid_c, s);
+}
+
+// One block Gauss-Seidel sweep: the phases' pass pairs in order over the one
+// contribution vector; phase q writes its (error, dangling) pairs to its own
+// block, and the next sweep's first tile pass adds all of them.
+void launch_gs(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
+  const uint32_t P = static_cast<uint32_t>(sp.phases->size());
+  for (uint32_t q = 0; q < P; ++q) {
+    SweepArgs aq = sa;
+    aq.part_out = sa.part_out + 2 * static_cast<size_t>(g.grid_c) * q;
+    launch_layout(g, sp.kind, (*sp.phases)[q], nullptr, aq, q == 0 ? 1u : 0u, P * g.grid_c, s);
+  }
+}
+
 // Captured chunk of `len` consecutive sweep launches whose first launch has
 // parity `parity` (sweep j has parity j & 1).
 hipGraphExec_t chunk_graph(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uint32_t parity, uint32_t len,
                            hipStream_t s) {
-  hipGraphExec_t& e = g.exec[sp.kind][parity][len];
+  hipGraphExec_t& e = g.exec[sp.phases ? kExecGs : sp.kind][parity][len];
   if (e) return e;
   NWGR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (uint32_t i = 0; i < len; ++i) {
     const SweepArgs a = sweep_args(g, sp, p, parity ^ (i & 1));
-    if (sp.tiled)
+    if (sp.phases)
+      launch_gs(g, sp, a, s);
+    else if (sp.tiled)
       launch_tiled(g, sp, a, s);
     else
       launch_sweep(sp.kind, g.grid, a, s);
@@ -1639,11 +1683,15 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   for (const void* ptr : {static_cast<const void*>(sp.offs), static_cast<const void*>(sp.idx),
                           static_cast<const void*>(sp.w), static_cast<const void*>(sp.div),
                           static_cast<const void*>(sp.gsrc), static_cast<const void*>(sp.plan),
-                          static_cast<const void*>(sp.tiled), static_cast<const void*>(sp.tw)})
+                          static_cast<const void*>(sp.tiled), static_cast<const void*>(sp.tw),
+                          sp.phases ? static_cast<const void*>(sp.phases->data()) : nullptr,
+                          static_cast<const void*>(g.gs_part.p)})
     key = fnv_mix(key, reinterpret_cast<uintptr_t>(ptr));
-  if (g.exec_key[sp.kind] != key) {
-    g.drop_graphs(static_cast<int>(sp.kind));
-    g.exec_key[sp.kind] = key;
+  key = fnv_mix(key, sp.phases ? sp.phases->size() : 0);
+  const uint32_t slot = sp.phases ? kExecGs : sp.kind;
+  if (g.exec_key[slot] != key) {
+    g.drop_graphs(static_cast<int>(slot));
+    g.exec_key[slot] = key;
   }
 
   Events evs;
@@ -1659,7 +1707,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.tel = has_tel ? g.tel.as<float>() : nullptr;
   ia.r0 = g.r[0].as<float>();
   ia.c0 = g.c[0].as<float>();
-  ia.part0 = g.partials.as<double>();
+  ia.part0 = sp.phases ? g.gs_part.as<double>() : g.partials.as<double>();
   ia.st = g.state.as<State>();
   ia.n = g.n;
   ia.adjoint = sp.kind == kAdjoint;
@@ -1843,6 +1891,38 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   (void)hipEventDestroy(e1);
   NWGR_HIP(le);
   res.ms += ms;
+}
+
+void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res,
+            hipStream_t s) {
+  validate_params(p);
+  NWGR_REQUIRE(p.dangling == kDanglingLeak,
+               "block Gauss-Seidel takes LEAK dangling (the reference's in-place sweep redistributes none)");
+  NWGR_REQUIRE(phases >= 1 && phases <= kMaxGsPhases, "phases must be in [1, 64]");
+  if (!g.vflag.p) g.vflag.alloc(sizeof(uint32_t));
+  check_vectors(p, g.n, g.vflag.as<uint32_t>(), s);
+  ensure_gs(g, phases, s);
+  SolveSpec sp;
+  sp.kind = kPlain;
+  sp.plan = &g.fwd;
+  sp.offs = g.offs.as<uint32_t>();
+  sp.idx = g.idx.as<uint32_t>();
+  sp.div = g.degf.as<float>();
+  sp.init = p.init;
+  sp.tel_src = p.teleport;
+  sp.phases = &g.gs;
+  sp.tiled = &g.gs[0];
+  sp.quiet = !p.teleport && !p.init && g.nz < g.n;
+  if (sp.quiet) prepare_quiet(g, s);
+  // blocks of pairs the first sweep's first phase adds beside init's block 0
+  NWGR_HIP(hipMemsetAsync(g.gs_part.p, 0, g.gs_part.bytes, s));
+  solve(g, sp, p, res, s);
+  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n,
+      sp.quiet ? g.nz : g.n, 1.0f, d_out);
+  NWGR_HIP(hipGetLastError());
+  NWGR_HIP(hipStreamSynchronize(s));
+  g.last_engine = 3;
 }
 
 void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
@@ -2057,6 +2137,70 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
     nwgr::run_plain(*g, q, out, *res, s);
     NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nwgr_pr_gs_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phases, float* d_r_out,
+                              nwgr_pr_result* res, void* stream) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    NWGR_REQUIRE(phases >= 1 && phases <= nwgr::kMaxGsPhases, "phases must be in [1, 64]");
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(d_r_out != nullptr, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
+    nwgr::run_gs(*g, *p, phases, d_r_out, *res, s);
+  });
+}
+
+int nwgr_pr_gs_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phases, float* r_out,
+                       nwgr_pr_result* res) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    NWGR_REQUIRE(phases >= 1 && phases <= nwgr::kMaxGsPhases, "phases must be in [1, 64]");
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(r_out != nullptr, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = g->own_stream;
+    const size_t nb = sizeof(float) * g->n;
+    auto staged = [&](nwgr::DevBuf& b) -> float* {
+      if (b.bytes < nb) b.alloc(nb);
+      return b.as<float>();
+    };
+    float* out = staged(g->out_buf);
+    nwgr_pr_params q = *p;
+    if (p->teleport) {
+      q.teleport = staged(g->tel_h);
+      NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.teleport), p->teleport, nb, hipMemcpyHostToDevice, s));
+    }
+    if (p->init) {
+      q.init = staged(g->init_h);
+      NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.init), p->init, nb, hipMemcpyHostToDevice, s));
+    }
+    nwgr::run_gs(*g, q, phases, out, *res, s);
+    NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nwgr_pr_gs_phases(const nwgr_pr_graph* g, uint32_t phases, uint32_t* order, uint32_t* cuts) {
+  return guarded([&] {
+    NWGR_REQUIRE(g != nullptr, "null argument");
+    NWGR_REQUIRE(phases >= 1 && phases <= nwgr::kMaxGsPhases, "phases must be in [1, 64]");
+    if (cuts) {
+      if (g->n == 0) {
+        std::fill(cuts, cuts + phases + 1, 0u);
+      } else {
+        const std::vector<uint32_t> c = nwgr::gs_cut_rows(*g, phases);
+        std::copy(c.begin(), c.end(), cuts);
+      }
+    }
+    if (order && g->n) {
+      nwgr::DeviceGuard dg(g->device);
+      NWGR_HIP(hipMemcpy(order, g->perm.p, sizeof(uint32_t) * g->n, hipMemcpyDeviceToHost));
+    }
   });
 }
 

@@ -332,12 +332,14 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
   NWGR_HIP(hipStreamSynchronize(s));
 }
 
-void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, hipStream_t s) {
+void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, uint64_t m, uint32_t row_lo,
+                 uint32_t row_hi, hipStream_t s) {
   if (t.built) return;
   const uint32_t n = g.n;
-  const uint64_t m = g.m;
   const uint32_t bits = g.tune.tile_bits;
   t.bits = bits;
+  t.row_lo = row_lo;
+  t.row_hi = row_hi;
   t.ntiles = static_cast<uint32_t>((static_cast<uint64_t>(n) + (1u << bits) - 1) >> bits);
   const uint32_t ntiles = t.ntiles;
   DevBuf erow, tkey, tval, tile_cnt, tk_sorted, tperm;
@@ -472,10 +474,10 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   NWGR_HIP(hipMemcpyAsync(pp.data(), t.pair_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, s));
   NWGR_HIP(hipStreamSynchronize(s));
   std::vector<uint32_t> wave_rows;
-  for (uint32_t r = 0; r < n; ++r)
+  for (uint32_t r = row_lo; r < row_hi; ++r)
     if (pp[r + 1] - pp[r] > kWaveRowMin) wave_rows.push_back(r);
-  t.row0 = 0;
-  while (t.row0 < wave_rows.size() && wave_rows[t.row0] == t.row0) ++t.row0;
+  t.row0 = row_lo;
+  while (t.row0 - row_lo < wave_rows.size() && wave_rows[t.row0 - row_lo] == t.row0) ++t.row0;
   t.nwave_rows = static_cast<uint32_t>(wave_rows.size());
   t.wave_rows.alloc(sizeof(uint32_t) * std::max<size_t>(wave_rows.size(), 1));
   if (!wave_rows.empty())
@@ -554,6 +556,56 @@ void tiled_weights(Graph& g, Tiled& t, const float* weights, hipStream_t s) {
   (void)g;
 }
 
+namespace {
+// Offsets of the rows [lo, hi) of a CSR as a CSR of its own over all n rows
+// (the other rows empty): offs[i] clamped to [offs[lo], offs[hi]], shifted.
+__global__ void k_sub_offsets(const uint32_t* __restrict__ offs, uint32_t n, uint32_t e_lo, uint32_t e_hi,
+                              uint32_t* __restrict__ out) {
+  GRID_STRIDE(i, static_cast<uint64_t>(n) + 1) out[i] = min(max(offs[i], e_lo), e_hi) - e_lo;
+}
+}  // namespace
+
+std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
+  // phase q starts at the first row whose entries begin at or after q m / P
+  std::vector<uint32_t> cuts(phases + 1, 0);
+  const auto& oh = g.offs_h;
+  for (uint32_t q = 1; q < phases; ++q) {
+    const uint64_t target = (static_cast<uint64_t>(q) * g.m + phases - 1) / phases;
+    const uint32_t pos = static_cast<uint32_t>(std::lower_bound(oh.begin(), oh.end(), target) - oh.begin());
+    cuts[q] = std::clamp<uint32_t>(pos, cuts[q - 1], g.n);
+  }
+  cuts[phases] = g.n;
+  return cuts;
+}
+
+void ensure_gs(Graph& g, uint32_t phases, hipStream_t s) {
+  if (g.gs.size() == phases && g.gs_bits == g.tune.tile_bits) {
+    for (Tiled& t : g.gs)
+      if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
+    return;
+  }
+  g.gs.clear();
+  g.drop_graphs(static_cast<int>(kExecGs));
+  const std::vector<uint32_t> cuts = gs_cut_rows(g, phases);
+  DevBuf sub;
+  sub.alloc(sizeof(uint32_t) * (static_cast<size_t>(g.n) + 1));
+  std::vector<Tiled> layouts(phases);
+  for (uint32_t q = 0; q < phases; ++q) {
+    const uint32_t lo = cuts[q], hi = cuts[q + 1];
+    const uint32_t e_lo = g.offs_h[lo], e_hi = g.offs_h[hi];
+    k_sub_offsets<<<grid_for(static_cast<uint64_t>(g.n) + 1), kT, 0, s>>>(g.offs.as<uint32_t>(), g.n, e_lo, e_hi,
+                                                                          sub.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    layouts[q].quiet_tail = q + 1 == phases;
+    build_tiled(g, layouts[q], sub.as<uint32_t>(), g.idx.as<uint32_t>() + e_lo, e_hi - e_lo, lo, hi, s);
+    NWGR_HIP(hipStreamSynchronize(s));  // `sub` is rewritten for the next phase
+  }
+  g.gs = std::move(layouts);
+  g.gs_cuts = cuts;
+  g.gs_bits = g.tune.tile_bits;
+  g.gs_part.alloc(sizeof(double) * 2 * 2 * static_cast<size_t>(phases) * g.grid_c);
+}
+
 Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
   if (g.tune.engine == kEngineBinned) return nullptr;
   const uint32_t* offs = adj ? g.t_offs.as<uint32_t>() : g.offs.as<uint32_t>();
@@ -582,7 +634,7 @@ Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
     }
     if (!choice) return nullptr;
   }
-  build_tiled(g, t, offs, idx, s);
+  build_tiled(g, t, offs, idx, g.m, 0, g.n, s);
   if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
   // second pass: accumulate when rows have about one entry per tile run
   const uint32_t mode = g.tune.tile_combine;

@@ -162,9 +162,10 @@ class PageRankGraph:
     def last_engine(self) -> str | None:
         """Engine of the last solve: "binned", "tiled" (second pass gathering
         each row's partials), "tiled-accumulate" (second pass accumulating
-        row blocks in LDS), or None before any."""
+        row blocks in LDS), "tiled-gauss-seidel" (block Gauss-Seidel phases),
+        or None before any."""
         e = pagerank_lib().nwgr_pr_last_engine(self._h)
-        return {0: "binned", 1: "tiled", 2: "tiled-accumulate"}.get(e)
+        return {0: "binned", 1: "tiled", 2: "tiled-accumulate", 3: "tiled-gauss-seidel"}.get(e)
 
     def last_sweep_times(self) -> np.ndarray:
         """Device ms of every sweep of the last run if it was traced, else empty."""
@@ -196,6 +197,49 @@ class PageRankGraph:
         res = PrResult()
         check(pagerank_lib().nwgr_pr_run_f32(self._h, ctypes.byref(p), out.ctypes.data, ctypes.byref(res)))
         return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run_gs(self, phases=8, damping=0.85, tol=1e-4, max_iters=20, teleport=None, init=None, chunk=0,
+               trace=False, out=None):
+        """Block Gauss-Seidel solve (nwgr_pr_gs_run_f32): the relabelled rows
+        cut into `phases` phases updated in place one after the other, LEAK
+        dangling; returns (ranks float32[n] in caller order, RunResult)."""
+        p = _params(damping, tol, max_iters, "leak", chunk, trace)
+        keep = []
+        for name, vec in (("teleport", teleport), ("init", init)):
+            if vec is not None:
+                v = np.ascontiguousarray(vec, dtype=np.float32)
+                if v.shape != (self.n,):
+                    raise ValueError(f"{name} must have shape ({self.n},)")
+                keep.append(v)
+                setattr(p, name, v.ctypes.data)
+        if out is None:
+            out = np.empty(self.n, np.float32)
+        elif out.dtype != np.float32 or out.shape != (self.n,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float32 array of n entries")
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_gs_run_f32(self._h, ctypes.byref(p), int(phases), out.ctypes.data,
+                                                ctypes.byref(res)))
+        return out, RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def run_gs_device(self, out_ptr: int, phases=8, damping=0.85, tol=1e-4, max_iters=20, teleport_ptr=None,
+                      init_ptr=None, chunk=0, stream=None) -> RunResult:
+        """Block Gauss-Seidel solve with the ranks written to device memory."""
+        p = _params(damping, tol, max_iters, "leak", chunk, False)
+        p.teleport = teleport_ptr
+        p.init = init_ptr
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_gs_run_device_f32(self._h, ctypes.byref(p), int(phases), out_ptr,
+                                                       ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
+    def gs_phases(self, phases=8):
+        """(order uint32[n]: the rows in the order the block Gauss-Seidel solve
+        visits them, caller ids; cuts uint32[phases + 1]: where each phase
+        begins in that order)."""
+        order = np.empty(self.n, np.uint32)
+        cuts = np.empty(int(phases) + 1, np.uint32)
+        check(pagerank_lib().nwgr_pr_gs_phases(self._h, int(phases), order.ctypes.data, cuts.ctypes.data))
+        return order, cuts
 
     def run64(self, damping=0.85, tol=1e-4, max_iters=20, dangling="leak", teleport=None, init=None, chunk=0,
               out=None):

@@ -13,6 +13,9 @@ Reference semantics followed (file:line in NWGraph):
   plus teleport / dangling-mode extensions (LEAK == the reference).
 * ``gs``       -- ``page_rank`` v11 on one thread
   (include/nwgraph/algorithms/page_rank.hpp:89-148).
+* ``bgs``      -- block Gauss-Seidel: rows in a given order cut into phases,
+  Jacobi inside a phase and in place between phases (one phase: ``jacobi``;
+  one row per phase: v11's serial in-place sweep, page_rank.hpp:124-139).
 * ``gs_par`` / ``jacobi_par`` -- v11 / v10 with OpenMP in place of TBB; the
   timed CPU baseline.
 * ``wpr_forward`` / ``wpr_adjoint`` -- weighted PageRank and its adjoint in
@@ -62,6 +65,9 @@ def _lib(fast: bool = False):
     L.orc_pr_jacobi_f64_par.restype = i32
     L.orc_pr_jacobi_f64_par.argtypes = [u32, _u32p, _u32p, _u32p, f64, f64, u32, i32, _vp, _vp, _f64p, _vp,
                                         i32]
+    L.orc_pr_bgs_f64.restype = i32
+    L.orc_pr_bgs_f64.argtypes = [u32, _u32p, _u32p, _u32p, f64, f64, u32, _u32p, u32, _u32p, _vp, _vp, _f64p, _vp,
+                                 i32]
     for fn, real, realp in (("orc_pr_gs_f64", f64, _f64p), ("orc_pr_gs_f32", f32, _f32p)):
         getattr(L, fn).restype = i32
         getattr(L, fn).argtypes = [u32, _u32p, _u32p, _u32p, real, real, u32, realp, _vp]
@@ -115,6 +121,27 @@ def jacobi(offsets, indices, deg, damping=0.85, tol=1e-4, max_iters=20, dangling
     else:
         it = _lib().orc_pr_jacobi_f64(n, offsets, indices, deg, damping, tol, max_iters, dangling,
                                       _ptr(tel), _ptr(ini), r, _ptr(hist))
+    return (r, it, hist[:it]) if return_hist else (r, it)
+
+
+def bgs(offsets, indices, deg, damping=0.85, tol=1e-4, max_iters=20, order=None, cuts=None, teleport=None,
+        init=None, return_hist=False, nthreads=1):
+    """FP64 block Gauss-Seidel (orc_pr_bgs_f64), LEAK dangling: rows visited
+    in `order` (default 0 .. n-1) cut into phases at positions `cuts`
+    (default one phase); returns (r, sweeps[, err_hist]).  nthreads > 1
+    splits each phase's rows over OpenMP threads (the same ranks)."""
+    offsets, indices, deg = _u32(offsets), _u32(indices), _u32(deg)
+    n = len(offsets) - 1
+    order = np.arange(n, dtype=np.uint32) if order is None else _u32(order)
+    cuts = np.array([0, n], np.uint32) if cuts is None else _u32(cuts)
+    if len(order) != n or cuts[0] != 0 or cuts[-1] != n or np.any(np.diff(cuts.astype(np.int64)) < 0):
+        raise ValueError("order must list the n rows and cuts run from 0 to n")
+    r = np.empty(n, np.float64)
+    tel = None if teleport is None else _f64(teleport)
+    ini = None if init is None else _f64(init)
+    hist = np.zeros(max(max_iters, 1), np.float64)
+    it = _lib().orc_pr_bgs_f64(n, offsets, indices, deg, damping, tol, max_iters, order, len(cuts) - 1, cuts,
+                               _ptr(tel), _ptr(ini), r, _ptr(hist), nthreads)
     return (r, it, hist[:it]) if return_hist else (r, it)
 
 

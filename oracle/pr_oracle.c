@@ -29,6 +29,10 @@
  *                       blocked_range(0, N) (page_rank.hpp:124-139); the CPU
  *                       baseline.
  *   orc_pr_jacobi_par_f32   page_rank_v10 with OpenMP (both passes).
+ *   orc_pr_bgs_f64      block Gauss-Seidel: rows in a given order cut into
+ *                       phases, Jacobi inside a phase, in place between
+ *                       phases (between v10 at one phase and serial v11 at
+ *                       one row per phase).
  *   orc_wpr_*           weighted PageRank and its adjoint (no reference
  *                       counterpart: the differentiable form of the same
  *                       iteration, see the functions).
@@ -173,6 +177,64 @@ int orc_pr_jacobi_f64_par(uint32_t n, const uint32_t* offsets, const uint32_t* i
 
 ORC_GS_SERIAL(orc_pr_gs_f64, double)
 ORC_GS_SERIAL(orc_pr_gs_f32, float)
+
+/* Block Gauss-Seidel in double: the rows visited in order[0 .. n) are cut
+ * into `phases` consecutive phases at cuts[0] = 0 < ... <= cuts[phases] = n
+ * (positions in `order`).  Inside a phase every row gathers the
+ * contributions as they stood when the phase began (Jacobi); a phase's new
+ * ranks and contributions replace the old ones before the next phase
+ * gathers (Gauss-Seidel between phases).  phases = 1 is the Jacobi sweep of
+ * orc_pr_jacobi_f64; one row per phase is the serial in-place sweep of
+ * page_rank v11 (page_rank.hpp:124-139) in that row order.  Update, stop rule
+ * and sweep count as orc_pr_jacobi_f64 with LEAK dangling (the reference's:
+ * no dangling mass is redistributed), error = sum over all phases of
+ * |r_new - r_old|. */
+int orc_pr_bgs_f64(uint32_t n, const uint32_t* offsets, const uint32_t* indices, const uint32_t* deg,
+                   double damping, double tol, uint32_t max_iters, const uint32_t* order, uint32_t phases,
+                   const uint32_t* cuts, const double* teleport, const double* init, double* r,
+                   double* err_hist, int nthreads) {
+  if (n == 0) return 0;
+  if (nthreads <= 0) nthreads = 1;
+  double* c = (double*)malloc(sizeof(double) * n);
+  double* rn = (double*)malloc(sizeof(double) * n);
+  const double un = 1.0 / n;
+  for (uint32_t i = 0; i < n; ++i) r[i] = init ? init[i] : un;
+  for (uint32_t i = 0; i < n; ++i) c[i] = r[i] / (double)deg[i]; /* inf for deg 0, never gathered */
+  uint32_t it;
+  for (it = 0; it < max_iters; ++it) {
+    double err = 0.0;
+    for (uint32_t q = 0; q < phases; ++q) {
+      /* the rows of one phase are independent: threads change only the
+         order of the error sum, not the ranks */
+      const int64_t lo = cuts[q], hi = cuts[q + 1];
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) if (hi - lo > 4096)
+      for (int64_t pos = lo; pos < hi; ++pos) {
+        const uint32_t i = order[pos];
+        double z = 0.0;
+        for (uint32_t k = offsets[i]; k < offsets[i + 1]; ++k) z += c[indices[k]];
+        const double t = teleport ? teleport[i] : un;
+        rn[i] = (1.0 - damping) * t + damping * z;
+      }
+      double e = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : e) num_threads(nthreads) if (hi - lo > 4096)
+      for (int64_t pos = lo; pos < hi; ++pos) {
+        const uint32_t i = order[pos];
+        e += fabs(rn[i] - r[i]);
+        r[i] = rn[i];
+        c[i] = r[i] / (double)deg[i];
+      }
+      err += e;
+    }
+    if (err_hist) err_hist[it] = err;
+    if (err < tol) {
+      ++it;
+      break;
+    }
+  }
+  free(c);
+  free(rn);
+  return (int)it;
+}
 
 /* Parallel v11 (the CPU baseline): the same in-place sweep with the vertex
  * range split over OpenMP threads, as tbb::parallel_reduce splits

@@ -329,3 +329,83 @@ def test_delta_float_restatement_close_to_double():
     r32, it32, _ = oracle.delta(g.offsets, g.indices, deg, D, 1e-7, 100, np.float32)
     assert abs(it64 - it32) <= 1
     assert l1(r32.astype(np.float64) / g.n, r64 / g.n) < 1e-6
+
+
+# ---- block Gauss-Seidel (the pin of the device's phased in-place mode)
+
+
+def _rmat_case(scale=10):
+    import nwgr_amd
+    g = nwgr_amd.rmat(scale, 8)
+    return g, nwgr_amd.build_degrees(g)
+
+
+def test_bgs_one_phase_is_jacobi():
+    g, deg = _rmat_case()
+    for tol, its in ((1e-6, 200), (0.0, 7)):
+        r1, k1, h1 = oracle.bgs(g.offsets, g.indices, deg, 0.85, tol, its, return_hist=True)
+        r2, k2, h2 = oracle.jacobi(g.offsets, g.indices, deg, 0.85, tol, its, return_hist=True)
+        assert k1 == k2
+        np.testing.assert_array_equal(r1, r2)
+        np.testing.assert_array_equal(h1, h2)
+
+
+def test_bgs_one_row_per_phase_is_serial_gauss_seidel():
+    """Phases of one row each in the identity order: the serial in-place
+    sweep of v11 (page_rank.hpp:124-139), same sweep count, ranks to double
+    rounding ((1-d) t with t = 1/n against (1-d)/n)."""
+    g, deg = _rmat_case()
+    cuts = np.arange(g.n + 1, dtype=np.uint32)
+    r1, k1, h1 = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-9, 300, cuts=cuts, return_hist=True)
+    r2, k2, h2 = oracle.gs(g.offsets, g.indices, deg, 0.85, 1e-9, 300, return_hist=True)
+    assert k1 == k2
+    np.testing.assert_allclose(r1, r2, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(h1, h2, rtol=1e-9, atol=1e-18)
+
+
+def test_bgs_fixed_point_and_fewer_sweeps_than_jacobi():
+    """Any order and phase cut reaches the LEAK fixed point
+    (I - d P^T) r = (1-d)/n (dense solve, which a dropped damping factor or
+    a wrong gather index would miss); on R-MAT, more phases take no more
+    sweeps to a tolerance, and 16 take fewer than Jacobi."""
+    g, deg = _rmat_case(9)
+    n = g.n
+    P = np.zeros((n, n))
+    for i in range(n):
+        for k in range(g.offsets[i], g.offsets[i + 1]):
+            j = g.indices[k]
+            P[i, j] += 1.0 / deg[j]
+    want = np.linalg.solve(np.eye(n) - 0.85 * P, np.full(n, 0.15 / n))
+    rng = np.random.default_rng(1)
+    order = rng.permutation(n).astype(np.uint32)
+    counts = []
+    for phases in (1, 2, 4, 16):
+        cuts = np.linspace(0, n, phases + 1).round().astype(np.uint32)
+        r, k = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-13, 1000, order=order, cuts=cuts)
+        assert np.abs(r - want).sum() < 1e-11, phases
+        _, k6 = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-6, 1000, order=order, cuts=cuts)
+        counts.append(k6)
+    assert counts[-1] < counts[0] and counts == sorted(counts, reverse=True), counts
+
+
+def test_bgs_teleport_and_init():
+    """A teleport vector and an initial iterate: the fixed point solves
+    (I - d P^T) r = (1-d) t whatever the start."""
+    g, deg = _rmat_case(8)
+    n = g.n
+    rng = np.random.default_rng(4)
+    t = rng.dirichlet(np.ones(n))
+    r0 = rng.dirichlet(np.ones(n))
+    cuts = np.array([0, n // 3, n], np.uint32)
+    r, k = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-14, 2000, cuts=cuts, teleport=t, init=r0)
+    rj, kj = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-14, 2000, teleport=t, init=r0)
+    assert np.abs(r - rj).sum() < 1e-12 and k < kj
+
+
+def test_bgs_threads_give_the_same_ranks():
+    g, deg = _rmat_case(14)
+    cuts = np.array([0, 1000, 5000, g.n], np.uint32)
+    r1, k1 = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-8, 200, cuts=cuts)
+    r8, k8 = oracle.bgs(g.offsets, g.indices, deg, 0.85, 1e-8, 200, cuts=cuts, nthreads=8)
+    assert k1 == k8
+    np.testing.assert_array_equal(r1, r8)
