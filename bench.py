@@ -72,8 +72,8 @@ def parse():
                          "G(n = 2^25, m = 16n); ppr: 256 personalized columns on R-MAT s20; "
                          "multi: 2048 R-MAT graphs (n log-uniform in [1k, 64k], ef 8) in one launch; "
                          "wpr: weighted forward + adjoint on R-MAT s20; delta: residual PageRank "
-                         "(page_rank_v13, threshold 1e-4) on R-MAT s22; shard: one GPU's row share of R-MAT s22 "
-                         "(--shards slices)")
+                         "(page_rank_v13, threshold 1e-4) on R-MAT s22; shard: each GPU's row share of R-MAT s22 "
+                         "swept alone on its own layout (--shards shares)")
     ap.add_argument("--shards", type=int, default=8, help="shard workload: number of row slices")
     ap.add_argument("--fwd-adjoint", type=int, default=1,
                     help="1: also time the weighted forward and forward + adjoint on R-MAT s20 (the metric's "
@@ -415,48 +415,45 @@ def bench_delta(args):
 
 def bench_shard(args):
     """One GPU's share of a row-partitioned solve of R-MAT s22 ef16: the
-    destination rows cut into `--shards` contiguous slices of about equal
-    entry counts, each run as the same n-vertex graph with entries only in its
-    slice (the full gathered vector resident, as each GPU would hold it).
-    Per share (and for the whole graph): the layout autotuned for its shape
-    (with --autotune 1), then device time per sweep over fixed-length solves
-    (tol 0, 20 sweeps, UNIFORM dangling); algorithmic bytes 8 M_k + 16 N_k.  The rows
-    outside the slice have no entries and cost only their contribution
-    store (4 B each), which a real share would not make."""
+    destination rows (in the handle's relabelled order) cut into `--shards`
+    consecutive ranges of about equal sweep bytes, each swept alone on a
+    tiled layout that holds only its entries and stores only its rows
+    (nwgr_pr_share_run_device_f32), the other rows' contributions resident
+    and fixed, as one GPU holds them between the all-gathers of its slice.
+    Per share: device time per sweep over fixed-length solves (tol 0, 20
+    sweeps, LEAK); algorithmic bytes 8 M_k + 16 N_k.  Beside it the whole
+    graph's sweep on the same handle."""
     import torch
     import nwgr_amd
     from nwgr_amd import PageRankGraph
     G = args.shards
     g = nwgr_amd.rmat(args.scale, args.edge_factor, seed=args.seed)
     deg = nwgr_amd.build_degrees(g)
-    cuts = nwgr_amd.row_cuts(g, G)
     out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
-    kw = dict(damping=0.85, tol=0.0, max_iters=20, dangling="uniform")
+    kw = dict(damping=0.85, tol=0.0, max_iters=20)
+    pg = PageRankGraph(g, deg)
 
-    def per_sweep_us(graph):
-        with PageRankGraph(graph, deg) as pg:
-            if args.autotune:
-                from nwgr_amd.autotune import autotune
-                autotune(pg, sweeps=8, reps=1, rounds=1)
-            for _ in range(max(args.warmup, 1)):
-                pg.run_device(out.data_ptr(), **kw)
-            ms, sw = 0.0, 0
-            for _ in range(args.steps):
-                res = pg.run_device(out.data_ptr(), **kw)
-                ms += pg.last_profile()["sweep_ms"]
-                sw += res.iters
-            return ms / sw * 1e3, pg.last_engine()
+    def per_sweep_us(run):
+        for _ in range(max(args.warmup, 1)):
+            run()
+        ms, sw = 0.0, 0
+        for _ in range(args.steps):
+            res = run()
+            ms += pg.last_profile()["sweep_ms"]
+            sw += res.iters
+        return ms / sw * 1e3
 
-    full_us, engine = per_sweep_us(g)
-    log(f"whole graph: {full_us:.1f} us per sweep ({engine})")
+    full_us = per_sweep_us(lambda: pg.run_device(out.data_ptr(), dangling="leak", **kw))
+    log(f"whole graph: {full_us:.1f} us per sweep ({pg.last_engine()})")
+    order, cuts = pg.gs_phases(G)
+    indeg = np.diff(g.offsets.astype(np.int64))[order]
     shares = []
-    offs = g.offsets.astype(np.int64)
     for k in range(G):
         lo, hi = int(cuts[k]), int(cuts[k + 1])
-        us, eng = per_sweep_us(nwgr_amd.row_shard(g, lo, hi))
-        m_k, n_k = int(offs[hi] - offs[lo]), hi - lo
+        us = per_sweep_us(lambda: pg.run_share_device(out.data_ptr(), G, k, **kw))
+        m_k, n_k = int(indeg[lo:hi].sum()), hi - lo
         b_k = 8 * m_k + 16 * n_k
-        shares.append({"rows": [lo, hi], "m": m_k, "us_per_sweep": round(us, 2), "engine": eng,
+        shares.append({"rows": [lo, hi], "m": m_k, "us_per_sweep": round(us, 2),
                        "gbs": round(b_k / (us / 1e6) / 1e9, 1)})
         log(f"share {k}: rows [{lo}, {hi}) m={m_k} {us:.1f} us per sweep")
     worst = max(shares, key=lambda s: s["us_per_sweep"])
@@ -466,15 +463,15 @@ def bench_shard(args):
         "value": worst["us_per_sweep"], "unit": "us", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"rmat s{args.scale} ef{args.edge_factor} rows cut into {G} slices of equal entry "
-                               f"counts, each share run alone with the full gathered vector; 20 sweeps per solve",
-                   "n": g.n, "m": g.m, "whole_graph_us_per_sweep": round(full_us, 2), "shares": shares,
+        "config": {"workload": f"rmat s{args.scale} ef{args.edge_factor}, relabelled rows cut into {G} ranges of "
+                               f"equal sweep bytes, each swept alone on its own layout (only its entries, only its "
+                               f"rows stored), the other contributions resident; 20 sweeps per solve, LEAK",
+                   "n": g.n, "m": g.m, "whole_graph_us_per_sweep": round(full_us, 2),
+                   "ideal_us_per_sweep": round(full_us / G, 2), "shares": shares,
                    "sum_of_shares_us": round(sum(s["us_per_sweep"] for s in shares), 2)},
         "roofline": {"bound": "hbm", "achieved": worst["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(worst["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": {"tiled": "k_tile+k_combine", "tiled-accumulate": "k_tile+k_accum"}.get(
-                         worst["engine"], "k_sweep"),
-                     "algorithmic_bytes_per_sweep": b_w},
+                     "kernel": "k_tile+k_combine (one share's layout)", "algorithmic_bytes_per_sweep": b_w},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -682,6 +679,24 @@ def main():
         torch.cuda.synchronize()
         leak_ms = (time.perf_counter() - t1) * 1e3 / args.steps
         line["gpu_leak_solve"] = {"ms_per_solve": round(leak_ms, 4), "sweeps_per_solve": res_l.iters}
+        # the in-place (block Gauss-Seidel) form of the same LEAK solve: fewer
+        # sweeps, two launches per phase and sweep
+        gs = {}
+        for phases in (2, 4, 8):
+            res_g = pg.run_gs_device(out_d.data_ptr(), phases, damping=0.85, tol=args.tol, max_iters=args.max_iters)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                res_g = pg.run_gs_device(out_d.data_ptr(), phases, damping=0.85, tol=args.tol,
+                                         max_iters=args.max_iters)
+            torch.cuda.synchronize()
+            gs_ms = (time.perf_counter() - t1) * 1e3 / args.steps
+            gs[str(phases)] = {"ms_per_solve": round(gs_ms, 4), "sweeps_per_solve": res_g.iters,
+                               "us_per_sweep": round(gs_ms * 1e3 / res_g.iters, 2)}
+        line["gpu_leak_solve_block_gauss_seidel"] = {
+            "phases": gs, "cpu_sweeps_per_solve": cpu["sweeps_per_solve"],
+            "note": "rows cut into phases updated in place one after the other (nwgr_pr_gs_run_device_f32); "
+                    "time to tolerance beside the Jacobi LEAK solve (gpu_leak_solve)"}
         line["gpu_vs_cpu_per_sweep"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
         line["gpu_vs_cpu_time_to_tolerance"] = round(cpu["ms_per_solve"] / leak_ms, 1)
     if args.workload == "pagerank":

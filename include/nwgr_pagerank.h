@@ -172,7 +172,8 @@ int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r
  * page_rank (v11, page_rank.hpp:124-139: each row's new rank and
  * contribution replace the old ones within the sweep) made deterministic.
  * The rows, in the handle's relabelled order (nwgr_pr_gs_phases), are cut
- * into `phases` consecutive phases of about equal entry counts; inside a
+ * into `phases` consecutive phases of about equal sweep bytes (8 per entry,
+ * 16 per row with entries, 8 per row without); inside a
  * phase every row gathers the contributions as they stood when the phase
  * began, and a phase's results replace the old ones before the next phase
  * gathers.  phases = 1 is the Jacobi solve of nwgr_pr_run_f32; more phases
@@ -185,6 +186,17 @@ int nwgr_pr_gs_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phase
                        nwgr_pr_result* res);
 int nwgr_pr_gs_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phases, float* d_r_out,
                               nwgr_pr_result* res, void* stream);
+/* One GPU's share of a row-partitioned solve (no reference counterpart: the
+ * single-GPU view of the multi-GPU design).  The rows are cut into `shares`
+ * consecutive ranges as the phases of the block Gauss-Seidel solve; the
+ * sweeps update the rows of share `share` alone, on a tiled layout that
+ * holds only that share's entries and stores only its rows, against the
+ * contributions of the other rows held at their initial values -- what one
+ * GPU computes between the all-gathers of its slice.  Stop rule on the
+ * share's own L1 change; d_r_out receives all n ranks (the other rows at
+ * their initial values).  LEAK dangling; shares in [1, 64]. */
+int nwgr_pr_share_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t shares, uint32_t share,
+                                 float* d_r_out, nwgr_pr_result* res, void* stream);
 /* The sweep order of the block Gauss-Seidel solve: order[i] (n entries,
  * caller ids; may be NULL) is the i-th row visited -- in-degree
  * non-increasing, ties by ascending id -- and cuts[0 .. phases] (may be

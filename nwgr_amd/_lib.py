@@ -118,6 +118,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_gs_run_f32.argtypes = [vp, P(PrParams), u32, vp, P(PrResult)]
     L.nwgr_pr_gs_run_device_f32.restype = i32
     L.nwgr_pr_gs_run_device_f32.argtypes = [vp, P(PrParams), u32, vp, P(PrResult), vp]
+    L.nwgr_pr_share_run_device_f32.restype = i32
+    L.nwgr_pr_share_run_device_f32.argtypes = [vp, P(PrParams), u32, u32, vp, P(PrResult), vp]
     L.nwgr_pr_gs_phases.restype = i32
     L.nwgr_pr_gs_phases.argtypes = [vp, u32, vp, vp]
     L.nwgr_pr_run_f64.restype = i32

@@ -334,6 +334,7 @@ struct SolveSpec {
   // block Gauss-Seidel: one tiled layout per phase (rows of consecutive
   // ranges), swept in order over one contribution vector updated in place
   const std::vector<Tiled>* phases = nullptr;
+  int only = -1;                   // >= 0: sweep that one phase alone (one GPU's row share)
 };
 
 // Device state of the batched personalized solve (ppr.hip).
@@ -512,8 +513,10 @@ inline size_t iter_len(uint32_t n) {
 // g.r[res.iters & 1] (new labels).
 void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
-// block Gauss-Seidel solve with `phases` phases (LEAK dangling)
-void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s);
+// block Gauss-Seidel solve with `phases` phases (LEAK dangling); with
+// only >= 0, the sweeps update that phase's rows alone (a row share)
+void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s,
+            int only = -1);
 // FP64 Jacobi solve (fp64.hip); d_out: n doubles in caller order
 void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_result& res, hipStream_t s);
 // residual ("delta") solve, page_rank_v13; p.tol is the push threshold and

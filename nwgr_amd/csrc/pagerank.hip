@@ -93,6 +93,7 @@ struct InitArgs {
   const float* __restrict__ gsrc;      // adjoint: lambda_0 (new labels)
   float* __restrict__ tel;             // permuted copy (when tel_src)
   float* __restrict__ r0;
+  float* __restrict__ r1;              // or null: the other iterate buffer gets the initial ranks too
   float* __restrict__ c0;
   double* __restrict__ part0;
   State* st;
@@ -356,6 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       const float r = a.init ? a.init[src] : a.un;
       const float dg = a.div[i];
       a.r0[i] = r;
+      if (a.r1) a.r1[i] = r;
       a.c0[i] = r / dg;
       if (dg == 0.f) dang += static_cast<double>(r);
     }
@@ -1547,6 +1549,10 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
 // block, and the next sweep's first tile pass adds all of them.
 void launch_gs(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
   const uint32_t P = static_cast<uint32_t>(sp.phases->size());
+  if (sp.only >= 0) {  // one row share alone: its pairs are the sweep's
+    launch_layout(g, sp.kind, (*sp.phases)[sp.only], nullptr, sa, 1u, g.grid_c, s);
+    return;
+  }
   for (uint32_t q = 0; q < P; ++q) {
     SweepArgs aq = sa;
     aq.part_out = sa.part_out + 2 * static_cast<size_t>(g.grid_c) * q;
@@ -1688,6 +1694,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
                           static_cast<const void*>(g.gs_part.p)})
     key = fnv_mix(key, reinterpret_cast<uintptr_t>(ptr));
   key = fnv_mix(key, sp.phases ? sp.phases->size() : 0);
+  key = fnv_mix(key, static_cast<uint64_t>(sp.only + 1));
   const uint32_t slot = sp.phases ? kExecGs : sp.kind;
   if (g.exec_key[slot] != key) {
     g.drop_graphs(static_cast<int>(slot));
@@ -1708,6 +1715,8 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.r0 = g.r[0].as<float>();
   ia.c0 = g.c[0].as<float>();
   ia.part0 = sp.phases ? g.gs_part.as<double>() : g.partials.as<double>();
+  // a row share leaves the other rows as they start, in both iterates
+  ia.r1 = sp.only >= 0 ? g.r[1].as<float>() : nullptr;
   ia.st = g.state.as<State>();
   ia.n = g.n;
   ia.adjoint = sp.kind == kAdjoint;
@@ -1893,8 +1902,8 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   res.ms += ms;
 }
 
-void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res,
-            hipStream_t s) {
+void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s,
+            int only) {
   validate_params(p);
   NWGR_REQUIRE(p.dangling == kDanglingLeak,
                "block Gauss-Seidel takes LEAK dangling (the reference's in-place sweep redistributes none)");
@@ -1910,9 +1919,12 @@ void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nw
   sp.div = g.degf.as<float>();
   sp.init = p.init;
   sp.tel_src = p.teleport;
+  NWGR_REQUIRE(only < static_cast<int>(phases), "share index out of range");
   sp.phases = &g.gs;
-  sp.tiled = &g.gs[0];
-  sp.quiet = !p.teleport && !p.init && g.nz < g.n;
+  sp.only = only;
+  sp.tiled = &g.gs[only >= 0 ? only : 0];
+  // quiet rows are kept as the scalar only where the last phase refreshes them
+  sp.quiet = !p.teleport && !p.init && g.nz < g.n && (only < 0 || only + 1 == static_cast<int>(phases));
   if (sp.quiet) prepare_quiet(g, s);
   // blocks of pairs the first sweep's first phase adds beside init's block 0
   NWGR_HIP(hipMemsetAsync(g.gs_part.p, 0, g.gs_part.bytes, s));
@@ -2182,6 +2194,21 @@ int nwgr_pr_gs_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phase
     nwgr::run_gs(*g, q, phases, out, *res, s);
     NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nwgr_pr_share_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t shares, uint32_t share,
+                                 float* d_r_out, nwgr_pr_result* res, void* stream) {
+  return guarded([&] {
+    NWGR_REQUIRE(g && p && res, "null argument");
+    nwgr::validate_params(*p);
+    NWGR_REQUIRE(shares >= 1 && shares <= nwgr::kMaxGsPhases, "shares must be in [1, 64]");
+    NWGR_REQUIRE(share < shares, "share must be < shares");
+    if (g->n == 0) return nwgr::empty_result(*p, *res);
+    NWGR_REQUIRE(d_r_out != nullptr, "null argument");
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
+    nwgr::run_gs(*g, *p, shares, d_r_out, *res, s, static_cast<int>(share));
   });
 }
 

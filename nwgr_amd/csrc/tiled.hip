@@ -566,13 +566,27 @@ __global__ void k_sub_offsets(const uint32_t* __restrict__ offs, uint32_t n, uin
 }  // namespace
 
 std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
-  // phase q starts at the first row whose entries begin at or after q m / P
+  // Ranges of about equal sweep bytes: 8 per entry (index, gathered
+  // contribution), 16 per row with entries (offset, divisor, old and new
+  // rank), 8 per row without (divisor and contribution; its rank is the
+  // quiet scalar).  Bytes of rows [0, i), a non-decreasing function of i;
+  // phase q starts at the first row at which it reaches q / P of the total.
   std::vector<uint32_t> cuts(phases + 1, 0);
   const auto& oh = g.offs_h;
+  const uint32_t nz = g.nz;
+  auto bytes_before = [&](uint32_t i) -> uint64_t {
+    return 8 * static_cast<uint64_t>(oh[i]) + 16 * static_cast<uint64_t>(std::min(i, nz)) +
+           8 * static_cast<uint64_t>(i > nz ? i - nz : 0);
+  };
+  const uint64_t total = bytes_before(g.n);
   for (uint32_t q = 1; q < phases; ++q) {
-    const uint64_t target = (static_cast<uint64_t>(q) * g.m + phases - 1) / phases;
-    const uint32_t pos = static_cast<uint32_t>(std::lower_bound(oh.begin(), oh.end(), target) - oh.begin());
-    cuts[q] = std::clamp<uint32_t>(pos, cuts[q - 1], g.n);
+    const uint64_t target = (static_cast<uint64_t>(q) * total + phases - 1) / phases;
+    uint32_t lo = cuts[q - 1], hi = g.n;  // first position with bytes_before >= target
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (bytes_before(mid) < target) lo = mid + 1; else hi = mid;
+    }
+    cuts[q] = lo;
   }
   cuts[phases] = g.n;
   return cuts;

@@ -232,6 +232,17 @@ class PageRankGraph:
                                                        ctypes.byref(res), stream))
         return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
+    def run_share_device(self, out_ptr: int, shares: int, share: int, damping=0.85, tol=1e-4, max_iters=20,
+                         chunk=0, stream=None) -> RunResult:
+        """One GPU's row share swept alone (nwgr_pr_share_run_device_f32):
+        rows of share `share` of `shares` updated against the others'
+        contributions held at their initial values; LEAK dangling."""
+        p = _params(damping, tol, max_iters, "leak", chunk, False)
+        res = PrResult()
+        check(pagerank_lib().nwgr_pr_share_run_device_f32(self._h, ctypes.byref(p), int(shares), int(share), out_ptr,
+                                                          ctypes.byref(res), stream))
+        return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
+
     def gs_phases(self, phases=8):
         """(order uint32[n]: the rows in the order the block Gauss-Seidel solve
         visits them, caller ids; cuts uint32[phases + 1]: where each phase

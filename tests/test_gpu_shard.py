@@ -44,3 +44,76 @@ def test_row_shards_compose_one_sweep(ename, parts):
         got[lo:hi] = r[lo:hi]
     assert l1(got, r64) <= 1e-6
     assert l1(got, r_full) <= 1e-6
+
+
+def share_reference(g, deg, rows, tol, max_iters):
+    """FP64 sweeps of the rows `rows` alone (LEAK), the other rows' ranks held
+    at 1/n: r_S' = (1-d)/n + d (A_S c), c = r / deg, stop on the L1 change of
+    r_S.  Returns (r, sweeps)."""
+    import scipy.sparse as sp
+    n = g.n
+    A = sp.csr_matrix((np.ones(g.m), g.indices, g.offsets.astype(np.int64)), shape=(n, n))
+    As = A[rows]
+    r = np.full(n, 1.0 / n)
+    with np.errstate(divide="ignore"):
+        c = r / deg.astype(np.float64)
+    c[~np.isfinite(c)] = 0.0  # never gathered when the divisors are out-degrees
+    for it in range(1, max_iters + 1):
+        rn = 0.15 / n + 0.85 * (As @ c)
+        err = np.abs(rn - r[rows]).sum()
+        r[rows] = rn
+        with np.errstate(divide="ignore"):
+            c[rows] = rn / deg[rows]
+        c[~np.isfinite(c)] = 0.0
+        if err < tol:
+            return r, it
+    return r, max_iters
+
+
+@pytest.mark.parametrize("shares", [3, 8])
+def test_each_share_alone_on_its_own_layout(shares):
+    """nwgr_pr_share_run_device_f32: a share's rows swept on the layout that
+    holds only its entries, against the FP64 restatement of that sweep (which
+    the dense solve of the share's fixed point pins at s10)."""
+    import torch
+    g = nwgr_amd.rmat(14, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    order, cuts = pg.gs_phases(shares)
+    # the shares are the row partition a multi-GPU solve takes: about equal
+    # sweep bytes (8 per entry, 16 per row with entries, 8 per row without)
+    counts = np.diff(g.offsets.astype(np.int64))[order]
+    cost = 8 * counts + np.where(counts > 0, 16, 8)
+    per = np.add.reduceat(cost, cuts[:-1].astype(np.int64))
+    assert per.max() <= cost.sum() / shares + cost.max(), (per, cost.sum() / shares)
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    for q in range(shares):
+        rows = order[cuts[q]:cuts[q + 1]].astype(np.int64)
+        res = pg.run_share_device(out.data_ptr(), shares, q, tol=1e-6, max_iters=200)
+        r = out.cpu().numpy()
+        r_ref, k_ref = share_reference(g, deg, rows, 1e-6, 200)
+        r_m, _ = share_reference(g, deg, rows, 0.0, res.iters)
+        assert abs(res.iters - k_ref) <= 1, (q, res.iters, k_ref)
+        assert l1(r[rows], r_m[rows]) <= 1e-6, q
+        others = np.setdiff1d(np.arange(g.n), rows)
+        np.testing.assert_allclose(r[others], 1.0 / g.n, rtol=1e-6)
+
+
+def test_share_reference_fixed_point_dense():
+    """The restatement above converges to the dense solution of the share's
+    fixed point (I - d A_SS D_S^-1) r_S = (1-d)/n + d A_{S,rest} c_rest."""
+    g = nwgr_amd.rmat(10, 8)
+    deg = nwgr_amd.build_degrees(g).astype(np.float64)
+    n = g.n
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(g.offsets[i], g.offsets[i + 1]):
+            A[i, g.indices[k]] += 1.0
+    rows = np.arange(100, 700)
+    rest = np.setdiff1d(np.arange(n), rows)
+    dinv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    M = np.eye(len(rows)) - 0.85 * A[np.ix_(rows, rows)] * dinv[rows]
+    rhs = 0.15 / n + 0.85 * A[np.ix_(rows, rest)] @ (dinv[rest] / n)
+    want = np.linalg.solve(M, rhs)
+    r, _ = share_reference(g, deg.astype(np.uint32), rows, 1e-15, 5000)
+    assert np.abs(r[rows] - want).sum() < 1e-12
