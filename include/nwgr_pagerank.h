@@ -108,7 +108,11 @@ int nwgr_pr_params_check(const nwgr_pr_params* p, uint32_t n);
  * needs no device; every solve on it returns NWGR_OK at once, as the
  * reference returns for N = 0 (page_rank.hpp:92-147): result.iters = 1
  * (max_iters when tol <= 0, 0 when max_iters = 0), error 0.
- * n and m must be < 2^31 (the device sorts count in int). */
+ * n < 2^31 (the sorts over vertices count in int) and m <= 2^32 - 2^28: the
+ * reference's uint32 offsets (util/defaults.hpp:23-24) with room for the
+ * tiled layout's padding in 32-bit ordinals.  Counts above one sort pass
+ * (2^30 items, nwgr_pr_set_sort_batch) are sorted and scanned in passes.
+ * Weighted solves (nwgr_wpr_*) need m < 2^31. */
 int nwgr_pr_graph_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
                          const uint32_t* degrees, int device, nwgr_pr_graph** out);
 
@@ -117,6 +121,13 @@ int nwgr_pr_graph_create(const uint32_t* offsets, const uint32_t* indices, uint3
 int nwgr_pr_graph_create_device(const uint32_t* d_offsets, const uint32_t* d_indices, uint32_t n,
                                 uint64_t m, const uint32_t* d_degrees, int device, void* stream,
                                 nwgr_pr_graph** out);
+
+/* Items per device sort or scan call when a graph is built (default 2^30,
+ * the range [2^16, 2^30]); larger counts go in passes of at most this many,
+ * with results identical to a single call.  Process-wide; a small value makes
+ * the passes testable on small graphs.  No reference counterpart (the
+ * reference sorts on the host, build.hpp:209-265). */
+int nwgr_pr_set_sort_batch(uint64_t items);
 
 /* Destroys a handle; a handle of the resident-graph cache (below) is also
  * dropped from the cache. */

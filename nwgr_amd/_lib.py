@@ -88,6 +88,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_graph_create_device.argtypes = [vp, vp, u32, u64, vp, i32, vp, P(vp)]
     L.nwgr_pr_graph_destroy.restype = None
     L.nwgr_pr_graph_destroy.argtypes = [vp]
+    L.nwgr_pr_set_sort_batch.restype = i32
+    L.nwgr_pr_set_sort_batch.argtypes = [u64]
     L.nwgr_pr_graph_cached.restype = i32
     L.nwgr_pr_graph_cached.argtypes = [vp, vp, u32, u64, vp, i32, P(vp), P(u32)]
     L.nwgr_pr_cache_clear.restype = None

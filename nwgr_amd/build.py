@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["graph.hip", "pagerank.hip", "weighted.hip", "tiled.hip", "ppr.hip", "multi.hip", "csr.hip",
-               "topk.hip", "cache.hip", "fp64.hip"]
+               "topk.hip", "cache.hip", "fp64.hip", "batched.hip"]
 
 PAGERANK_LIB = os.path.join(PKG, "libnwgr_pagerank.so")
 GRAPHGEN_LIB = os.path.join(PKG, "libnwgr_graphgen.so")

@@ -80,6 +80,45 @@ __global__ void k_edge_keys(const uint32_t* __restrict__ offs, const uint32_t* _
   }
 }
 
+// Passes form of the edge order (more entries than one sort pass holds):
+// renamed source of every entry, counted per source.
+__global__ void k_new_src(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ iperm, uint64_t m,
+                          uint32_t* __restrict__ src, uint32_t* __restrict__ cnt) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < m; k += stride) {
+    const uint32_t u = iperm[idx[k]];
+    src[k] = u;
+    atomicAdd(&cnt[u], 1u);
+  }
+}
+
+// New row of the entry that sits at position j of the source order.
+__global__ void k_row_keys(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ iperm,
+                           const uint32_t* __restrict__ by_src, uint32_t n, uint64_t m, uint32_t* __restrict__ key) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < m; j += stride) {
+    const uint32_t k = by_src[j];
+    uint32_t lo = 0, hi = n;  // offs[lo] <= k < offs[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (offs[mid] <= k) lo = mid; else hi = mid;
+    }
+    key[j] = iperm[lo];
+  }
+}
+
+// Position p of the new CSR takes entry by_src[by_row[p]].
+__global__ void k_place_entries(const uint32_t* __restrict__ by_src, const uint32_t* __restrict__ by_row,
+                                const uint32_t* __restrict__ src, uint64_t m, uint32_t* __restrict__ idx,
+                                uint32_t* __restrict__ epos) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < m; p += stride) {
+    const uint32_t k = by_src[by_row[p]];
+    epos[p] = k;
+    idx[p] = src[k];
+  }
+}
+
 __global__ void k_split_keys(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* idx) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < m; k += stride)
@@ -245,6 +284,8 @@ void ensure_transpose(Graph& g, hipStream_t stream) {
   if (g.has_t) return;
   const uint32_t n = g.n;
   const uint64_t m = g.m;
+  // one radix sort of (source, row) keys over every entry, counted in int
+  NWGR_REQUIRE(m < 0x80000000ULL, "the out-CSR (weighted solves) needs m < 2^31");
   const size_t nb = sizeof(uint32_t) * n, mb = sizeof(uint32_t) * m;
   DevBuf outdeg, ids, outdeg_sorted, row_list;
   outdeg.alloc(nb);
@@ -371,9 +412,9 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
     g->device = device;
     return g.release();
   }
-  NWGR_REQUIRE(m <= 0xffffffffULL, "m must fit the uint32 offsets (m < 2^32)");
-  // the device radix sorts count items in int
-  NWGR_REQUIRE(n < 0x80000000u && m < 0x80000000ULL, "n and m must be < 2^31");
+  NWGR_REQUIRE(m <= kMaxGraphEntries, "m must be <= 2^32 - 2^28 (uint32 offsets with room for the tile padding)");
+  // the sorts over vertices count items in int
+  NWGR_REQUIRE(n < 0x80000000u, "n must be < 2^31");
   NWGR_REQUIRE(offsets != nullptr, "offsets is null");
   NWGR_REQUIRE(m == 0 || indices != nullptr, "indices is null");
   int ndev = 0;
@@ -483,7 +524,38 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   g->idx.alloc(mb + sizeof(uint32_t) * kEdgePad);
   NWGR_HIP(hipMemsetAsync(g->idx.p, 0, g->idx.bytes, stream));
   g->epos.alloc(mb);
-  if (m) {
+  if (m > sort_batch()) {
+    // More entries than one sort pass: the (row, source) order as two
+    // stable passes sorts, by source and then by row, the entries in input
+    // order (so equal pairs keep it, as in the one-pass sort below).
+    DevBuf src, cnt, first, by_src, key, by_row;
+    src.alloc(mb);
+    cnt.alloc(nb);
+    first.alloc(nb);
+    NWGR_HIP(hipMemsetAsync(cnt.p, 0, nb, stream));
+    k_new_src<<<grid_for(m), kT, 0, stream>>>(d_idx, g->iperm.as<uint32_t>(), m, src.as<uint32_t>(),
+                                              cnt.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    exclusive_sum_u32(cnt.as<uint32_t>(), first.as<uint32_t>(), n, stream);
+    const int end_bit = static_cast<int>(bits_for(n));
+    by_src.alloc(mb);
+    stable_sort_place(src.as<uint32_t>(), m, end_bit, n, first.as<uint32_t>(), nullptr, by_src.as<uint32_t>(),
+                      stream);
+    cnt.reset();
+    first.reset();
+    key.alloc(mb);
+    k_row_keys<<<grid_for(m), kT, 0, stream>>>(d_offs, g->iperm.as<uint32_t>(), by_src.as<uint32_t>(), n, m,
+                                               key.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    by_row.alloc(mb);
+    stable_sort_place(key.as<uint32_t>(), m, end_bit, n, g->offs.as<uint32_t>(), nullptr, by_row.as<uint32_t>(),
+                      stream);
+    key.reset();
+    k_place_entries<<<grid_for(m), kT, 0, stream>>>(by_src.as<uint32_t>(), by_row.as<uint32_t>(), src.as<uint32_t>(),
+                                                    m, g->idx.as<uint32_t>(), g->epos.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    NWGR_HIP(hipStreamSynchronize(stream));
+  } else if (m) {
     DevBuf ek_in, ek_out, ev_in;
     ek_in.alloc(sizeof(uint64_t) * m);
     ek_out.alloc(sizeof(uint64_t) * m);

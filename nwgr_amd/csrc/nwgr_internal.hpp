@@ -448,8 +448,17 @@ struct Graph {
   ~Graph();
 };
 
-inline uint64_t bytes_of(const Graph& g) {
+inline uint64_t bytes_of(const Tiled& t) {
   uint64_t b = 0;
+  for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
+                          &t.wg_task, &t.wave_rows, &t.ablo, &t.aseg, &t.afrow})
+    b += d->bytes;
+  return b;
+}
+
+inline uint64_t bytes_of(const Graph& g) {
+  uint64_t b = bytes_of(g.tfwd) + bytes_of(g.tadj) + g.gs_part.bytes;
+  for (const Tiled& t : g.gs) b += bytes_of(t);
   for (const DevBuf* d : {&g.perm, &g.iperm, &g.offs, &g.idx, &g.epos, &g.degf, &g.fwd.hub_units,
                           &g.fwd.hub_part, &g.fwd.hub_ticket, &g.fwd.rows, &g.t_offs, &g.t_idx, &g.t_pos,
                           &g.adj.hub_units, &g.adj.hub_part, &g.adj.hub_ticket, &g.adj.rows, &g.w_in,
@@ -459,6 +468,23 @@ inline uint64_t bytes_of(const Graph& g) {
     b += d->bytes;
   return b;
 }
+
+// Items per hipCUB sort / scan call (batched.hip); larger counts are cut into
+// passes of at most this many.
+uint64_t sort_batch();
+void set_sort_batch(uint64_t items);
+// Stable sort of keys[0, count) (each < nbuckets, below 2^end_bit) as a
+// placement: out_val[pos] = original index and out_key[pos] = key (out_key may
+// be null when count exceeds one pass), with bucket_start[key] the first
+// position of the key's run (the exclusive prefix of the key counts).
+void stable_sort_place(const uint32_t* keys, uint64_t count, int end_bit, uint32_t nbuckets,
+                       const uint32_t* bucket_start, uint32_t* out_key, uint32_t* out_val, hipStream_t s);
+// out = exclusive prefix sum of in[0, count) (u32), in passes
+void exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t count, hipStream_t s);
+// Most entries a graph may hold: below 2^32 with room for the tiled layout's
+// padding (at most 1023 entries per source tile, and at most 2^18 tiles of
+// 2^13 sources for n < 2^31) in 32-bit ordinals.
+constexpr uint64_t kMaxGraphEntries = (uint64_t{1} << 32) - (uint64_t{1} << 28);
 
 // resident-graph cache (cache.hip): the handle of a graph the caller passed
 // before (same arrays and contents), or a new one the cache then owns

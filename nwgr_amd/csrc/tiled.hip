@@ -42,7 +42,7 @@ uint32_t bits_for(uint64_t v) {
 __global__ __launch_bounds__(kT) void k_rows_and_tiles(const uint32_t* __restrict__ offs,
                                                        const uint32_t* __restrict__ idx, uint32_t n, uint64_t m,
                                                        uint32_t bits, uint32_t ntiles, bool lds_hist, uint32_t* erow,
-                                                       uint32_t* tkey, uint32_t* tval, uint32_t* tile_cnt) {
+                                                       uint32_t* tkey, uint32_t* tile_cnt) {
   extern __shared__ uint32_t s_hist[];
   if (lds_hist) {
     for (uint32_t i = threadIdx.x; i < ntiles; i += kT) s_hist[i] = 0;
@@ -57,7 +57,6 @@ __global__ __launch_bounds__(kT) void k_rows_and_tiles(const uint32_t* __restric
     erow[k] = lo;
     const uint32_t t = idx[k] >> bits;
     tkey[k] = t;
-    tval[k] = static_cast<uint32_t>(k);
     if (lds_hist)
       atomicAdd(&s_hist[t], 1u);
     else
@@ -127,10 +126,6 @@ __global__ void k_subpairs(const uint32_t* __restrict__ flagv, const uint32_t* _
 __global__ void k_lane_sp0(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ flagv, uint64_t nlanes,
                            uint32_t* lane_sp0) {
   GRID_STRIDE(l, nlanes) lane_sp0[l] = ord[l * kLaneEdges] + flagv[l * kLaneEdges] - 1u;
-}
-
-__global__ void k_iota(uint32_t* v, uint64_t count) {
-  GRID_STRIDE(i, count) v[i] = static_cast<uint32_t>(i);
 }
 
 // (row, tile) runs of a CSR with rows sorted by source: one wavefront per row,
@@ -342,10 +337,9 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   t.row_hi = row_hi;
   t.ntiles = static_cast<uint32_t>((static_cast<uint64_t>(n) + (1u << bits) - 1) >> bits);
   const uint32_t ntiles = t.ntiles;
-  DevBuf erow, tkey, tval, tile_cnt, tk_sorted, tperm;
+  DevBuf erow, tkey, tile_cnt, tk_sorted, tperm;
   erow.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
   tkey.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
-  tval.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
   tile_cnt.alloc(sizeof(uint32_t) * ntiles);
   NWGR_HIP(hipMemsetAsync(tile_cnt.p, 0, tile_cnt.bytes, s));
   if (m) {
@@ -354,7 +348,7 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
     const bool lds_hist = ntiles <= 8192;
     const uint32_t grid = lds_hist ? std::min<uint32_t>(grid_for(m), g.num_cu * 4) : grid_for(m);
     k_rows_and_tiles<<<grid, kT, lds_hist ? sizeof(uint32_t) * ntiles : 0, s>>>(
-        offs, idx, n, m, bits, ntiles, lds_hist, erow.as<uint32_t>(), tkey.as<uint32_t>(), tval.as<uint32_t>(),
+        offs, idx, n, m, bits, ntiles, lds_hist, erow.as<uint32_t>(), tkey.as<uint32_t>(),
         tile_cnt.as<uint32_t>());
     NWGR_HIP(hipGetLastError());
   }
@@ -378,16 +372,9 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   // stable sort of the entries by tile keeps (row, source) order inside a tile
   tk_sorted.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
   tperm.alloc(sizeof(uint32_t) * std::max<uint64_t>(m, 1));
-  if (m) {
-    const int end_bit = static_cast<int>(bits_for(ntiles));
-    cub_call([&](void* tmp, size_t& bytes) {
-      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, tkey.as<uint32_t>(), tk_sorted.as<uint32_t>(),
-                                                tval.as<uint32_t>(), tperm.as<uint32_t>(), static_cast<int>(m), 0,
-                                                end_bit, s);
-    }, s);
-  }
+  stable_sort_place(tkey.as<uint32_t>(), m, static_cast<int>(bits_for(ntiles)), ntiles, d_start.as<uint32_t>(),
+                    tk_sorted.as<uint32_t>(), tperm.as<uint32_t>(), s);
   tkey.reset();
-  tval.reset();
 
   t.eloc.alloc(sizeof(uint16_t) * std::max<uint64_t>(mp, 1));
   t.epos.alloc(sizeof(uint32_t) * std::max<uint64_t>(mp, 1));
@@ -413,12 +400,7 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   tperm.reset();
 
   // fragment ordinals
-  if (mp) {
-    cub_call([&](void* tmp, size_t& bytes) {
-      return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flagv.as<uint32_t>(), ord.as<uint32_t>(),
-                                              static_cast<int>(mp), s);
-    }, s);
-  }
+  exclusive_sum_u32(flagv.as<uint32_t>(), ord.as<uint32_t>(), mp, s);
   uint32_t last[2] = {0, 0};
   if (mp) {
     NWGR_HIP(hipMemcpyAsync(&last[0], ord.as<uint32_t>() + mp - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -454,19 +436,9 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
                                             static_cast<int>(n) + 1, s);
   }, s);
   t.rslot.alloc(sizeof(uint32_t) * std::max<uint32_t>(nsub, 1));
-  if (nsub) {
-    DevBuf ids, rows_sorted;
-    ids.alloc(sizeof(uint32_t) * nsub);
-    rows_sorted.alloc(sizeof(uint32_t) * nsub);
-    k_iota<<<grid_for(nsub), kT, 0, s>>>(ids.as<uint32_t>(), nsub);
-    NWGR_HIP(hipGetLastError());
-    const int end_bit = static_cast<int>(bits_for(n));
-    cub_call([&](void* tmp, size_t& bytes) {
-      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, sprow.as<uint32_t>(), rows_sorted.as<uint32_t>(),
-                                                ids.as<uint32_t>(), t.rslot.as<uint32_t>(), static_cast<int>(nsub),
-                                                0, end_bit, s);
-    }, s);
-  }
+  // (dummy fragments carry row n: they sort last, after pair_ptr[n])
+  stable_sort_place(sprow.as<uint32_t>(), nsub, static_cast<int>(bits_for(n)), n + 1, t.pair_ptr.as<uint32_t>(),
+                    nullptr, t.rslot.as<uint32_t>(), s);
   t.partial.alloc(sizeof(float) * std::max<uint32_t>(nsub, 1));
   NWGR_HIP(hipMemsetAsync(t.partial.p, 0, t.partial.bytes, s));
   // rows whose partials a wavefront adds

@@ -183,6 +183,25 @@ def test_empty_graph_with_entries_is_einval():
     assert L.nwgr_pr_graph_create(off.ctypes.data, off.ctypes.data, 0, 1, None, 0, ctypes.byref(h)) == _lib.NWGR_EINVAL
 
 
+def test_entry_count_limit_and_sort_pass_bounds():
+    """m up to 2^32 - 2^28 is accepted (the uint32 offsets of
+    util/defaults.hpp:23-24 with room for the tile padding); beyond it, and
+    n >= 2^31, the create call refuses before reading the arrays or touching
+    a device.  The pass size of the device sorts is bounded to [2^16, 2^30]."""
+    L = _lib.pagerank_lib()
+    off = np.zeros(2, np.uint32)
+    h = ctypes.c_void_p()
+    too_many = (1 << 32) - (1 << 28) + 1
+    assert L.nwgr_pr_graph_create(off.ctypes.data, off.ctypes.data, 1, too_many, None, 0,
+                                  ctypes.byref(h)) == _lib.NWGR_EINVAL
+    assert "2^32 - 2^28" in L.nwgr_last_error().decode()
+    assert L.nwgr_pr_graph_create(off.ctypes.data, off.ctypes.data, 1 << 31, 0, None, 0,
+                                  ctypes.byref(h)) == _lib.NWGR_EINVAL
+    for bad in (0, (1 << 16) - 1, (1 << 30) + 1):
+        assert L.nwgr_pr_set_sort_batch(bad) == _lib.NWGR_EINVAL
+    assert L.nwgr_pr_set_sort_batch(1 << 30) == _lib.NWGR_OK
+
+
 def test_empty_graph_through_the_cache_and_the_mirror():
     import nwgr_amd
     g = nwgr_amd.CSRGraph(np.zeros(1, np.uint32), np.zeros(0, np.uint32))
