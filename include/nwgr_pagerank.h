@@ -406,8 +406,10 @@ int nwgr_multi_run_device_f32(nwgr_multi_graph* mg, const nwgr_pr_params* params
  * each; either pointer may be NULL). */
 int nwgr_multi_graph_results(const nwgr_multi_graph* mg, uint32_t* iters, double* errors);
 /* Decomposition: graphs with at most lds_max_n vertices (<= 36864) gather
- * from an LDS copy of their contributions; rows with more than heavy_min
- * entries are reduced by a wavefront each. */
+ * from an LDS copy of their contributions; rows with more than
+ * min(heavy_min, 512) entries are reduced by a wavefront each, shorter rows
+ * by 2 to 32 lanes (16 entries per lane and pass) or one lane (at most 16
+ * entries).  Default 36864, 512. */
 int nwgr_multi_get_tuning(const nwgr_multi_graph* mg, uint32_t* lds_max_n, uint32_t* heavy_min);
 int nwgr_multi_set_tuning(nwgr_multi_graph* mg, uint32_t lds_max_n, uint32_t heavy_min);
 uint32_t nwgr_multi_n(const nwgr_multi_graph* mg);

@@ -48,7 +48,7 @@ SEARCH_SPACE_BINNED = {
 
 SEARCH_SPACE_MULTI = {
     "lds_max_n": [0, 4096, 16384, 36864],
-    "heavy_min": [16, 32, 64, 128, 256],
+    "heavy_min": [64, 128, 256, 512, 1024],
 }
 
 

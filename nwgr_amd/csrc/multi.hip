@@ -9,15 +9,16 @@
 // caller's vector, dangling LEAK / UNIFORM / TELEPORT, stop on the graph's
 // own L1 change) with workgroup barriers between sweeps, then takes the next
 // graph.  The contributions of a graph's first `lds_max_n` vertices (all of
-// a smaller graph) are mirrored in LDS for the gathers; rows with more than
-// `heavy_min` entries are reduced by a wavefront each, the others by a thread
-// each.  Cross-workgroup
+// a smaller graph) are mirrored in LDS for the gathers.  Cross-workgroup
 // communication is the queue counter alone (one RMW atomic per graph).
 //
 // Inside every graph the vertices are relabelled by non-increasing in-degree
-// (stable), so the rows a wavefront takes together have about the same
-// length and its lanes stay converged; the caller's vectors (teleport,
-// initial ranks, output) are read and written through the permutation.
+// (stable), so rows of about one length are neighbours: a row gets a group
+// of lanes sized to its length (a wavefront for rows of more than
+// `heavy_min` entries, down to one lane for rows of at most 16), and the
+// lanes of a group read consecutive entries of the row.  The caller's
+// vectors (teleport, initial ranks, output) are read and written through the
+// permutation.
 
 #include <algorithm>
 #include <chrono>
@@ -37,8 +38,8 @@ constexpr int kMT = 1024;                 // threads per workgroup
 constexpr uint32_t kMultiMaxN = 65536;    // vertices per graph (16-bit local ids)
 constexpr uint32_t kMultiLdsMax = 36864;  // contributions mirrored in LDS at most (144 KiB)
 constexpr uint32_t kDLeak = 0, kDUniform = 1, kDTeleport = 2;
-constexpr int kEntriesPerStep = 4;  // entries of a row loaded together
-constexpr int kHold = 4;            // rows of a thread whose results are stored together
+constexpr int kU = 16;              // entries a lane loads per pass over its row
+constexpr int kClasses = 7;         // row classes: class c gives a row 64 >> c lanes
 
 struct MultiArgs {
   const uint32_t* __restrict__ offs;   // n + 1 (global)
@@ -47,8 +48,6 @@ struct MultiArgs {
   const uint32_t* __restrict__ perm;   // n: internal vertex -> caller vertex (inside its graph)
   const float* __restrict__ degf;      // n: divisor of the contribution
   const uint32_t* __restrict__ order;  // processing order of the graphs
-  const uint32_t* __restrict__ heavy;  // rows (global ids) with more than heavy_min entries, by graph
-  const uint32_t* __restrict__ hbase;  // count + 1
   const float* __restrict__ tel;       // n or null
   const float* __restrict__ init;      // n or null
   float* r[2];
@@ -112,30 +111,33 @@ __device__ __forceinline__ float cget(const float* __restrict__ cl, const float*
   else return cg[j];
 }
 
-// A heavy row by one wavefront: lane l adds entries l, l + 64, ... in order,
-// four index loads per lane in flight (loads past the row read the padding
-// of lidx and are masked to 0), then a butterfly.
-template <int M>
-__device__ __forceinline__ float gather_wave(const MultiArgs& a, const float* __restrict__ cl,
-                                             const float* __restrict__ cg, uint32_t K, uint32_t b, uint32_t e,
-                                             uint32_t wl) {
-  float z = 0.f;
-  for (uint32_t k = b + wl; k < e; k += 4 * 64) {
-    uint32_t j[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) j[q] = a.lidx[k + q * 64 < e ? k + q * 64 : a.m];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float x = cget<M>(cl, cg, K, j[q]);
-      z += k + q * 64 < e ? x : 0.f;
+// One graph, start to finish, by the whole workgroup.
+// Row classes of a graph (its rows are in non-increasing degree order):
+// class 0 takes the rows of more than min(heavy_min, kU * 32) entries, a
+// wavefront each; class c >= 1 the rows of at most min(heavy_min,
+// kU * (64 >> c)) entries, down to the next class's bound, 64 >> c lanes each
+// (at most kU entries per lane and pass).  band[c] = first row of class c,
+// band[kClasses] = n_g.
+__device__ __forceinline__ void class_bounds(const MultiArgs& a, uint32_t v0, uint32_t ng, uint32_t tid,
+                                             uint32_t* s_band) {
+  if (tid + 1 < kClasses) {
+    const uint32_t c = tid + 1;
+    const uint32_t thr = min(a.heavy_min, static_cast<uint32_t>(kU) * (64u >> c));
+    uint32_t lo = 0, hi = ng;  // first row with at most thr entries
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.offs[v0 + mid + 1] - a.offs[v0 + mid] <= thr) hi = mid; else lo = mid + 1;
     }
+    s_band[c] = lo;
+  } else if (tid + 1 == kClasses) {
+    s_band[0] = 0;
+    s_band[kClasses] = ng;
   }
-  return wsumf(z);
 }
 
-// One graph, start to finish, by the whole workgroup.
 template <int M>
-__device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*s_red)[kMT / 64]) {
+__device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*s_red)[kMT / 64],
+                            uint32_t* s_band) {
   const uint32_t tid = threadIdx.x, wl = tid & 63, wv = tid >> 6;
   const uint32_t v0 = a.vbase[g], ng = a.vbase[g + 1] - v0;
   // vertices [0, K) are mirrored in LDS
@@ -154,10 +156,17 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
     if (v < K) cl[v] = c;
     if (dg == 0.f) D += static_cast<double>(r);
   }
-  bsum2(D, unused, s_red);  // barrier: LDS and the start vectors are complete
+  class_bounds(a, v0, ng, tid, s_band);
+  bsum2(D, unused, s_red);  // barrier: LDS, the start vectors and the classes are complete
+  uint32_t band[kClasses + 1], ucum[kClasses + 1];  // class bounds, units before each class
+#pragma unroll
+  for (int k = 0; k <= kClasses; ++k) band[k] = __builtin_amdgcn_readfirstlane(s_band[k]);
+  ucum[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kClasses; ++k) ucum[k + 1] = ucum[k] + ((band[k + 1] - band[k] + (1u << k) - 1u) >> k);
+  const uint32_t nunits = ucum[kClasses];
   uint32_t p = 0, it = 0;
   double E = 0;
-  const uint32_t h0 = a.hbase[g], h1 = a.hbase[g + 1];
   while (it < a.max_iters) {
     const float* __restrict__ cg = a.c[p] + v0;
     const float* __restrict__ rin = a.r[p] + v0;
@@ -186,89 +195,85 @@ __device__ void solve_graph(const MultiArgs& a, uint32_t g, float* cl, double (*
       rout[v] = epilogue(v, z, t, dg, ro, c);
       if constexpr (M != kMirrorDouble) cout[v] = c;
     };
-    // Rows of ordinary length, a thread each, in a two-stage pipeline: the
-    // bounds and epilogue operands of the row after next and the first
-    // kEntriesPerStep entries of the next row are loaded while the current
-    // row is gathered.  Entries are loaded kEntriesPerStep at a time with no
-    // bounds branch (loads past a row's end read the padding of lidx and are
-    // masked to 0).  Each row adds its entries in order.  The results of
-    // kHold consecutive rows of a thread wait in registers and are stored
-    // together: loads and stores share one counter and may complete out of
-    // order, so the wait for a prefetched load after a store waits for the
-    // store too -- once per kHold rows instead of once per row.
+    // Rows by class: a class-c unit is 2^c consecutive rows with 64 >> c
+    // lanes each, and wavefront wv takes the units wv, wv + 16, ... of all
+    // classes in turn.  The lanes of a row load its entries j, j + G, ...
+    // (kU of them per pass, no bounds branch: loads past the row's end read
+    // the padding of lidx and are masked to 0), so the lanes of a load read
+    // consecutive entries; a butterfly over the row's lanes adds the lane
+    // sums, and the row's first lane finishes the row.  Two-stage pipeline:
+    // the bounds and epilogue operands of the unit after next and the first
+    // pass of entries of the next unit are loaded while a unit is gathered.
     const uint32_t pad = a.m;  // an index into the padding of lidx
-    struct RowOps {
-      uint32_t b, e;
+    struct Unit {
+      uint32_t rr, b, e, lg, j;
       float t, dg, ro;
+      bool fin;  // the lane finishes row rr
     };
-    auto row_in = [&](uint32_t v) {
-      RowOps o;
-      const bool ok = v < ng;
-      const uint32_t vc = ok ? v : 0u;
-      o.b = a.offs[v0 + vc];
-      o.e = ok ? a.offs[v0 + vc + 1] : o.b;
-      o.t = a.tel ? a.tel[a.perm[v0 + vc]] : un;
-      o.dg = a.degf[v0 + vc];
-      o.ro = rin[vc];
+    auto unit_at = [&](uint32_t u) {
+      Unit o;
+      uint32_t c = 0, first = 0, rlo = 0, rhi = 0;
+#pragma unroll
+      for (int k = 0; k < kClasses; ++k) {
+        if (u >= ucum[k]) {
+          c = k;
+          first = ucum[k];
+          rlo = band[k];
+          rhi = band[k + 1];
+        }
+      }
+      o.lg = 6u - c;  // log2 of the lanes per row
+      o.j = wl & ((1u << o.lg) - 1u);
+      const uint32_t row = rlo + ((u - first) << c) + (wl >> o.lg);
+      // past the last unit every lane takes row 0 and no entries
+      const bool ok = u < nunits && row < rhi;
+      o.rr = ok ? row : (u < nunits ? rlo : 0u);
+      o.b = a.offs[v0 + o.rr];
+      o.e = ok ? a.offs[v0 + o.rr + 1] : o.b;
+      o.t = a.tel ? a.tel[a.perm[v0 + o.rr]] : un;
+      o.dg = a.degf[v0 + o.rr];
+      o.ro = rin[o.rr];
+      o.fin = ok && o.j == 0;
       return o;
     };
-    auto first_entries = [&](const RowOps& o, uint32_t (&j)[kEntriesPerStep]) {
+    auto first_pass = [&](const Unit& o, uint32_t (&jj)[kU]) {
+      const uint32_t G = 1u << o.lg;
 #pragma unroll
-      for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[o.b + q < o.e ? o.b + q : pad];
+      for (int q = 0; q < kU; ++q) {
+        const uint32_t k = o.b + o.j + q * G;
+        jj[q] = a.lidx[k < o.e ? k : pad];
+      }
     };
-    RowOps cur = row_in(tid), nxt = row_in(tid + kMT);
-    uint32_t jc[kEntriesPerStep];
-    first_entries(cur, jc);
-    for (uint32_t vh = tid; vh < ng; vh += kHold * kMT) {
-      float hr[kHold], hc[kHold];
-      bool hk[kHold];
+    Unit cu = unit_at(wv), nu = unit_at(wv + kMT / 64);
+    uint32_t jc[kU];
+    first_pass(cu, jc);
+    for (uint32_t u = wv; u < nunits; u += kMT / 64) {
+      const Unit nn = unit_at(u + 2 * (kMT / 64));
+      uint32_t jn[kU];
+      first_pass(nu, jn);
+      const uint32_t G = 1u << cu.lg;
+      float z = 0.f;
 #pragma unroll
-      for (int h = 0; h < kHold; ++h) {
-        const uint32_t v = vh + h * kMT;
-        hk[h] = false;
-        if (v < ng) {
-          const RowOps nn = row_in(v + 2 * kMT);
-          uint32_t jn[kEntriesPerStep];
-          first_entries(nxt, jn);
-          if (cur.e - cur.b <= a.heavy_min) {
-            float z = 0.f;
+      for (int q = 0; q < kU; ++q) {
+        const float x = cget<M>(lcur, cg, K, jc[q]);
+        z += cu.b + cu.j + q * G < cu.e ? x : 0.f;
+      }
+      for (uint32_t k = cu.b + cu.j + G * kU; k < cu.e; k += G * kU) {
+        uint32_t jj[kU];
 #pragma unroll
-            for (int q = 0; q < kEntriesPerStep; ++q) {
-              const float x = cget<M>(lcur, cg, K, jc[q]);
-              z += cur.b + q < cur.e ? x : 0.f;
-            }
-            for (uint32_t k = cur.b + kEntriesPerStep; k < cur.e; k += kEntriesPerStep) {
-              uint32_t j[kEntriesPerStep];
+        for (int q = 0; q < kU; ++q) jj[q] = a.lidx[k + q * G < cu.e ? k + q * G : pad];
 #pragma unroll
-              for (int q = 0; q < kEntriesPerStep; ++q) j[q] = a.lidx[k + q < cur.e ? k + q : pad];
-#pragma unroll
-              for (int q = 0; q < kEntriesPerStep; ++q) {
-                const float x = cget<M>(lcur, cg, K, j[q]);
-                z += k + q < cur.e ? x : 0.f;
-              }
-            }
-            hr[h] = epilogue(v, z, cur.t, cur.dg, cur.ro, hc[h]);
-            hk[h] = true;
-          }
-          cur = nxt;
-          nxt = nn;
-#pragma unroll
-          for (int q = 0; q < kEntriesPerStep; ++q) jc[q] = jn[q];
+        for (int q = 0; q < kU; ++q) {
+          const float x = cget<M>(lcur, cg, K, jj[q]);
+          z += k + q * G < cu.e ? x : 0.f;
         }
       }
+      for (uint32_t o = 1; o < G; o <<= 1) z += __shfl_xor(z, static_cast<int>(o));
+      if (cu.fin) finish(cu.rr, z, cu.t, cu.dg, cu.ro);
+      cu = nu;
+      nu = nn;
 #pragma unroll
-      for (int h = 0; h < kHold; ++h) {
-        if (hk[h]) {
-          rout[vh + h * kMT] = hr[h];
-          if constexpr (M != kMirrorDouble) cout[vh + h * kMT] = hc[h];
-        }
-      }
-    }
-    for (uint32_t h = h0 + wv; h < h1; h += kMT / 64) {
-      const uint32_t v = a.heavy[h] - v0;
-      const uint32_t b = a.offs[v0 + v], e = a.offs[v0 + v + 1];
-      const float z = gather_wave<M>(a, lcur, cg, K, b, e, wl);
-      if (wl == 0) finish(v, z, a.tel ? a.tel[a.perm[v0 + v]] : un, a.degf[v0 + v], rin[v]);
+      for (int q = 0; q < kU; ++q) jc[q] = jn[q];
     }
     bsum2(err, dang, s_red);  // barrier: every gather of this sweep is done
     if constexpr (M == kMirrorAll || M == kMirrorPrefix) {
@@ -293,6 +298,7 @@ __global__ __launch_bounds__(kMT) void k_multi(MultiArgs a) {
   extern __shared__ float cl[];
   __shared__ double s_red[2][kMT / 64];
   __shared__ uint32_t s_g;
+  __shared__ uint32_t s_band[kClasses + 1];
   for (;;) {
     if (threadIdx.x == 0) {
       const uint32_t q = __hip_atomic_fetch_add(a.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -304,13 +310,13 @@ __global__ __launch_bounds__(kMT) void k_multi(MultiArgs a) {
     if (g == 0xffffffffu) return;
     const uint32_t ng = a.vbase[g + 1] - a.vbase[g];
     if (ng <= a.lds_max_n / 2)
-      solve_graph<kMirrorDouble>(a, g, cl, s_red);
+      solve_graph<kMirrorDouble>(a, g, cl, s_red, s_band);
     else if (ng <= a.lds_max_n)
-      solve_graph<kMirrorAll>(a, g, cl, s_red);
+      solve_graph<kMirrorAll>(a, g, cl, s_red, s_band);
     else if (a.lds_max_n > 0)
-      solve_graph<kMirrorPrefix>(a, g, cl, s_red);
+      solve_graph<kMirrorPrefix>(a, g, cl, s_red, s_band);
     else
-      solve_graph<kMirrorNone>(a, g, cl, s_red);
+      solve_graph<kMirrorNone>(a, g, cl, s_red, s_band);
     __syncthreads();
   }
 }
@@ -372,8 +378,8 @@ struct MultiGraph {
   uint64_t m = 0;
   uint32_t num_cu = 0;
   uint32_t lds_max_n = kMultiLdsMax;
-  uint32_t heavy_min = 64;
-  DevBuf offs, lidx, vbase, perm, degf, order, heavy, hbase, r[2], c[2], iters, errs, queue;
+  uint32_t heavy_min = 512;
+  DevBuf offs, lidx, vbase, perm, degf, order, r[2], c[2], iters, errs, queue;
   std::vector<uint32_t> offs_h, vbase_h;
   // per-graph results of the last run, copied into pinned host memory
   uint32_t* iters_h = nullptr;
@@ -391,30 +397,14 @@ struct MultiGraph {
 
 namespace {
 
-void multi_plan_heavy(MultiGraph& mg) {
-  std::vector<uint32_t> heavy, hbase(mg.count + 1, 0);
-  for (uint32_t g = 0; g < mg.count; ++g) {
-    hbase[g] = static_cast<uint32_t>(heavy.size());
-    for (uint32_t v = mg.vbase_h[g]; v < mg.vbase_h[g + 1]; ++v)
-      if (mg.offs_h[v + 1] - mg.offs_h[v] > mg.heavy_min) heavy.push_back(v);
-  }
-  hbase[mg.count] = static_cast<uint32_t>(heavy.size());
-  mg.heavy.alloc(sizeof(uint32_t) * std::max<size_t>(heavy.size(), 1));
-  mg.hbase.alloc(sizeof(uint32_t) * (mg.count + 1));
-  if (!heavy.empty())
-    NWGR_HIP(hipMemcpyAsync(mg.heavy.p, heavy.data(), sizeof(uint32_t) * heavy.size(), hipMemcpyHostToDevice,
-                            mg.stream));
-  NWGR_HIP(hipMemcpyAsync(mg.hbase.p, hbase.data(), sizeof(uint32_t) * (mg.count + 1), hipMemcpyHostToDevice,
-                          mg.stream));
-  NWGR_HIP(hipStreamSynchronize(mg.stream));
-}
 
 MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
                          const uint32_t* vbase, uint32_t count, const uint32_t* degrees, int device) {
   NWGR_REQUIRE(offsets && vbase, "null argument");
   NWGR_REQUIRE(m == 0 || indices, "indices is null");
   NWGR_REQUIRE(count >= 1, "count must be >= 1");
-  NWGR_REQUIRE(m < 0xffffffffULL, "m must fit the uint32 offsets");
+  // (a row's lanes form entry positions up to 4 * 64 past its end)
+  NWGR_REQUIRE(m < 0xffffffffULL - 256, "m must fit the uint32 offsets with room for the row passes");
   NWGR_REQUIRE(offsets[0] == 0 && offsets[n] == m, "offsets must start at 0 and end at m");
   NWGR_REQUIRE(vbase[0] == 0 && vbase[count] == n, "vbase must start at 0 and end at n");
   for (uint32_t g = 0; g < count; ++g) {
@@ -513,7 +503,6 @@ MultiGraph* multi_create(const uint32_t* offsets, const uint32_t* indices, uint3
   mg->iters.alloc(sizeof(uint32_t) * count);
   mg->errs.alloc(sizeof(double) * count);
   mg->queue.alloc(sizeof(uint32_t));
-  multi_plan_heavy(*mg);
   mg->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return mg.release();
 }
@@ -538,8 +527,6 @@ void multi_run(MultiGraph& mg, const nwgr_pr_params& p, float* d_out, nwgr_pr_re
   a.perm = mg.perm.as<uint32_t>();
   a.degf = mg.degf.as<float>();
   a.order = mg.order.as<uint32_t>();
-  a.heavy = mg.heavy.as<uint32_t>();
-  a.hbase = mg.hbase.as<uint32_t>();
   a.tel = p.teleport;
   a.init = p.init;
   for (int b = 0; b < 2; ++b) {
@@ -685,10 +672,7 @@ int nwgr_multi_set_tuning(nwgr_multi_graph* mg, uint32_t lds_max_n, uint32_t hea
     NWGR_REQUIRE(heavy_min >= 1, "heavy_min must be >= 1");
     nwgr::DeviceGuard dg(mg->device);
     mg->lds_max_n = lds_max_n;
-    if (mg->heavy_min != heavy_min) {
-      mg->heavy_min = heavy_min;
-      nwgr::multi_plan_heavy(*mg);
-    }
+    mg->heavy_min = heavy_min;
   });
 }
 

@@ -49,11 +49,11 @@ def test_autotune_multi_picks_valid_setting_and_stays_exact():
         t = res["tuning"]
         say(f"autotune_multi pick {t} {res['ms_per_run']:.3f} ms")
         for name, values in SEARCH_SPACE_MULTI.items():
-            assert t[name] in values or t[name] == {"lds_max_n": 36864, "heavy_min": 64}[name]
+            assert t[name] in values or t[name] == {"lds_max_n": 36864, "heavy_min": 512}[name]
         assert mp.tuning == t
         r1, res1 = mp.run(tol=1e-6)
         assert res1.converged
-        if t["heavy_min"] == 64:
-            # only the row-length threshold (thread vs wavefront sums) changes the summation order
+        if t["heavy_min"] == 512:
+            # only the row-length threshold (lanes per row) changes the summation order
             np.testing.assert_array_equal(r0, r1)
         check_graphs(g, vbase, deg, r1, "uniform", range(len(ns)), mp=mp)

@@ -41,11 +41,14 @@ def check_graphs(g, vbase, deg, r, mode, which, tol=1e-6, max_iters=100, mp=None
         err = l1(r[v0:v1], r64)
         assert err <= 1e-6, (gi, err)
         assert abs(int(iters[gi]) - int(k_ref)) <= 1, (gi, iters[gi], k_ref)
-        assert_same_top_k(r[v0:v1], r64, min(20, v1 - v0))
+        # FP64 ranks closer than 1e-6 relative are within the FP32 sums'
+        # rounding (a pair 2.2e-7 apart, two FP32 ulps, swaps when the lanes
+        # of a row add in another grouping)
+        assert_same_top_k(r[v0:v1], r64, min(20, v1 - v0), tie_rel=1e-6)
 
 
 @pytest.mark.parametrize("mode", ["leak", "uniform", "teleport"])
-@pytest.mark.parametrize("tuning", [dict(lds_max_n=36864, heavy_min=64), dict(lds_max_n=0, heavy_min=16),
+@pytest.mark.parametrize("tuning", [dict(lds_max_n=36864, heavy_min=512), dict(lds_max_n=0, heavy_min=16),
                                     dict(lds_max_n=2048, heavy_min=100000)])
 def test_multi_graphs_match_oracle(mode, tuning):
     say(f"multi {mode} {tuning}")
