@@ -127,6 +127,15 @@ def load_traffic(scale, ef, workload="pagerank"):
     return d.get("hbm_bytes_per_sweep")
 
 
+def load_profile_bytes(name, key):
+    """A measured byte count from a committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(key)
+
+
 def host_info(threads):
     """The host the CPU baseline runs on: CPU model, sockets, physical cores,
     logical CPUs, the CPUs this process may use (affinity, cgroup quota) and
@@ -333,6 +342,8 @@ def bench_ppr(args):
     n, m = g.n, g.m
     b_iter = 4 * m + 4 * k * m + 8 * k * n + 8 * n
     achieved = b_iter * sweeps / (dev_ms / 1e3) / 1e9
+    traffic = (load_profile_bytes(f"k_ppr_traffic_rmat_s{scale}_k{k}.json", "hbm_bytes_per_sweep")
+               if args.edge_factor == 16 else None)
     line = {
         "metric": "batched personalized PageRank, edge-columns per second", "value": round(m * k * sweeps / wall / 1e9, 3),
         "unit": "G edge-columns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -342,8 +353,10 @@ def bench_ppr(args):
                                f"tol {args.tol:g} per column", "n": n, "m": m, "k": k,
                    "sweeps_per_solve": res.iters, "device_ms_per_solve": round(dev_ms / args.steps, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_ppr_sweep+k_ppr_hub+k_ppr_reduce", "algorithmic_bytes_per_sweep": b_iter},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_ppr_sweep+k_ppr_hub+k_ppr_reduce", "algorithmic_bytes_per_sweep": b_iter,
+                     "traffic_unit": "bytes per sweep (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc)",
+                     "traffic_rate_gbs": None if traffic is None else round(traffic * sweeps / (dev_ms / 1e3) / 1e9, 1)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -512,6 +525,7 @@ def bench_multi(args):
     n_g = np.diff(vbase.astype(np.int64))
     b_sweeps = int(np.sum((8 * m_g + 16 * n_g) * iters.astype(np.int64)))  # last run's sweeps per graph
     achieved = b_sweeps / (dev_ms / args.steps / 1e3) / 1e9
+    traffic = load_profile_bytes("k_multi_traffic_2048_rmat.json", "hbm_bytes_per_launch")
     line = {
         "metric": "many-graph PageRank GTEPS (sum over graphs of edges x sweeps / s)",
         "value": round(edge_sweeps / wall / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
@@ -522,8 +536,12 @@ def bench_multi(args):
                    "sweeps_min": int(iters.min()), "sweeps_max": int(iters.max()),
                    "device_ms_per_solve": round(dev_ms / args.steps, 3), "tuning": mp.tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_multi",
-                     "algorithmic_bytes": "sum over graphs of (8 m_g + 16 n_g) x sweeps_g"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_multi",
+                     "algorithmic_bytes": "sum over graphs of (8 m_g + 16 n_g) x sweeps_g",
+                     "algorithmic_bytes_per_launch": b_sweeps,
+                     "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc); below the "
+                                     "algorithmic bytes: 2-byte local ids, contributions gathered from LDS",
+                     "traffic_rate_gbs": None if traffic is None else round(traffic / (dev_ms / args.steps / 1e3) / 1e9, 1)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
