@@ -378,12 +378,12 @@ void replan(Graph& g, hipStream_t stream) {
     if (!tl->built) continue;
     if (tl->bits != t.tile_bits)
       *tl = Tiled{};  // rebuilt on next use
-    else if (tl->nblocks != tile_blocks(g) || tl->fill_cost != t.tile_fill_cost)
+    else if (tl->nblocks != layout_blocks(g, *tl) || tl->fill_cost != t.tile_fill_cost)
       plan_tasks(g, *tl, stream);
   }
   if (!g.gs.empty() && g.gs_bits != t.tile_bits) g.gs.clear();  // the phases' layouts likewise
   for (Tiled& tl : g.gs)
-    if (tl.nblocks != tile_blocks(g) || tl.fill_cost != t.tile_fill_cost) plan_tasks(g, tl, stream);
+    if (tl.nblocks != layout_blocks(g, tl) || tl.fill_cost != t.tile_fill_cost) plan_tasks(g, tl, stream);
   g.grid = std::max<uint32_t>(
       1, std::min<uint32_t>(g.num_cu * t.blocks_per_cu, std::max<uint32_t>(g.fwd.segs.total_units, 64)));
   g.grid_c = std::max<uint32_t>(1, g.num_cu * 8);

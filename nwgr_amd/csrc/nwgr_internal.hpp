@@ -282,6 +282,14 @@ struct Tiled {
   DevBuf tasks;      // TiledTask[ntasks]
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
   DevBuf wave_rows;  // u32[nwave_rows] rows with more than kWaveRowMin partials
+  // Wave rows past row0 are left out of the windows of the gather pass: lpp
+  // and lslot are pair_ptr and rslot over the other rows only (a wave row's
+  // range empty), and heavy marks the wave rows (a bit per row).  Without wave
+  // rows past row0 (light_split false) the pass reads pair_ptr and rslot.
+  bool light_split = false;
+  DevBuf lpp;        // u32[n + 1]
+  DevBuf lslot;      // u32[fragments of the other rows]
+  DevBuf heavy;      // u32[(n + 31) / 32]
   std::vector<uint32_t> tile_chunk0;  // host: first chunk of each tile (ntiles + 1)
   // accumulate form of the second pass (built on first use): rows cut into
   // blocks of at most kAccRows rows and about kAccFrags fragments; inside a
@@ -451,7 +459,7 @@ struct Graph {
 inline uint64_t bytes_of(const Tiled& t) {
   uint64_t b = 0;
   for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
-                          &t.wg_task, &t.wave_rows, &t.ablo, &t.aseg, &t.afrow})
+                          &t.wg_task, &t.wave_rows, &t.ablo, &t.aseg, &t.afrow, &t.lpp, &t.lslot, &t.heavy})
     b += d->bytes;
   return b;
 }
@@ -523,6 +531,8 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s);
 
 // workgroups of the tile pass for the current tile width
 uint32_t tile_blocks(const Graph& g);
+// workgroups of the tile pass of layout t (fewer for a row-range layout)
+uint32_t layout_blocks(const Graph& g, const Tiled& t);
 // the tiled layout the engine choice selects for the in-CSR (adj = false) or
 // the transposed CSR (adj = true), built on first use; null = degree-binned
 Tiled* select_tiled(Graph& g, bool adj, hipStream_t s);

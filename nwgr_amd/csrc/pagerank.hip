@@ -572,6 +572,11 @@ struct CombineArgs {
   const uint32_t* __restrict__ rslot;
   const float* __restrict__ partial;
   const uint32_t* __restrict__ wave_rows;
+  // the windows' row ranges and slots (pair_ptr and rslot without the wave
+  // rows past row0), and the wave-row bits (null: no wave rows past row0)
+  const uint32_t* __restrict__ lpp;
+  const uint32_t* __restrict__ lslot;
+  const uint32_t* __restrict__ heavy;
   uint32_t nwave_rows;
   uint32_t row0;        // rows [row_lo, row0) all have more than kWaveRowMin partials (a wave each)
   uint32_t row_hi;      // the layout's rows end here (they start at or before row0)
@@ -1126,33 +1131,37 @@ __global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
   // steps whose partials fit the LDS stage (the longest prefix of its rows
   // that fits, found with one counting barrier); each step's partials are
   // gathered into LDS by the whole workgroup, with four loads in flight per
-  // thread, then every thread adds its row's partials in tile order.
+  // thread, then every thread adds its row's partials in tile order.  Wave
+  // rows inside a window have empty ranges in lpp and are not finished here.
+  const uint32_t* __restrict__ lp = a.lpp;
   const uint32_t n = min(a.s.nrows, a.row_hi);  // quiet rows [nrows, n) are not stored
   for (uint32_t w0 = a.row0 + vb * kCombineThreads; w0 < n; w0 += G * kCombineThreads) {
     const uint32_t w1 = min(w0 + kCombineThreads, n);
     for (uint32_t r0 = w0; r0 < w1;) {
       const uint32_t row = r0 + tid;
       const bool valid = row < w1;
-      const uint32_t base = pp[r0];
+      const uint32_t base = lp[r0];
       uint32_t b = 0, e = 0;
+      bool wave = false;
       RowIn in{};
       if (valid) {
-        b = pp[row];
-        e = pp[row + 1];
+        b = lp[row];
+        e = lp[row + 1];
+        wave = a.heavy && ((a.heavy[row >> 5] >> (row & 31)) & 1u);
         in = load_row<KIND>(a.s, row);
       }
       // rows r0 .. r0 + nfit - 1 fit the stage (a prefix: pp is monotone)
       uint32_t nfit = static_cast<uint32_t>(__syncthreads_count(valid && e - base <= kCombineStage));
       const bool staged = nfit > 0;  // else row r0 alone overflows: a wave row, skipped here
       nfit = max(nfit, 1u);
-      const uint32_t end = staged ? pp[r0 + nfit] : base;
+      const uint32_t end = staged ? lp[r0 + nfit] : base;
       for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kCombineThreads) {
         uint32_t sl[4];
         float v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const uint32_t k = k0 + j * kCombineThreads;
-          sl[j] = k < end ? a.rslot[k] : 0u;
+          sl[j] = k < end ? a.lslot[k] : 0u;
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[j] = k0 + j * kCombineThreads < end ? a.partial[sl[j]] : 0.f;
@@ -1161,7 +1170,7 @@ __global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
           if (k0 + j * kCombineThreads < end) s_buf[k0 + j * kCombineThreads - base] = v[j];
       }
       __syncthreads();
-      if (staged && tid < nfit && e - b <= kWaveRowMin) {
+      if (staged && tid < nfit && !wave && e - b <= kWaveRowMin) {
         float z = 0.f;
         for (uint32_t k = b; k < e; ++k) z += s_buf[k - base];
         finish_row<KIND>(a.s, row, z, Df, in, err, dang);
@@ -1469,6 +1478,9 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ca.rslot = t.rslot.as<uint32_t>();
   ca.partial = t.partial.as<float>();
   ca.wave_rows = t.wave_rows.as<uint32_t>();
+  ca.lpp = t.light_split ? t.lpp.as<uint32_t>() : ca.pair_ptr;
+  ca.lslot = t.light_split ? t.lslot.as<uint32_t>() : ca.rslot;
+  ca.heavy = t.light_split ? t.heavy.as<uint32_t>() : nullptr;
   ca.nwave_rows = t.nwave_rows;
   ca.row0 = t.row0;
   ca.row_hi = t.row_hi;

@@ -145,6 +145,18 @@ __global__ __launch_bounds__(kT) void k_count_runs(const uint32_t* __restrict__ 
   if (wl == 0 && cnt) atomicAdd(runs, static_cast<unsigned long long>(cnt));
 }
 
+// The slots of the rows that are not wave rows, in row order (lslot).
+__global__ void k_light_slots(const uint32_t* __restrict__ pp, const uint32_t* __restrict__ lpp,
+                              const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ rslot, uint32_t row_lo,
+                              uint32_t row_hi, uint32_t* __restrict__ lslot) {
+  GRID_STRIDE(i, static_cast<uint64_t>(row_hi - row_lo)) {
+    const uint32_t r = row_lo + static_cast<uint32_t>(i);
+    if ((heavy[r >> 5] >> (r & 31)) & 1u) continue;
+    const uint32_t b = pp[r], e = pp[r + 1], o = lpp[r];
+    for (uint32_t k = b; k < e; ++k) lslot[o + (k - b)] = rslot[k];
+  }
+}
+
 // Weights in tiled order (stored in the chunk storage order of wgt_pos).
 __global__ void k_gather_w(const float* __restrict__ src, const uint32_t* __restrict__ pos, uint64_t count,
                            float* __restrict__ dst) {
@@ -221,6 +233,19 @@ void cub_call(Fn&& fn, hipStream_t s) {
 
 }  // namespace
 
+// A row-range layout (a phase or a row share) holds a fraction of the entries
+// but every workgroup of its tile pass fills a whole tile: fewer workgroups
+// fill fewer tiles.
+#ifndef NWGR_SUB_LAYOUT_DIV
+#define NWGR_SUB_LAYOUT_DIV 1
+#endif
+constexpr uint32_t kSubLayoutDiv = NWGR_SUB_LAYOUT_DIV;
+
+uint32_t layout_blocks(const Graph& g, const Tiled& t) {
+  const bool sub = t.row_lo > 0 || t.row_hi < g.n;
+  return sub ? std::max(1u, tile_blocks(g) / kSubLayoutDiv) : tile_blocks(g);
+}
+
 uint32_t tile_blocks(const Graph& g) {
   // one 1024-thread workgroup per CU holds a 128 KiB tile; narrower tiles
   // leave room for two
@@ -281,7 +306,7 @@ __global__ void k_chunk_frags(const uint32_t* __restrict__ lane_sp0, uint32_t nc
 }  // namespace
 
 void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
-  const uint32_t G = tile_blocks(g);
+  const uint32_t G = layout_blocks(g, t);
   const double F = g.tune.tile_fill_cost;
   // chunk costs from the fragments of every chunk (a chunk starts a group,
   // so its first lane starts a fragment)
@@ -455,6 +480,30 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   if (!wave_rows.empty())
     NWGR_HIP(hipMemcpyAsync(t.wave_rows.p, wave_rows.data(), sizeof(uint32_t) * wave_rows.size(),
                             hipMemcpyHostToDevice, s));
+  // wave rows past row0: the windows of the gather pass skip them
+  t.light_split = t.nwave_rows > t.row0 - row_lo;
+  if (t.light_split) {
+    std::vector<uint32_t> lpp(static_cast<size_t>(n) + 1), hb((static_cast<size_t>(n) + 31) / 32, 0u);
+    uint64_t acc = 0;
+    for (uint32_t r : wave_rows) hb[r >> 5] |= 1u << (r & 31);
+    for (uint32_t r = 0; r < n; ++r) {
+      lpp[r] = static_cast<uint32_t>(acc);
+      if (!((hb[r >> 5] >> (r & 31)) & 1u)) acc += pp[r + 1] - pp[r];
+    }
+    lpp[n] = static_cast<uint32_t>(acc);
+    t.lpp.alloc(sizeof(uint32_t) * lpp.size());
+    t.heavy.alloc(sizeof(uint32_t) * hb.size());
+    t.lslot.alloc(sizeof(uint32_t) * std::max<uint64_t>(acc, 1));
+    NWGR_HIP(hipMemcpyAsync(t.lpp.p, lpp.data(), sizeof(uint32_t) * lpp.size(), hipMemcpyHostToDevice, s));
+    NWGR_HIP(hipMemcpyAsync(t.heavy.p, hb.data(), sizeof(uint32_t) * hb.size(), hipMemcpyHostToDevice, s));
+    if (row_hi > row_lo) {
+      k_light_slots<<<grid_for(row_hi - row_lo), kT, 0, s>>>(t.pair_ptr.as<uint32_t>(), t.lpp.as<uint32_t>(),
+                                                             t.heavy.as<uint32_t>(), t.rslot.as<uint32_t>(), row_lo,
+                                                             row_hi, t.lslot.as<uint32_t>());
+      NWGR_HIP(hipGetLastError());
+    }
+    NWGR_HIP(hipStreamSynchronize(s));  // the host vectors must outlive the copies
+  }
   plan_tasks(g, t, s);
   t.built = true;
 }
@@ -566,8 +615,14 @@ std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
 
 void ensure_gs(Graph& g, uint32_t phases, hipStream_t s) {
   if (g.gs.size() == phases && g.gs_bits == g.tune.tile_bits) {
+    bool replanned = false;
     for (Tiled& t : g.gs)
-      if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
+      if (t.nblocks != layout_blocks(g, t) || t.fill_cost != g.tune.tile_fill_cost) {
+        plan_tasks(g, t, s);
+        replanned = true;
+      }
+    // captured sweeps hold the old task lists
+    if (replanned) g.drop_graphs(static_cast<int>(kExecGs));
     return;
   }
   g.gs.clear();
@@ -621,7 +676,7 @@ Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
     if (!choice) return nullptr;
   }
   build_tiled(g, t, offs, idx, g.m, 0, g.n, s);
-  if (t.nblocks != tile_blocks(g) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
+  if (t.nblocks != layout_blocks(g, t) || t.fill_cost != g.tune.tile_fill_cost) plan_tasks(g, t, s);
   // second pass: accumulate when rows have about one entry per tile run
   const uint32_t mode = g.tune.tile_combine;
   const bool acc = mode == kCombineAccumulate ||
