@@ -99,9 +99,11 @@ def test_multi_rejects_bad_layouts():
 
 
 @pytest.mark.slow
+@pytest.mark.timeout(900)
 def test_config4b_2048_rmat_graphs():
     """2,048 R-MAT graphs, n log-uniform in [1k, 64k], edge factor 8, one
-    launch; every graph sums to 1 (UNIFORM), 48 graphs against the oracle."""
+    launch; every graph sums to 1 (UNIFORM) and every graph against the
+    oracle (L1, sweep count, top order)."""
     say("multi 2048 graphs")
     ns = sizes(2048, 1024, 65536, 27491095)
     g, vbase = nwgr_amd.rmat_many(ns, 8)
@@ -113,6 +115,5 @@ def test_config4b_2048_rmat_graphs():
         assert np.abs(sums - 1.0).max() < 1e-4
         iters, _ = mp.graph_results()
         say(f"  {g.m} edges, {res.ms:.1f} ms, sweeps {iters.min()}..{iters.max()}")
-        order = np.argsort(-ns)
-        which = list(order[:16]) + list(np.random.default_rng(0).choice(len(ns), 32, replace=False))
-        check_graphs(g, vbase, deg, r, "uniform", which, mp=mp)
+        check_graphs(g, vbase, deg, r, "uniform", range(len(ns)), mp=mp)
+        say(f"  all {len(ns)} graphs match the oracle")
