@@ -386,7 +386,12 @@ void replan(Graph& g, hipStream_t stream) {
     if (tl.nblocks != layout_blocks(g, tl) || tl.fill_cost != t.tile_fill_cost) plan_tasks(g, tl, stream);
   g.grid = std::max<uint32_t>(
       1, std::min<uint32_t>(g.num_cu * t.blocks_per_cu, std::max<uint32_t>(g.fwd.segs.total_units, 64)));
-  g.grid_c = std::max<uint32_t>(1, g.num_cu * 8);
+  // the gather pass runs one wave of resident workgroups taking items from a queue
+  g.grid_c = std::max<uint32_t>(1, g.num_cu * combine_blocks_per_cu());
+  for (Tiled* tl : {&g.tfwd, &g.tadj})
+    if (tl->built && tl->gplan_blocks != g.grid_c) plan_gather(g, *tl, stream);
+  for (Tiled& tl : g.gs)
+    if (tl.gplan_blocks != g.grid_c) plan_gather(g, tl, stream);
   const uint32_t stride = std::max(g.grid, g.grid_c);
   if (stride != g.part_stride || !g.partials.p) {
     g.part_stride = stride;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -238,6 +239,23 @@ constexpr uint32_t kLaneEdges = 16;
 constexpr uint32_t kGroupEdges = 16 * kLaneEdges;
 constexpr uint32_t kChunkEdges = 64 * kLaneEdges;
 constexpr uint32_t kWaveRowMin = 32;           // rows with more partials are combined by a wavefront
+// The gather form of the second pass works through items, each workgroup its
+// own list: rows with more than kBigRowMin partials one each (the whole
+// workgroup adds them), windows of consecutive rows -- at most kWindowRows
+// rows (kRowsPerThread per thread) and kCombineStage partials, staged in LDS
+// in one step -- and blocks of kQuietRows quiet rows.
+constexpr uint32_t kCombineThreads = 256;
+constexpr uint32_t kRowsPerThread = 4;
+constexpr uint32_t kWindowRows = kCombineThreads * kRowsPerThread;
+constexpr uint32_t kCombineStage = 2048;
+constexpr uint32_t kBigRowMin = 2048;
+constexpr uint32_t kQuietRows = 16 * kCombineThreads;
+static_assert(kBigRowMin <= kCombineStage, "a row that is not big fits one window");
+
+// Fragment sums of one chunk pass out of the tile pass through a per-wavefront
+// LDS window of this many slots; a chunk with more fragments takes one pass
+// per window.
+constexpr uint32_t kTileWindow = 496;
 
 // Storage order inside a chunk.  The layout is built in logical positions
 // (entry i of lane l of chunk c at c*1024 + 16 l + i); the local ids and the
@@ -270,8 +288,7 @@ struct Tiled {
   uint32_t nblocks = 0;          // workgroups of the tile pass the tasks were planned for
   uint32_t fill_cost = 0;        // tile fill cost the tasks were planned with
   uint32_t ntasks = 0;
-  uint32_t nwave_rows = 0;
-  uint32_t row0 = 0;             // rows [row_lo, row0) all have more than kWaveRowMin partials
+  uint32_t nbig = 0;             // rows with more than kBigRowMin partials
   DevBuf eloc;       // u16[nchunks * kChunkEdges]
   DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
   DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)
@@ -281,15 +298,32 @@ struct Tiled {
   DevBuf partial;    // f32[nsub] one partial sum per fragment ordinal
   DevBuf tasks;      // TiledTask[ntasks]
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
-  DevBuf wave_rows;  // u32[nwave_rows] rows with more than kWaveRowMin partials
-  // Wave rows past row0 are left out of the windows of the gather pass: lpp
-  // and lslot are pair_ptr and rslot over the other rows only (a wave row's
-  // range empty), and heavy marks the wave rows (a bit per row).  Without wave
-  // rows past row0 (light_split false) the pass reads pair_ptr and rslot.
+  DevBuf big_rows;   // u32[nbig] rows with more than kBigRowMin partials, ascending
+  // Big rows are left out of the windows of the gather pass: lpp and lslot are
+  // pair_ptr and rslot over the other rows only (a big row's range empty), and
+  // heavy marks the big rows (a bit per row).  Without big rows (light_split
+  // false) the pass reads pair_ptr and rslot.
   bool light_split = false;
   DevBuf lpp;        // u32[n + 1]
   DevBuf lslot;      // u32[fragments of the other rows]
   DevBuf heavy;      // u32[(n + 31) / 32]
+  // Windows of the gather pass: row boundaries win[0] = row_lo < ... <
+  // win[nwin] = row_hi; no window crosses nz (so a solve whose quiet rows are
+  // kept as a scalar takes the windows that start before nz).
+  uint32_t nwin = 0;
+  DevBuf win;                     // u32[nwin + 1]
+  std::vector<uint32_t> win_h;    // host copy
+  // Item lists of the gather pass for [0] a sweep that stores every row and
+  // [1] one that keeps the quiet rows as the scalar: items are numbered big
+  // rows, then windows, then quiet-row blocks; workgroup b takes
+  // items[ptr[b] .. ptr[b + 1]), planned for gplan_blocks workgroups.
+  struct GatherPlan {
+    uint32_t nwin = 0, nqb = 0;
+    DevBuf ptr, items;
+  };
+  GatherPlan gplan[2];
+  uint32_t gplan_blocks = 0;
+  std::vector<float> big_cost, win_cost;  // host: estimated cost of each big row and window
   std::vector<uint32_t> tile_chunk0;  // host: first chunk of each tile (ntiles + 1)
   // accumulate form of the second pass (built on first use): rows cut into
   // blocks of at most kAccRows rows and about kAccFrags fragments; inside a
@@ -383,8 +417,8 @@ struct Graph {
   Tiled tfwd, tadj;  // tiled layouts of the in-CSR and of the transposed CSR (built on first use)
   int auto_tiled[2] = {-1, -1};  // engine auto choice per CSR (fwd, adj): -1 not measured yet
   uint32_t auto_bits = 0;        // tile width the auto choices were measured for
-  uint32_t grid_c = 0;  // workgroups of the tiled engine's combine pass
-  uint32_t part_stride = 0;  // partial pairs per parity in `partials` (max of both producers)
+  uint32_t grid_c = 0;  // workgroups of the tiled engine's second pass (the resident count of the gather form)
+  uint32_t part_stride = 0;  // partial pairs per parity in `partials` (at least every producer's count)
 
   // weighted / adjoint work arrays
   DevBuf w_in, w_out;  // f32[m] weights in in-CSR (new) and out-CSR order
@@ -421,11 +455,13 @@ struct Graph {
 
   // block Gauss-Seidel (built on first use): the phases' layouts, their row
   // cuts (positions in the relabelled order, phases + 1 entries) and the
-  // per-phase (error, dangling) pairs [2 parities][phases][grid_c][2]
+  // per-phase (error, dangling) pairs [2 parities][gs_stride][2], phase q's
+  // at the sum of the earlier phases' counts
   std::vector<Tiled> gs;
   std::vector<uint32_t> gs_cuts;
   uint32_t gs_bits = 0;
   DevBuf gs_part;
+  uint32_t gs_stride = 0;  // pairs per parity in gs_part
   hipStream_t own_stream = nullptr;
 
   std::vector<double> last_errs;
@@ -459,7 +495,9 @@ struct Graph {
 inline uint64_t bytes_of(const Tiled& t) {
   uint64_t b = 0;
   for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
-                          &t.wg_task, &t.wave_rows, &t.ablo, &t.aseg, &t.afrow, &t.lpp, &t.lslot, &t.heavy})
+                          &t.wg_task, &t.big_rows, &t.win, &t.gplan[0].ptr, &t.gplan[0].items,
+                          &t.gplan[1].ptr, &t.gplan[1].items, &t.ablo, &t.aseg, &t.afrow, &t.lpp,
+                          &t.lslot, &t.heavy})
     b += d->bytes;
   return b;
 }
@@ -547,6 +585,20 @@ inline size_t iter_len(uint32_t n) {
 // solves (pagerank.hip, weighted.hip)
 // Runs the chunked fixed-point iteration; the final iterate is left in
 // g.r[res.iters & 1] (new labels).
+// The gather plan a sweep of this kind takes (1: quiet rows kept as the
+// scalar, which only the plain kind does).
+inline uint32_t gather_plan_index(const Tiled& t, uint32_t kind, bool quiet) {
+  (void)t;
+  return kind == kPlain && quiet ? 1u : 0u;
+}
+// (Error, dangling) pairs one sweep of the solve writes (one per workgroup of
+// each second pass): the count the next sweep's stop test adds.
+uint32_t solve_pairs(const Graph& g, const SolveSpec& sp);
+// Workgroups of the gather pass resident on one CU.
+uint32_t combine_blocks_per_cu();
+// Item lists of the gather pass (tiled.hip), for g.grid_c workgroups.
+void plan_gather(Graph& g, Tiled& t, hipStream_t s);
+
 void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
 // block Gauss-Seidel solve with `phases` phases (LEAK dangling); with

@@ -103,6 +103,7 @@ struct InitArgs {
   uint32_t dangling;
   float un;
   float thr, d, res0;  // push threshold, damping, initial residual 1 - d (kDelta)
+  uint32_t npairs;     // pairs sweep 1 adds: this grid's, then zeros
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -363,6 +364,11 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
     }
   }
   block_sum2(dang, sends, s_red);
+  // the pairs past the grid's, up to the count sweep 1 adds, hold nothing
+  for (uint32_t b = gridDim.x + blockIdx.x * kBlock + threadIdx.x; b < a.npairs; b += stride) {
+    a.part0[2 * static_cast<size_t>(b)] = 0.0;
+    a.part0[2 * static_cast<size_t>(b) + 1] = 0.0;
+  }
   if (threadIdx.x == 0) {
     // "previous error" seen by sweep 1: none, or the pushes of iteration 1
     a.part0[2 * blockIdx.x] = a.delta ? sends : INFINITY;
@@ -571,16 +577,18 @@ struct CombineArgs {
   const uint32_t* __restrict__ pair_ptr;
   const uint32_t* __restrict__ rslot;
   const float* __restrict__ partial;
-  const uint32_t* __restrict__ wave_rows;
-  // the windows' row ranges and slots (pair_ptr and rslot without the wave
-  // rows past row0), and the wave-row bits (null: no wave rows past row0)
+  // the windows' row ranges and slots (pair_ptr and rslot without the big
+  // rows), and the big-row bits (null: no big rows)
   const uint32_t* __restrict__ lpp;
   const uint32_t* __restrict__ lslot;
   const uint32_t* __restrict__ heavy;
-  uint32_t nwave_rows;
-  uint32_t row0;        // rows [row_lo, row0) all have more than kWaveRowMin partials (a wave each)
-  uint32_t row_hi;      // the layout's rows end here (they start at or before row0)
-  uint32_t quiet_tail;  // 1: this pass also refreshes the quiet rows
+  const uint32_t* __restrict__ big_rows;
+  const uint32_t* __restrict__ win;  // window row boundaries
+  // items: big rows [0, nbig), windows [nbig, nbig + nwin), quiet-row blocks
+  // after them; workgroup b takes items[ptr[b] .. ptr[b + 1])
+  const uint32_t* __restrict__ ptr;
+  const uint32_t* __restrict__ items;
+  uint32_t nbig, nwin, nqb;
 };
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -637,7 +645,7 @@ constexpr uint64_t kRowLastLanes = 0x8000800080008000ull;
 // Per-wavefront LDS window through which a chunk's fragment sums leave in
 // coalesced stores.  With a 2^15-entry tile the 16 windows fill the rest of
 // the 160 KiB of LDS; a chunk with more fragments passes through it in turns.
-constexpr uint32_t kStageW = 496;
+constexpr uint32_t kStageW = kTileWindow;
 // After the windows, one slot per lane that takes the stores of running sums
 // that end no fragment, so every entry's store is unconditional (no lane-mask
 // switch per entry).  Never read.
@@ -966,11 +974,20 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
     __syncthreads();
     if (*s_flag) return;
   } else if constexpr (KIND != kCoef) {
-    // totals of the previous sweep, identical in every workgroup
+    // totals of the previous sweep, identical in every workgroup: a fixed
+    // per-thread order (four pairs in flight), a fixed butterfly, waves in order
     double E = 0, D = 0;
-    for (uint32_t b = tid; b < a.nparts; b += kSweepThreads) {
-      E += a.part_in[2 * b];
-      D += a.part_in[2 * b + 1];
+    const double2* __restrict__ pin = reinterpret_cast<const double2*>(a.part_in);
+    for (uint32_t b0 = tid; b0 < a.nparts; b0 += 4 * kSweepThreads) {
+      double2 v[4];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) v[j] = b0 + j * kSweepThreads < a.nparts ? pin[b0 + j * kSweepThreads]
+                                                                                : make_double2(0.0, 0.0);
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        E += v[j].x;
+        D += v[j].y;
+      }
     }
     if (tid == 0) *s_flag = a.st->done;
     __syncthreads();
@@ -1069,122 +1086,255 @@ __device__ __forceinline__ void quiet_rows(const SweepArgs& a, float Df, double&
   }
 }
 
-// Threads (and rows per window) of a k_combine workgroup, and the partials
-// staged in LDS per step: 512-row windows with an 8192-partial stage measured
-// fastest at R-MAT s22 (39.9 us against 41.6 for 256 / 4096, 41.0 for
-// 512 / 4096, 48.8 for 1024 / 16384).
-constexpr int kCombineThreads = 512;
-constexpr uint32_t kCombineStage = 8192;
-// Consecutive combine steps kept on one XCD.
-constexpr uint32_t kCombineRun = 8;
+// Gather form of the second pass.  One wave of resident workgroups, each
+// working through its own list of items (big rows, windows, quiet-row
+// blocks), dealt on the host by estimated cost so that the lists end
+// together.  A fixed list per workgroup keeps every (error, dangling) pair --
+// one per workgroup -- the same from run to run.
+constexpr uint32_t kCombineWaves = kCombineThreads / kWave;
+
+// Sum over the 64 lanes without LDS traffic: an inclusive scan inside each
+// 16-lane DPP row, then the four row totals (lanes 15, 31, 47, 63) added in a
+// fixed order.  Every lane receives the total.
+__device__ __forceinline__ float wave_total(float v) {
+  v += dpp_f32<kDppRowShr | 1>(v);
+  v += dpp_f32<kDppRowShr | 2>(v);
+  v += dpp_f32<kDppRowShr | 4>(v);
+  v += dpp_f32<kDppRowShr | 8>(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return (r0 + r1) + (r2 + r3);
+}
+// wavefront-summed rows of one window (more than kWaveRowMin partials each)
+constexpr uint32_t kMaxWaveRows = kCombineStage / (kWaveRowMin + 1) + 1;
+
+// Workgroup barrier for LDS only: waits for the wavefront's LDS operations,
+// not its global stores (a __syncthreads fence would hold every wavefront
+// until its rank and contribution stores were performed), and keeps the
+// compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt at their maxima (no wait)
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+struct CombineLds {
+  float buf[kCombineStage];  // the window's partials, row by row
+  uint2 hl[kMaxWaveRows];    // wave rows: (first partial - window's first, count)
+  float z[kMaxWaveRows];     // sums of the wave rows, in list order
+  float zw[kCombineWaves];   // a big row's per-wavefront sums
+  double red[2][kCombineWaves];
+  uint32_t nh, flag;
+};
+
+// A row with more than kBigRowMin partials: every thread adds a strided share
+// (eight loads in flight), wavefronts by butterfly, then their sums in order.
+template <uint32_t KIND>
+__device__ __forceinline__ void combine_big(const CombineArgs& a, uint32_t row, float Df, CombineLds& L,
+                                            double& err, double& dang) {
+  constexpr uint32_t U = 8;
+  const uint32_t tid = threadIdx.x, wl = tid & (kWave - 1), wv = tid / kWave;
+  const uint32_t b = a.pair_ptr[row], e = a.pair_ptr[row + 1];
+  float z = 0.f;
+  for (uint32_t k0 = b + tid; k0 < e; k0 += U * kCombineThreads) {
+    uint32_t sl[U];
+    float v[U];
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) {
+      const uint32_t k = k0 + j * kCombineThreads;
+      sl[j] = k < e ? a.rslot[k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) v[j] = k0 + j * kCombineThreads < e ? a.partial[sl[j]] : 0.f;
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) z += v[j];
+  }
+  z = wave_total(z);
+  if (wl == 0) L.zw[wv] = z;
+  lds_barrier();
+  if (tid == 0) {
+    float t = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < kCombineWaves; ++i) t += L.zw[i];
+    finish_row<KIND>(a.s, row, t, Df, load_row<KIND>(a.s, row), err, dang);
+  }
+}
+
+// One window (rows [win[k], win[k + 1]), kRowsPerThread per thread: rows
+// w0 + tid + j * kCombineThreads): its partials (at most kCombineStage, big
+// rows left out) are staged in LDS in one step -- all index loads of a thread
+// issued together, then all gathers, in two halves -- then a thread adds each
+// of its rows of at most kWaveRowMin partials in tile order, a wavefront each
+// longer row (lanes strided, then a DPP total).
+template <uint32_t KIND>
+__device__ __forceinline__ void combine_window(const CombineArgs& a, uint32_t wk, float Df, CombineLds& L,
+                                               double& err, double& dang) {
+  constexpr uint32_t U = kCombineStage / kCombineThreads;
+  constexpr uint32_t R = kRowsPerThread;
+  const uint32_t tid = threadIdx.x, wl = tid & (kWave - 1), wv = tid / kWave;
+  const uint32_t w0 = a.win[wk], w1 = a.win[wk + 1];
+  const uint32_t* __restrict__ lp = a.lpp;
+  const uint32_t b0 = lp[w0], e0 = lp[w1];
+  uint32_t b[R], e[R];
+  uint32_t wave_m = 0, light_m = 0;  // bit j: row j of this thread is a wave row / a light row
+  uint32_t hj[R];                     // a wave row's place in the list
+#pragma unroll
+  for (uint32_t j = 0; j < R; ++j) {
+    const uint32_t row = w0 + tid + j * kCombineThreads;
+    b[j] = e[j] = 0;
+    if (row < w1) {
+      b[j] = lp[row];
+      e[j] = lp[row + 1];
+    }
+  }
+  // two halves of U / 2 loads per thread: the second half's index loads are
+  // in flight while the first half's gathers are
+  static_assert(U % 2 == 0, "combine_window: two halves");
+  {
+    constexpr uint32_t H = U / 2;
+    uint32_t sa[H], sb[H];
+    float va[H], vb[H];
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j) {
+      const uint32_t k = b0 + tid + j * kCombineThreads;
+      sa[j] = k < e0 ? a.lslot[k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j) {
+      const uint32_t k = b0 + tid + (H + j) * kCombineThreads;
+      sb[j] = k < e0 ? a.lslot[k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j) va[j] = b0 + tid + j * kCombineThreads < e0 ? a.partial[sa[j]] : 0.f;
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j)
+      vb[j] = b0 + tid + (H + j) * kCombineThreads < e0 ? a.partial[sb[j]] : 0.f;
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j)
+      if (b0 + tid + j * kCombineThreads < e0) L.buf[tid + j * kCombineThreads] = va[j];
+#pragma unroll
+    for (uint32_t j = 0; j < H; ++j)
+      if (b0 + tid + (H + j) * kCombineThreads < e0) L.buf[tid + (H + j) * kCombineThreads] = vb[j];
+  }
+  // the rows' epilogue operands, in flight while the rows are added up
+  RowIn in[R];
+#pragma unroll
+  for (uint32_t j = 0; j < R; ++j) {
+    const uint32_t row = w0 + tid + j * kCombineThreads;
+    in[j] = row < w1 ? load_row<KIND>(a.s, row) : RowIn{};
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < R; ++j) {
+    const uint32_t row = w0 + tid + j * kCombineThreads;
+    if (row >= w1 || (a.heavy && ((a.heavy[row >> 5] >> (row & 31)) & 1u))) continue;  // big rows: items of their own
+    if (e[j] - b[j] > kWaveRowMin) {
+      wave_m |= 1u << j;
+      hj[j] = atomicAdd(&L.nh, 1u);  // list order is free: each row is summed by one wavefront
+      L.hl[hj[j]] = make_uint2(b[j] - b0, e[j] - b[j]);
+    } else {
+      light_m |= 1u << j;
+    }
+  }
+  lds_barrier();
+#pragma unroll
+  for (uint32_t j = 0; j < R; ++j) {
+    if (!((light_m >> j) & 1u)) continue;
+    float z = 0.f;
+#pragma unroll 8
+    for (uint32_t k = b[j]; k < e[j]; ++k) z += L.buf[k - b0];
+    finish_row<KIND>(a.s, w0 + tid + j * kCombineThreads, z, Df, in[j], err, dang);
+  }
+  const uint32_t nh = L.nh;
+  for (uint32_t h = wv; h < nh; h += kCombineWaves) {
+    const uint2 x = L.hl[h];
+    float z = 0.f;
+    for (uint32_t k = wl; k < x.y; k += kWave) z += L.buf[x.x + k];
+    z = wave_total(z);
+    if (wl == 0) L.z[h] = z;
+  }
+  lds_barrier();
+#pragma unroll
+  for (uint32_t j = 0; j < R; ++j)
+    if ((wave_m >> j) & 1u) finish_row<KIND>(a.s, w0 + tid + j * kCombineThreads, L.z[hj[j]], Df, in[j], err, dang);
+}
+
+// Quiet-row block q (plain kind, rows [nrows, n) without entries): the row
+// epilogue with z = 0 -- the same operations, so the value a stored row
+// without entries gets -- stored as contributions only; block 0 adds their
+// residual and dangling mass as count x value and records the common rank.
+template <uint32_t KIND>
+__device__ __forceinline__ void combine_quiet(const CombineArgs& a, uint32_t q, float Df, double& err,
+                                              double& dang) {
+  if constexpr (KIND == kPlain) {
+    const SweepArgs& s = a.s;
+    const float u = s.dangling == kDanglingUniform ? s.un : 0.f;
+    const float sq = fmaf(Df, u, 0.f);
+    const float rn = fmaf(s.d_hi, sq, fmaf(s.d_lo, sq, s.base_u));
+    const uint32_t lo = s.nrows + q * kQuietRows, hi = min(s.n, lo + kQuietRows);
+    constexpr uint32_t U = kQuietRows / kCombineThreads;  // one round of loads
+    for (uint32_t v0 = lo + threadIdx.x; v0 < hi; v0 += U * kCombineThreads) {
+      float dv[U];
+#pragma unroll
+      for (uint32_t j = 0; j < U; ++j) dv[j] = v0 + j * kCombineThreads < hi ? s.div[v0 + j * kCombineThreads] : 1.f;
+#pragma unroll
+      for (uint32_t j = 0; j < U; ++j)
+        if (v0 + j * kCombineThreads < hi) s.c_out[v0 + j * kCombineThreads] = rn / dv[j];
+    }
+    if (q == 0 && threadIdx.x == 0) {
+      const float ro = s.st->rq[s.p ^ 1];
+      s.st->rq[s.p] = rn;
+      err += static_cast<double>(s.n - s.nrows) * fabs(static_cast<double>(rn) - static_cast<double>(ro));
+      dang += static_cast<double>(s.nq_dangling) * static_cast<double>(rn);
+    }
+  }
+}
 
 template <uint32_t KIND>
-__global__ __launch_bounds__(kCombineThreads) void k_combine(CombineArgs a) {
-  __shared__ double s_red[2][kCombineThreads / 64];
-  __shared__ float s_buf[kCombineStage];
-  __shared__ uint32_t s_flag;
-  const uint32_t tid = threadIdx.x;
-  float Df = 0.f;
-  if constexpr (KIND != kCoef) {
-    if (tid == 0) s_flag = a.s.st->done;
-    __syncthreads();
-    if (s_flag) return;
-    Df = static_cast<float>(a.s.st->dsum);
+__global__ __launch_bounds__(kCombineThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_combine(CombineArgs a) {
+  __shared__ CombineLds L;
+  const uint32_t tid = threadIdx.x, wl = tid & (kWave - 1), wv = tid / kWave;
+  if (tid == 0) {
+    L.flag = KIND != kCoef ? a.s.st->done : 0u;
+    L.nh = 0;
   }
+  __syncthreads();
+  if (L.flag) return;
+  const float Df = KIND != kCoef ? static_cast<float>(a.s.st->dsum) : 0.f;
   double err = 0, dang = 0;
-  const uint32_t wl = tid & (kWave - 1);
-  const uint32_t* __restrict__ pp = a.pair_ptr;
-  // Neighbouring rows read neighbouring `rslot` and `partial` entries (every
-  // tile's partials advance with the row), so runs of kCombineRun consecutive
-  // steps go to workgroups of one XCD, the runs dealt to the XCDs in turn
-  // (heavy leading rows stay spread over all of them).
-  const uint32_t G = gridDim.x;
-  uint32_t vb = blockIdx.x;
-  if (G % (kXcds * kCombineRun) == 0) {
-    const uint32_t x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
-    vb = ((l / kCombineRun) * kXcds + x) * kCombineRun + l % kCombineRun;
-  }
-  // rows with many partials: one wavefront each (lane-strided, then a
-  // butterfly)
-  const uint32_t nwaves = G * (kCombineThreads / kWave);
-  for (uint32_t i = vb * (kCombineThreads / kWave) + tid / kWave; i < a.nwave_rows; i += nwaves) {
-    const uint32_t row = a.wave_rows[i];
-    const uint32_t b = pp[row], e = pp[row + 1];
-    const RowIn in = load_row<KIND>(a.s, row);
-    float z = 0.f;
-    // four slots per lane in flight per step
-    for (uint32_t k0 = b + wl; k0 < e; k0 += 4 * kWave) {
-      uint32_t sl[4];
-      float v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sl[j] = k0 + j * kWave < e ? a.rslot[k0 + j * kWave] : 0u;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = k0 + j * kWave < e ? a.partial[sl[j]] : 0.f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) z += v[j];
+  const uint32_t i0 = a.ptr[blockIdx.x], i1 = a.ptr[blockIdx.x + 1];
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t item = a.items[i];
+    if (item < a.nbig) {
+      combine_big<KIND>(a, a.big_rows[item], Df, L, err, dang);
+    } else if (item - a.nbig < a.nwin) {
+      combine_window<KIND>(a, item - a.nbig, Df, L, err, dang);
+    } else if (item - a.nbig - a.nwin < a.nqb) {
+      combine_quiet<KIND>(a, item - a.nbig - a.nwin, Df, err, dang);
     }
-    z = wave_sum(z);
-    if (wl == 0) finish_row<KIND>(a.s, row, z, Df, in, err, dang);
+    lds_barrier();  // every thread is done with the item's LDS
+    if (tid == 0) L.nh = 0;
+    lds_barrier();
   }
-  // The other rows, from the first row after the leading run of wave rows: a
-  // thread each, in windows of kCombineThreads consecutive rows.  A window is cut into
-  // steps whose partials fit the LDS stage (the longest prefix of its rows
-  // that fits, found with one counting barrier); each step's partials are
-  // gathered into LDS by the whole workgroup, with four loads in flight per
-  // thread, then every thread adds its row's partials in tile order.  Wave
-  // rows inside a window have empty ranges in lpp and are not finished here.
-  const uint32_t* __restrict__ lp = a.lpp;
-  const uint32_t n = min(a.s.nrows, a.row_hi);  // quiet rows [nrows, n) are not stored
-  for (uint32_t w0 = a.row0 + vb * kCombineThreads; w0 < n; w0 += G * kCombineThreads) {
-    const uint32_t w1 = min(w0 + kCombineThreads, n);
-    for (uint32_t r0 = w0; r0 < w1;) {
-      const uint32_t row = r0 + tid;
-      const bool valid = row < w1;
-      const uint32_t base = lp[r0];
-      uint32_t b = 0, e = 0;
-      bool wave = false;
-      RowIn in{};
-      if (valid) {
-        b = lp[row];
-        e = lp[row + 1];
-        wave = a.heavy && ((a.heavy[row >> 5] >> (row & 31)) & 1u);
-        in = load_row<KIND>(a.s, row);
-      }
-      // rows r0 .. r0 + nfit - 1 fit the stage (a prefix: pp is monotone)
-      uint32_t nfit = static_cast<uint32_t>(__syncthreads_count(valid && e - base <= kCombineStage));
-      const bool staged = nfit > 0;  // else row r0 alone overflows: a wave row, skipped here
-      nfit = max(nfit, 1u);
-      const uint32_t end = staged ? lp[r0 + nfit] : base;
-      for (uint32_t k0 = base + tid; k0 < end; k0 += 4 * kCombineThreads) {
-        uint32_t sl[4];
-        float v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const uint32_t k = k0 + j * kCombineThreads;
-          sl[j] = k < end ? a.lslot[k] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = k0 + j * kCombineThreads < end ? a.partial[sl[j]] : 0.f;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (k0 + j * kCombineThreads < end) s_buf[k0 + j * kCombineThreads - base] = v[j];
-      }
-      __syncthreads();
-      if (staged && tid < nfit && !wave && e - b <= kWaveRowMin) {
-        float z = 0.f;
-        for (uint32_t k = b; k < e; ++k) z += s_buf[k - base];
-        finish_row<KIND>(a.s, row, z, Df, in, err, dang);
-      }
-      __syncthreads();
-      r0 += nfit;
-    }
-  }
-  if (a.quiet_tail) quiet_rows<KIND, kCombineThreads>(a.s, Df, err, dang);
   if constexpr (KIND != kCoef) {
-    block_sum2<kCombineThreads>(err, dang, s_red);
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    if (wl == 0) {
+      L.red[0][wv] = err;
+      L.red[1][wv] = dang;
+    }
+    lds_barrier();
     if (tid == 0) {
-      a.s.part_out[2 * blockIdx.x] = err;
-      a.s.part_out[2 * blockIdx.x + 1] = dang;
+      double te = 0, td = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kCombineWaves; ++k) {
+        te += L.red[0][k];
+        td += L.red[1][k];
+      }
+      a.s.part_out[2 * blockIdx.x] = te;
+      a.s.part_out[2 * blockIdx.x + 1] = td;
     }
   }
 }
@@ -1375,7 +1525,7 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
     // block Gauss-Seidel: contributions updated in place, phase by phase;
     // pairs in blocks of grid_c per phase
     a.c_in = a.c_out = g.c[0].as<float>();
-    const size_t stride = 2 * static_cast<size_t>(g.grid_c) * sp.phases->size();
+    const size_t stride = 2 * static_cast<size_t>(g.gs_stride);
     a.part_in = g.gs_part.as<double>() + stride * (parity ^ 1);
     a.part_out = g.gs_part.as<double>() + stride * parity;
   } else {
@@ -1436,6 +1586,27 @@ void launch_sweep(uint32_t kind, uint32_t grid, const SweepArgs& a, hipStream_t 
   }
 }
 
+}  // namespace
+
+uint32_t solve_pairs(const Graph& g, const SolveSpec& sp) {
+  if (sp.phases) return sp.only >= 0 ? g.grid_c : g.grid_c * static_cast<uint32_t>(sp.phases->size());
+  return sp.tiled ? g.grid_c : g.grid;
+}
+
+uint32_t combine_blocks_per_cu() {
+  static const uint32_t v = [] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_combine<kPlain>),
+                                                     kCombineThreads, 0) != hipSuccess ||
+        nb < 1)
+      nb = 2;
+    return static_cast<uint32_t>(nb);
+  }();
+  return v;
+}
+
+namespace {
+
 // One tiled pass pair over layout `t`: k_tile over the workgroup task ranges,
 // then the second pass (k_combine, or k_accum in the accumulate form).
 // `check`: the tile pass takes the stop decision from the `nparts` pairs
@@ -1477,14 +1648,18 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ca.pair_ptr = t.pair_ptr.as<uint32_t>();
   ca.rslot = t.rslot.as<uint32_t>();
   ca.partial = t.partial.as<float>();
-  ca.wave_rows = t.wave_rows.as<uint32_t>();
   ca.lpp = t.light_split ? t.lpp.as<uint32_t>() : ca.pair_ptr;
   ca.lslot = t.light_split ? t.lslot.as<uint32_t>() : ca.rslot;
   ca.heavy = t.light_split ? t.heavy.as<uint32_t>() : nullptr;
-  ca.nwave_rows = t.nwave_rows;
-  ca.row0 = t.row0;
-  ca.row_hi = t.row_hi;
-  ca.quiet_tail = t.quiet_tail ? 1u : 0u;
+  ca.big_rows = t.big_rows.as<uint32_t>();
+  ca.win = t.win.as<uint32_t>();
+  NWGR_REQUIRE(t.gplan_blocks == g.grid_c, "gather plan made for another grid");
+  const Tiled::GatherPlan& gp = t.gplan[gather_plan_index(t, kind, sa.quiet != 0)];
+  ca.ptr = gp.ptr.as<uint32_t>();
+  ca.items = gp.items.as<uint32_t>();
+  ca.nbig = t.nbig;
+  ca.nwin = gp.nwin;
+  ca.nqb = kind == kPlain ? gp.nqb : 0u;
   const size_t shm = tile_lds_bytes(t.bits);
   const uint32_t nb = t.nblocks;
   if (t.acc_on) {
@@ -1561,14 +1736,15 @@ void launch_tiled(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_
 // block, and the next sweep's first tile pass adds all of them.
 void launch_gs(Graph& g, const SolveSpec& sp, const SweepArgs& sa, hipStream_t s) {
   const uint32_t P = static_cast<uint32_t>(sp.phases->size());
+  const uint32_t total = solve_pairs(g, sp);
   if (sp.only >= 0) {  // one row share alone: its pairs are the sweep's
-    launch_layout(g, sp.kind, (*sp.phases)[sp.only], nullptr, sa, 1u, g.grid_c, s);
+    launch_layout(g, sp.kind, (*sp.phases)[sp.only], nullptr, sa, 1u, total, s);
     return;
   }
   for (uint32_t q = 0; q < P; ++q) {
     SweepArgs aq = sa;
     aq.part_out = sa.part_out + 2 * static_cast<size_t>(g.grid_c) * q;
-    launch_layout(g, sp.kind, (*sp.phases)[q], nullptr, aq, q == 0 ? 1u : 0u, P * g.grid_c, s);
+    launch_layout(g, sp.kind, (*sp.phases)[q], nullptr, aq, q == 0 ? 1u : 0u, total, s);
   }
 }
 
@@ -1682,6 +1858,19 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     g.tel.alloc(sizeof(float) * g.n);
     g.drop_graphs();
   }
+  // (error, dangling) pairs of one sweep; init writes as many
+  const uint32_t npairs = solve_pairs(g, sp);
+  if (sp.phases) {
+    if (g.gs_stride < npairs || !g.gs_part.p) {
+      g.gs_stride = npairs;
+      g.gs_part.alloc(sizeof(double) * 2 * 2 * static_cast<size_t>(npairs));
+      g.drop_graphs();
+    }
+  } else if (g.part_stride < npairs || !g.partials.p) {
+    g.part_stride = std::max(npairs, g.part_stride);
+    g.partials.alloc(sizeof(double) * 2 * 2 * static_cast<size_t>(g.part_stride));
+    g.drop_graphs();
+  }
 
   // the captured graphs bake in the arguments and the stream
   uint64_t key = 0xcbf29ce484222325ULL, bits;
@@ -1703,8 +1892,10 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
                           static_cast<const void*>(sp.gsrc), static_cast<const void*>(sp.plan),
                           static_cast<const void*>(sp.tiled), static_cast<const void*>(sp.tw),
                           sp.phases ? static_cast<const void*>(sp.phases->data()) : nullptr,
-                          static_cast<const void*>(g.gs_part.p)})
+                          static_cast<const void*>(g.gs_part.p), static_cast<const void*>(g.partials.p)})
     key = fnv_mix(key, reinterpret_cast<uintptr_t>(ptr));
+  key = fnv_mix(key, g.part_stride);
+  key = fnv_mix(key, g.gs_stride);
   key = fnv_mix(key, sp.phases ? sp.phases->size() : 0);
   key = fnv_mix(key, static_cast<uint64_t>(sp.only + 1));
   const uint32_t slot = sp.phases ? kExecGs : sp.kind;
@@ -1738,7 +1929,8 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.thr = sp.thr;
   ia.d = static_cast<float>(p.damping);
   ia.res0 = static_cast<float>(1.0 - static_cast<double>(ia.d));
-  k_init<<<sp.tiled ? g.grid_c : g.grid, kBlock, 0, s>>>(ia);
+  ia.npairs = npairs;
+  k_init<<<std::min(sp.tiled ? g.grid_c : g.grid, npairs), kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());
 
   // Chunks of captured sweeps, at most two in flight.  Launch j performs
@@ -1938,8 +2130,6 @@ void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nw
   // quiet rows are kept as the scalar only where the last phase refreshes them
   sp.quiet = !p.teleport && !p.init && g.nz < g.n && (only < 0 || only + 1 == static_cast<int>(phases));
   if (sp.quiet) prepare_quiet(g, s);
-  // blocks of pairs the first sweep's first phase adds beside init's block 0
-  NWGR_HIP(hipMemsetAsync(g.gs_part.p, 0, g.gs_part.bytes, s));
   solve(g, sp, p, res, s);
   k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
       g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n,

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kT) void k_count_runs(const uint32_t* __restrict__ 
   if (wl == 0 && cnt) atomicAdd(runs, static_cast<unsigned long long>(cnt));
 }
 
-// The slots of the rows that are not wave rows, in row order (lslot).
+// The slots of the rows that are not big rows, in row order (lslot).
 __global__ void k_light_slots(const uint32_t* __restrict__ pp, const uint32_t* __restrict__ lpp,
                               const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ rslot, uint32_t row_lo,
                               uint32_t row_hi, uint32_t* __restrict__ lslot) {
@@ -255,11 +255,33 @@ uint32_t tile_blocks(const Graph& g) {
 // Cut the chunk sequence (tile-major) into one contiguous range per
 // workgroup, each split at tile boundaries into tasks.  A chunk costs
 // 1 + fragments / kFragsPerChunkCost (each fragment adds a partial to reduce,
-// stage and store) and every task `fill_cost` more (it copies its tile into
-// LDS and refills the load pipeline).  The greedy cut at a cost target is
-// bisected to the smallest target that needs at most G ranges.
+// stage and store), plus kSecondWindowCost when its fragments need a second
+// pass through the wavefront's LDS window and kThirdWindowCost when they need
+// a third; every task costs `fill_cost` more (it copies its tile into LDS and
+// refills the load pipeline).  The constants are a least-squares fit of the
+// workgroups' measured chunk-loop times at R-MAT s22 (256 workgroups: a chunk
+// whose 1024 entries each start a fragment costs 3.3 plain chunks).  The
+// greedy cut at a cost target is bisected to the smallest target that needs
+// at most G ranges.
 namespace {
-constexpr double kFragsPerChunkCost = 800.0;
+constexpr double kFragsPerChunkCost = 1200.0;
+constexpr double kSecondWindowCost = 0.36;
+constexpr double kThirdWindowCost = 0.93;
+
+// Estimated cost (us) of the gather pass's items: windows by partials, rows
+// and rows a wavefront adds, quiet-row blocks by rows.  The constants are
+// those that, dealing the items measured at R-MAT s22 (one sweep, every
+// item's time) by this model, gave the shortest slowest list.
+float window_cost(uint32_t partials, uint32_t rows, uint32_t wave_rows) {
+  return 1.0f + 3.5e-3f * partials + 6.0e-3f * rows + 0.2f * wave_rows;
+}
+float big_row_cost(uint32_t partials) { return 2.0f + 3.5e-3f * partials; }
+float quiet_block_cost(uint32_t rows) { return 0.5f + 6.0e-4f * rows; }
+
+double chunk_cost(uint32_t frags) {
+  return 1.0 + frags / kFragsPerChunkCost + (frags > kTileWindow ? kSecondWindowCost : 0.0) +
+         (frags > 2 * kTileWindow ? kThirdWindowCost : 0.0);
+}
 
 struct Cut {
   std::vector<TiledTask> tasks;
@@ -320,12 +342,12 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
     NWGR_HIP(hipGetLastError());
     NWGR_HIP(hipMemcpyAsync(fr.data(), d.p, sizeof(uint32_t) * t.nchunks, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
-    for (uint32_t c = 0; c < t.nchunks; ++c) pc[c + 1] = pc[c] + 1.0 + fr[c] / kFragsPerChunkCost;
+    for (uint32_t c = 0; c < t.nchunks; ++c) pc[c + 1] = pc[c] + chunk_cost(fr[c]);
   }
   uint32_t busy = 0;
   for (uint32_t tile = 0; tile < t.ntiles; ++tile) busy += t.tile_chunk0[tile + 1] > t.tile_chunk0[tile];
   // one range holding everything always fits
-  double hi = pc.back() + F * busy + F + 2 * (1.0 + kChunkEdges / kFragsPerChunkCost);
+  double hi = pc.back() + F * busy + F + 2 * chunk_cost(kChunkEdges);
   double lo = std::max(1.0, (pc.back() + F * busy) / G);
   Cut best = greedy_cut(t, pc, F, hi);
   for (int it = 0; it < 48 && hi - lo > 0.25; ++it) {
@@ -466,26 +488,26 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
                     nullptr, t.rslot.as<uint32_t>(), s);
   t.partial.alloc(sizeof(float) * std::max<uint32_t>(nsub, 1));
   NWGR_HIP(hipMemsetAsync(t.partial.p, 0, t.partial.bytes, s));
-  // rows whose partials a wavefront adds
+  // rows whose partials the whole workgroup adds, each an item of its own
   std::vector<uint32_t> pp(static_cast<size_t>(n) + 1);
   NWGR_HIP(hipMemcpyAsync(pp.data(), t.pair_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, s));
   NWGR_HIP(hipStreamSynchronize(s));
-  std::vector<uint32_t> wave_rows;
+  std::vector<uint32_t> big;
   for (uint32_t r = row_lo; r < row_hi; ++r)
-    if (pp[r + 1] - pp[r] > kWaveRowMin) wave_rows.push_back(r);
-  t.row0 = row_lo;
-  while (t.row0 - row_lo < wave_rows.size() && wave_rows[t.row0 - row_lo] == t.row0) ++t.row0;
-  t.nwave_rows = static_cast<uint32_t>(wave_rows.size());
-  t.wave_rows.alloc(sizeof(uint32_t) * std::max<size_t>(wave_rows.size(), 1));
-  if (!wave_rows.empty())
-    NWGR_HIP(hipMemcpyAsync(t.wave_rows.p, wave_rows.data(), sizeof(uint32_t) * wave_rows.size(),
-                            hipMemcpyHostToDevice, s));
-  // wave rows past row0: the windows of the gather pass skip them
-  t.light_split = t.nwave_rows > t.row0 - row_lo;
+    if (pp[r + 1] - pp[r] > kBigRowMin) big.push_back(r);
+  t.nbig = static_cast<uint32_t>(big.size());
+  t.big_rows.alloc(sizeof(uint32_t) * std::max<size_t>(big.size(), 1));
+  if (!big.empty())
+    NWGR_HIP(hipMemcpyAsync(t.big_rows.p, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice, s));
+  // the windows skip the big rows
+  std::vector<uint32_t> hb;
+  t.light_split = !big.empty();
+  std::vector<uint32_t> lpp;
   if (t.light_split) {
-    std::vector<uint32_t> lpp(static_cast<size_t>(n) + 1), hb((static_cast<size_t>(n) + 31) / 32, 0u);
+    lpp.resize(static_cast<size_t>(n) + 1);
+    hb.assign((static_cast<size_t>(n) + 31) / 32, 0u);
     uint64_t acc = 0;
-    for (uint32_t r : wave_rows) hb[r >> 5] |= 1u << (r & 31);
+    for (uint32_t r : big) hb[r >> 5] |= 1u << (r & 31);
     for (uint32_t r = 0; r < n; ++r) {
       lpp[r] = static_cast<uint32_t>(acc);
       if (!((hb[r >> 5] >> (r & 31)) & 1u)) acc += pp[r + 1] - pp[r];
@@ -502,10 +524,90 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
                                                              row_hi, t.lslot.as<uint32_t>());
       NWGR_HIP(hipGetLastError());
     }
+  }
+  const std::vector<uint32_t>& wp = t.light_split ? lpp : pp;
+  // windows: greedy runs of consecutive rows, cut before kWindowRows rows or
+  // kCombineStage partials would be passed, and at nz
+  t.win_h.assign(1, row_lo);
+  for (uint32_t r = row_lo; r < row_hi; ++r) {
+    const uint32_t w0 = t.win_h.back();
+    if (r > w0 && (r - w0 >= kWindowRows || wp[r + 1] - wp[w0] > kCombineStage || r == g.nz)) t.win_h.push_back(r);
+  }
+  if (row_hi > row_lo) t.win_h.push_back(row_hi);
+  t.nwin = static_cast<uint32_t>(t.win_h.size() - 1);
+  t.win.alloc(sizeof(uint32_t) * t.win_h.size());
+  NWGR_HIP(hipMemcpyAsync(t.win.p, t.win_h.data(), sizeof(uint32_t) * t.win_h.size(), hipMemcpyHostToDevice, s));
+  t.big_cost.resize(big.size());
+  for (size_t i = 0; i < big.size(); ++i) t.big_cost[i] = big_row_cost(pp[big[i] + 1] - pp[big[i]]);
+  t.win_cost.resize(t.nwin);
+  for (uint32_t k = 0; k < t.nwin; ++k) {
+    const uint32_t w0 = t.win_h[k], w1 = t.win_h[k + 1];
+    uint32_t wave_rows = 0;
+    for (uint32_t r = w0; r < w1; ++r) wave_rows += wp[r + 1] - wp[r] > kWaveRowMin;
+    t.win_cost[k] = window_cost(wp[w1] - wp[w0], w1 - w0, wave_rows);
+  }
+  NWGR_HIP(hipStreamSynchronize(s));  // the host vectors must outlive the copies
+  plan_tasks(g, t, s);
+  plan_gather(g, t, s);
+  t.built = true;
+}
+
+// Item lists of the gather pass: each item goes, heaviest first, to the
+// workgroup whose list is cheapest so far (ties to the lower index), and each
+// list is worked through in the order it was dealt.
+void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
+  const uint32_t G = g.grid_c;
+  const uint32_t quiet_rows = g.n - std::min(g.nz, g.n);
+  for (uint32_t mode = 0; mode < 2; ++mode) {
+    Tiled::GatherPlan& gp = t.gplan[mode];
+    // mode 1: quiet rows [nz, n) kept as the scalar: windows that start
+    // before nz, plus the quiet-row blocks when this layout refreshes them
+    const uint32_t stop = mode ? std::min(g.nz, t.row_hi) : t.row_hi;
+    gp.nwin = static_cast<uint32_t>(std::lower_bound(t.win_h.begin(), t.win_h.begin() + t.nwin, stop) -
+                                    t.win_h.begin());
+    gp.nqb = mode && t.quiet_tail && quiet_rows ? (quiet_rows + kQuietRows - 1) / kQuietRows : 0;
+    const uint32_t nitems = t.nbig + gp.nwin + gp.nqb;
+    std::vector<std::pair<float, uint32_t>> items(nitems);
+    for (uint32_t i = 0; i < nitems; ++i) {
+      float c;
+      if (i < t.nbig)
+        c = t.big_cost[i];
+      else if (i - t.nbig < gp.nwin)
+        c = t.win_cost[i - t.nbig];
+      else
+        c = quiet_block_cost(std::min(kQuietRows, quiet_rows - (i - t.nbig - gp.nwin) * kQuietRows));
+      items[i] = {c, i};
+    }
+    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    // (load, workgroup) min-heap
+    std::vector<std::pair<double, uint32_t>> heap;
+    heap.reserve(G);
+    for (uint32_t b = 0; b < G; ++b) heap.push_back({0.0, b});
+    auto cmp = [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second > b.second); };
+    std::make_heap(heap.begin(), heap.end(), cmp);
+    std::vector<std::vector<uint32_t>> lists(G);
+    for (const auto& it : items) {
+      std::pop_heap(heap.begin(), heap.end(), cmp);
+      auto& h = heap.back();
+      lists[h.second].push_back(it.second);
+      h.first += it.first;
+      std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    std::vector<uint32_t> ptr(static_cast<size_t>(G) + 1, 0), flat;
+    flat.reserve(nitems);
+    for (uint32_t b = 0; b < G; ++b) {
+      ptr[b] = static_cast<uint32_t>(flat.size());
+      flat.insert(flat.end(), lists[b].begin(), lists[b].end());
+    }
+    ptr[G] = static_cast<uint32_t>(flat.size());
+    gp.ptr.alloc(sizeof(uint32_t) * ptr.size());
+    gp.items.alloc(sizeof(uint32_t) * std::max<size_t>(flat.size(), 1));
+    NWGR_HIP(hipMemcpyAsync(gp.ptr.p, ptr.data(), sizeof(uint32_t) * ptr.size(), hipMemcpyHostToDevice, s));
+    if (!flat.empty())
+      NWGR_HIP(hipMemcpyAsync(gp.items.p, flat.data(), sizeof(uint32_t) * flat.size(), hipMemcpyHostToDevice, s));
     NWGR_HIP(hipStreamSynchronize(s));  // the host vectors must outlive the copies
   }
-  plan_tasks(g, t, s);
-  t.built = true;
+  t.gplan_blocks = G;
 }
 
 void build_accum(Graph& g, Tiled& t, hipStream_t s) {
@@ -644,7 +746,6 @@ void ensure_gs(Graph& g, uint32_t phases, hipStream_t s) {
   g.gs = std::move(layouts);
   g.gs_cuts = cuts;
   g.gs_bits = g.tune.tile_bits;
-  g.gs_part.alloc(sizeof(double) * 2 * 2 * static_cast<size_t>(phases) * g.grid_c);
 }
 
 Tiled* select_tiled(Graph& g, bool adj, hipStream_t s) {
