@@ -273,6 +273,8 @@ __host__ __device__ constexpr uint64_t wgt_pos(uint64_t P) {
 static_assert(eloc_pos(16 + 9) == 512 + 8 + 1 && wgt_pos(16 + 9) == 512 + 4 + 1 && eloc_pos(1024 + 7) == 1024 + 7,
               "chunk storage order");
 
+constexpr uint32_t kNoTile = 0xffffffffu;
+
 struct TiledTask {
   uint32_t tile;
   uint32_t chunk_lo, chunk_hi;  // chunks [chunk_lo, chunk_hi) of this tile
@@ -298,7 +300,7 @@ struct Tiled {
   DevBuf partial;    // f32[nsub] one partial sum per fragment ordinal
   DevBuf tasks;      // TiledTask[ntasks]
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
-  DevBuf big_rows;   // u32[nbig] rows with more than kBigRowMin partials, ascending
+  DevBuf wg_tile0;   // u32[nblocks] tile of workgroup b's first task (kNoTile: it has none)
   // Big rows are left out of the windows of the gather pass: lpp and lslot are
   // pair_ptr and rslot over the other rows only (a big row's range empty), and
   // heavy marks the big rows (a bit per row).  Without big rows (light_split
@@ -307,23 +309,26 @@ struct Tiled {
   DevBuf lpp;        // u32[n + 1]
   DevBuf lslot;      // u32[fragments of the other rows]
   DevBuf heavy;      // u32[(n + 31) / 32]
-  // Windows of the gather pass: row boundaries win[0] = row_lo < ... <
-  // win[nwin] = row_hi; no window crosses nz (so a solve whose quiet rows are
-  // kept as a scalar takes the windows that start before nz).
+  // Windows of the gather pass (host): row boundaries win_h[0] = row_lo < ...
+  // < win_h[nwin] = row_hi; no window crosses nz (so a solve whose quiet rows
+  // are kept as a scalar takes the windows that start before nz).
   uint32_t nwin = 0;
-  DevBuf win;                     // u32[nwin + 1]
-  std::vector<uint32_t> win_h;    // host copy
+  std::vector<uint32_t> win_h;
   // Item lists of the gather pass for [0] a sweep that stores every row and
   // [1] one that keeps the quiet rows as the scalar: items are numbered big
   // rows, then windows, then quiet-row blocks; workgroup b takes
-  // items[ptr[b] .. ptr[b + 1]), planned for gplan_blocks workgroups.
+  // items[ptr[b] .. ptr[b + 1]), each with its descriptor (GatherDesc),
+  // planned for gplan_blocks workgroups.
   struct GatherPlan {
     uint32_t nwin = 0, nqb = 0;
-    DevBuf ptr, items;
+    DevBuf ptr, items, desc;
   };
   GatherPlan gplan[2];
   uint32_t gplan_blocks = 0;
-  std::vector<float> big_cost, win_cost;  // host: estimated cost of each big row and window
+  // host: estimated cost of each big row and window; big rows with their
+  // pair_ptr bounds; each window boundary's position in lpp
+  std::vector<float> big_cost, win_cost;
+  std::vector<uint32_t> big_h, big_pp, win_b;
   std::vector<uint32_t> tile_chunk0;  // host: first chunk of each tile (ntiles + 1)
   // accumulate form of the second pass (built on first use): rows cut into
   // blocks of at most kAccRows rows and about kAccFrags fragments; inside a
@@ -495,8 +500,8 @@ struct Graph {
 inline uint64_t bytes_of(const Tiled& t) {
   uint64_t b = 0;
   for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
-                          &t.wg_task, &t.big_rows, &t.win, &t.gplan[0].ptr, &t.gplan[0].items,
-                          &t.gplan[1].ptr, &t.gplan[1].items, &t.ablo, &t.aseg, &t.afrow, &t.lpp,
+                          &t.wg_task, &t.wg_tile0, &t.gplan[0].ptr, &t.gplan[0].items,
+                          &t.gplan[1].ptr, &t.gplan[1].items, &t.gplan[0].desc, &t.gplan[1].desc, &t.ablo, &t.aseg, &t.afrow, &t.lpp,
                           &t.lslot, &t.heavy})
     b += d->bytes;
   return b;
@@ -585,6 +590,14 @@ inline size_t iter_len(uint32_t n) {
 // solves (pagerank.hip, weighted.hip)
 // Runs the chunked fixed-point iteration; the final iterate is left in
 // g.r[res.iters & 1] (new labels).
+// What an item of the gather pass needs before its first vector load, read
+// with its item number: a window's rows [x, y) and their partials [z, w) in
+// lpp order; a big row x with its partials [y, z) in pair_ptr order; a
+// quiet-row block's index x.
+struct GatherDesc {
+  uint32_t x, y, z, w;
+};
+
 // The gather plan a sweep of this kind takes (1: quiet rows kept as the
 // scalar, which only the plain kind does).
 inline uint32_t gather_plan_index(const Tiled& t, uint32_t kind, bool quiet) {

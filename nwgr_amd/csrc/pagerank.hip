@@ -556,6 +556,7 @@ struct TileArgs {
   const uint32_t* __restrict__ lane_sp0;  // read at lane 0 of each chunk: the chunk's first ordinal
   const TiledTask* __restrict__ tasks;
   const uint32_t* __restrict__ wg_task;
+  const uint32_t* __restrict__ wg_tile0;  // tile of each workgroup's first task (kNoTile: no task)
   const float* __restrict__ c_in;  // padded to whole tiles
   float* __restrict__ partial;
   const double* __restrict__ part_in;
@@ -582,12 +583,11 @@ struct CombineArgs {
   const uint32_t* __restrict__ lpp;
   const uint32_t* __restrict__ lslot;
   const uint32_t* __restrict__ heavy;
-  const uint32_t* __restrict__ big_rows;
-  const uint32_t* __restrict__ win;  // window row boundaries
   // items: big rows [0, nbig), windows [nbig, nbig + nwin), quiet-row blocks
   // after them; workgroup b takes items[ptr[b] .. ptr[b + 1])
   const uint32_t* __restrict__ ptr;
   const uint32_t* __restrict__ items;
+  const GatherDesc* __restrict__ desc;  // per list entry
   uint32_t nbig, nwin, nqb;
 };
 
@@ -967,6 +967,19 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   // the task range, the stop flag and the previous sweep's pairs are loaded
   // together: one memory round trip before the first tile, not three
   const uint32_t t_lo = a.wg_task[vb], t_hi = a.wg_task[vb + 1];
+  // the first tile's contributions are loaded now, in flight during the stop
+  // test, and written to LDS after it
+  const uint32_t mask = (1u << a.bits) - 1;
+  const uint32_t nq = (mask + 1) / 4;  // float4 per tile
+  constexpr uint32_t kJ = kMaxTile / 4 / kSweepThreads;
+  const uint32_t tile0 = a.wg_tile0[vb];
+  float4 f0[kJ];
+  if (tile0 != kNoTile) {
+    const float4* src = reinterpret_cast<const float4*>(a.c_in + (static_cast<size_t>(tile0) << a.bits));
+#pragma unroll
+    for (uint32_t j = 0; j < kJ; ++j)
+      f0[j] = tid + j * kSweepThreads < nq ? src[tid + j * kSweepThreads] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   if (KIND != kCoef && !a.check) {
     // a later phase of a block Gauss-Seidel sweep: the sweep's first tile
     // pass took the stop decision
@@ -1035,16 +1048,20 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
     }
   }
 
-  const uint32_t mask = (1u << a.bits) - 1;
-  const uint32_t nq = (mask + 1) / 4;  // float4 per tile
   // LDS address of the tile (held in a register: a constant LDS base would be
   // added again at every access)
   uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_f32*)s_tile));
   __asm__("" : "+s"(lds0));
+  if (tile0 != kNoTile) {
+    float4* dst = reinterpret_cast<float4*>(s_tile);
+#pragma unroll
+    for (uint32_t j = 0; j < kJ; ++j)
+      if (tid + j * kSweepThreads < nq) dst[tid + j * kSweepThreads] = f0[j];
+  }
   for (uint32_t ti = t_lo; ti < t_hi; ++ti) {
     const TiledTask task = a.tasks[ti];
-    if (ti > t_lo) __syncthreads();  // every wave is done with the previous tile
-    {
+    if (ti > t_lo) {
+      __syncthreads();  // every wave is done with the previous tile
       const float4* src = reinterpret_cast<const float4*>(a.c_in + (static_cast<size_t>(task.tile) << a.bits));
       float4* dst = reinterpret_cast<float4*>(s_tile);
       if (nq == kMaxTile / 4)
@@ -1133,11 +1150,11 @@ struct CombineLds {
 // A row with more than kBigRowMin partials: every thread adds a strided share
 // (eight loads in flight), wavefronts by butterfly, then their sums in order.
 template <uint32_t KIND>
-__device__ __forceinline__ void combine_big(const CombineArgs& a, uint32_t row, float Df, CombineLds& L,
+__device__ __forceinline__ void combine_big(const CombineArgs& a, const GatherDesc& d, float Df, CombineLds& L,
                                             double& err, double& dang) {
   constexpr uint32_t U = 8;
   const uint32_t tid = threadIdx.x, wl = tid & (kWave - 1), wv = tid / kWave;
-  const uint32_t b = a.pair_ptr[row], e = a.pair_ptr[row + 1];
+  const uint32_t row = d.x, b = d.y, e = d.z;
   float z = 0.f;
   for (uint32_t k0 = b + tid; k0 < e; k0 += U * kCombineThreads) {
     uint32_t sl[U];
@@ -1170,14 +1187,13 @@ __device__ __forceinline__ void combine_big(const CombineArgs& a, uint32_t row, 
 // of its rows of at most kWaveRowMin partials in tile order, a wavefront each
 // longer row (lanes strided, then a DPP total).
 template <uint32_t KIND>
-__device__ __forceinline__ void combine_window(const CombineArgs& a, uint32_t wk, float Df, CombineLds& L,
+__device__ __forceinline__ void combine_window(const CombineArgs& a, const GatherDesc& d, float Df, CombineLds& L,
                                                double& err, double& dang) {
   constexpr uint32_t U = kCombineStage / kCombineThreads;
   constexpr uint32_t R = kRowsPerThread;
   const uint32_t tid = threadIdx.x, wl = tid & (kWave - 1), wv = tid / kWave;
-  const uint32_t w0 = a.win[wk], w1 = a.win[wk + 1];
+  const uint32_t w0 = d.x, w1 = d.y, b0 = d.z, e0 = d.w;
   const uint32_t* __restrict__ lp = a.lpp;
-  const uint32_t b0 = lp[w0], e0 = lp[w1];
   uint32_t b[R], e[R];
   uint32_t wave_m = 0, light_m = 0;  // bit j: row j of this thread is a wave row / a light row
   uint32_t hj[R];                     // a wave row's place in the list
@@ -1305,15 +1321,22 @@ __global__ __launch_bounds__(kCombineThreads) __attribute__((amdgpu_waves_per_eu
   const float Df = KIND != kCoef ? static_cast<float>(a.s.st->dsum) : 0.f;
   double err = 0, dang = 0;
   const uint32_t i0 = a.ptr[blockIdx.x], i1 = a.ptr[blockIdx.x + 1];
+  // each entry's item number and descriptor are read one entry ahead
+  uint32_t item = i0 < i1 ? a.items[i0] : 0u;
+  GatherDesc d = i0 < i1 ? a.desc[i0] : GatherDesc{};
   for (uint32_t i = i0; i < i1; ++i) {
-    const uint32_t item = a.items[i];
+    const uint32_t nj = min(i + 1, i1 - 1);
+    const uint32_t nitem = a.items[nj];
+    const GatherDesc nd = a.desc[nj];
     if (item < a.nbig) {
-      combine_big<KIND>(a, a.big_rows[item], Df, L, err, dang);
+      combine_big<KIND>(a, d, Df, L, err, dang);
     } else if (item - a.nbig < a.nwin) {
-      combine_window<KIND>(a, item - a.nbig, Df, L, err, dang);
+      combine_window<KIND>(a, d, Df, L, err, dang);
     } else if (item - a.nbig - a.nwin < a.nqb) {
-      combine_quiet<KIND>(a, item - a.nbig - a.nwin, Df, err, dang);
+      combine_quiet<KIND>(a, d.x, Df, err, dang);
     }
+    item = nitem;
+    d = nd;
     lds_barrier();  // every thread is done with the item's LDS
     if (tid == 0) L.nh = 0;
     lds_barrier();
@@ -1630,6 +1653,7 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ta.lane_sp0 = t.lane_sp0.as<uint32_t>();
   ta.tasks = t.tasks.as<TiledTask>();
   ta.wg_task = t.wg_task.as<uint32_t>();
+  ta.wg_tile0 = t.wg_tile0.as<uint32_t>();
   ta.c_in = sa.c_in;
   ta.partial = t.partial.as<float>();
   ta.part_in = sa.part_in;
@@ -1651,12 +1675,11 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ca.lpp = t.light_split ? t.lpp.as<uint32_t>() : ca.pair_ptr;
   ca.lslot = t.light_split ? t.lslot.as<uint32_t>() : ca.rslot;
   ca.heavy = t.light_split ? t.heavy.as<uint32_t>() : nullptr;
-  ca.big_rows = t.big_rows.as<uint32_t>();
-  ca.win = t.win.as<uint32_t>();
   NWGR_REQUIRE(t.gplan_blocks == g.grid_c, "gather plan made for another grid");
   const Tiled::GatherPlan& gp = t.gplan[gather_plan_index(t, kind, sa.quiet != 0)];
   ca.ptr = gp.ptr.as<uint32_t>();
   ca.items = gp.items.as<uint32_t>();
+  ca.desc = gp.desc.as<GatherDesc>();
   ca.nbig = t.nbig;
   ca.nwin = gp.nwin;
   ca.nqb = kind == kPlain ? gp.nqb : 0u;

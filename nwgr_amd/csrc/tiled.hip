@@ -264,9 +264,9 @@ uint32_t tile_blocks(const Graph& g) {
 // greedy cut at a cost target is bisected to the smallest target that needs
 // at most G ranges.
 namespace {
-constexpr double kFragsPerChunkCost = 1200.0;
-constexpr double kSecondWindowCost = 0.36;
-constexpr double kThirdWindowCost = 0.93;
+constexpr double kFragsPerChunkCost = 1500.0;
+constexpr double kSecondWindowCost = 0.48;
+constexpr double kThirdWindowCost = 1.03;
 
 // Estimated cost (us) of the gather pass's items: windows by partials, rows
 // and rows a wavefront adds, quiet-row blocks by rows.  The constants are
@@ -368,6 +368,11 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
   t.fill_cost = g.tune.tile_fill_cost;
   t.tasks.alloc(sizeof(TiledTask) * std::max<size_t>(tasks.size(), 1));
   t.wg_task.alloc(sizeof(uint32_t) * (G + 1));
+  std::vector<uint32_t> tile0(G, kNoTile);
+  for (uint32_t b = 0; b < G; ++b)
+    if (wg[b] < wg[b + 1]) tile0[b] = tasks[wg[b]].tile;
+  t.wg_tile0.alloc(sizeof(uint32_t) * G);
+  NWGR_HIP(hipMemcpyAsync(t.wg_tile0.p, tile0.data(), sizeof(uint32_t) * G, hipMemcpyHostToDevice, s));
   if (!tasks.empty())
     NWGR_HIP(hipMemcpyAsync(t.tasks.p, tasks.data(), sizeof(TiledTask) * tasks.size(), hipMemcpyHostToDevice, s));
   NWGR_HIP(hipMemcpyAsync(t.wg_task.p, wg.data(), sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, s));
@@ -496,9 +501,6 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   for (uint32_t r = row_lo; r < row_hi; ++r)
     if (pp[r + 1] - pp[r] > kBigRowMin) big.push_back(r);
   t.nbig = static_cast<uint32_t>(big.size());
-  t.big_rows.alloc(sizeof(uint32_t) * std::max<size_t>(big.size(), 1));
-  if (!big.empty())
-    NWGR_HIP(hipMemcpyAsync(t.big_rows.p, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice, s));
   // the windows skip the big rows
   std::vector<uint32_t> hb;
   t.light_split = !big.empty();
@@ -535,10 +537,16 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   }
   if (row_hi > row_lo) t.win_h.push_back(row_hi);
   t.nwin = static_cast<uint32_t>(t.win_h.size() - 1);
-  t.win.alloc(sizeof(uint32_t) * t.win_h.size());
-  NWGR_HIP(hipMemcpyAsync(t.win.p, t.win_h.data(), sizeof(uint32_t) * t.win_h.size(), hipMemcpyHostToDevice, s));
   t.big_cost.resize(big.size());
-  for (size_t i = 0; i < big.size(); ++i) t.big_cost[i] = big_row_cost(pp[big[i] + 1] - pp[big[i]]);
+  t.big_h = big;
+  t.big_pp.resize(2 * big.size());
+  for (size_t i = 0; i < big.size(); ++i) {
+    t.big_cost[i] = big_row_cost(pp[big[i] + 1] - pp[big[i]]);
+    t.big_pp[2 * i] = pp[big[i]];
+    t.big_pp[2 * i + 1] = pp[big[i] + 1];
+  }
+  t.win_b.resize(t.win_h.size());
+  for (size_t k = 0; k < t.win_h.size(); ++k) t.win_b[k] = wp[t.win_h[k]];
   t.win_cost.resize(t.nwin);
   for (uint32_t k = 0; k < t.nwin; ++k) {
     const uint32_t w0 = t.win_h[k], w1 = t.win_h[k + 1];
@@ -600,11 +608,26 @@ void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
       flat.insert(flat.end(), lists[b].begin(), lists[b].end());
     }
     ptr[G] = static_cast<uint32_t>(flat.size());
+    std::vector<GatherDesc> desc(flat.size());
+    for (size_t i = 0; i < flat.size(); ++i) {
+      const uint32_t it = flat[i];
+      if (it < t.nbig) {
+        desc[i] = {t.big_h[it], t.big_pp[2 * it], t.big_pp[2 * it + 1], 0u};
+      } else if (it - t.nbig < gp.nwin) {
+        const uint32_t k = it - t.nbig;
+        desc[i] = {t.win_h[k], t.win_h[k + 1], t.win_b[k], t.win_b[k + 1]};
+      } else {
+        desc[i] = {it - t.nbig - gp.nwin, 0u, 0u, 0u};
+      }
+    }
     gp.ptr.alloc(sizeof(uint32_t) * ptr.size());
     gp.items.alloc(sizeof(uint32_t) * std::max<size_t>(flat.size(), 1));
+    gp.desc.alloc(sizeof(GatherDesc) * std::max<size_t>(flat.size(), 1));
     NWGR_HIP(hipMemcpyAsync(gp.ptr.p, ptr.data(), sizeof(uint32_t) * ptr.size(), hipMemcpyHostToDevice, s));
-    if (!flat.empty())
+    if (!flat.empty()) {
       NWGR_HIP(hipMemcpyAsync(gp.items.p, flat.data(), sizeof(uint32_t) * flat.size(), hipMemcpyHostToDevice, s));
+      NWGR_HIP(hipMemcpyAsync(gp.desc.p, desc.data(), sizeof(GatherDesc) * desc.size(), hipMemcpyHostToDevice, s));
+    }
     NWGR_HIP(hipStreamSynchronize(s));  // the host vectors must outlive the copies
   }
   t.gplan_blocks = G;
