@@ -560,11 +560,19 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   t.built = true;
 }
 
-// Item lists of the gather pass: each item goes, heaviest first, to the
-// workgroup whose list is cheapest so far (ties to the lower index), and each
-// list is worked through in the order it was dealt.
+// Item lists of the gather pass.  Each item goes, heaviest first, to the
+// workgroup whose list is cheapest so far -- but to the cheapest list of its
+// home XCD when that list is within kAffinitySlack of the cheapest overall.
+// Home XCDs cut the items in row order into contiguous runs of about equal
+// cost, so neighbouring windows, which share the partials' cache lines at
+// their ends in every tile, mostly run on one XCD (workgroup b runs on XCD
+// b % kXcds) while the lists still end together.  Each list is worked through
+// in the order it was dealt.
+constexpr double kAffinitySlack = 1.0;  // estimated us
+
 void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
   const uint32_t G = g.grid_c;
+  const uint32_t X = G % kXcds == 0 ? kXcds : 1u;  // XCDs
   const uint32_t quiet_rows = g.n - std::min(g.nz, g.n);
   for (uint32_t mode = 0; mode < 2; ++mode) {
     Tiled::GatherPlan& gp = t.gplan[mode];
@@ -575,31 +583,45 @@ void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
                                     t.win_h.begin());
     gp.nqb = mode && t.quiet_tail && quiet_rows ? (quiet_rows + kQuietRows - 1) / kQuietRows : 0;
     const uint32_t nitems = t.nbig + gp.nwin + gp.nqb;
-    std::vector<std::pair<float, uint32_t>> items(nitems);
+    std::vector<float> cost(nitems);
+    double total = 0;
     for (uint32_t i = 0; i < nitems; ++i) {
-      float c;
       if (i < t.nbig)
-        c = t.big_cost[i];
+        cost[i] = t.big_cost[i];
       else if (i - t.nbig < gp.nwin)
-        c = t.win_cost[i - t.nbig];
+        cost[i] = t.win_cost[i - t.nbig];
       else
-        c = quiet_block_cost(std::min(kQuietRows, quiet_rows - (i - t.nbig - gp.nwin) * kQuietRows));
-      items[i] = {c, i};
+        cost[i] = quiet_block_cost(std::min(kQuietRows, quiet_rows - (i - t.nbig - gp.nwin) * kQuietRows));
+      total += cost[i];
     }
+    std::vector<uint32_t> home(nitems);
+    {
+      double acc = 0;
+      for (uint32_t i = 0; i < nitems; ++i) {
+        home[i] = std::min<uint32_t>(X - 1, static_cast<uint32_t>((acc + 0.5 * cost[i]) / total * X));
+        acc += cost[i];
+      }
+    }
+    std::vector<std::pair<float, uint32_t>> items(nitems);
+    for (uint32_t i = 0; i < nitems; ++i) items[i] = {cost[i], i};
     std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-    // (load, workgroup) min-heap
-    std::vector<std::pair<double, uint32_t>> heap;
-    heap.reserve(G);
-    for (uint32_t b = 0; b < G; ++b) heap.push_back({0.0, b});
+    // one (load, workgroup) min-heap per XCD
     auto cmp = [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second > b.second); };
-    std::make_heap(heap.begin(), heap.end(), cmp);
+    std::vector<std::vector<std::pair<double, uint32_t>>> heaps(X);
+    for (uint32_t b = 0; b < G; ++b) heaps[b % X].push_back({0.0, b});
+    for (auto& h : heaps) std::make_heap(h.begin(), h.end(), cmp);
     std::vector<std::vector<uint32_t>> lists(G);
     for (const auto& it : items) {
-      std::pop_heap(heap.begin(), heap.end(), cmp);
-      auto& h = heap.back();
-      lists[h.second].push_back(it.second);
-      h.first += it.first;
-      std::push_heap(heap.begin(), heap.end(), cmp);
+      uint32_t best = 0;
+      for (uint32_t x = 1; x < X; ++x)
+        if (cmp(heaps[best].front(), heaps[x].front())) best = x;
+      const uint32_t h = home[it.second];
+      const uint32_t x = heaps[h].front().first <= heaps[best].front().first + kAffinitySlack ? h : best;
+      auto& hp = heaps[x];
+      std::pop_heap(hp.begin(), hp.end(), cmp);
+      lists[hp.back().second].push_back(it.second);
+      hp.back().first += it.first;
+      std::push_heap(hp.begin(), hp.end(), cmp);
     }
     std::vector<uint32_t> ptr(static_cast<size_t>(G) + 1, 0), flat;
     flat.reserve(nitems);
