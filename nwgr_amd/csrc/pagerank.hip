@@ -1281,6 +1281,9 @@ __device__ __forceinline__ void combine_window(const CombineArgs& a, const Gathe
 // epilogue with z = 0 -- the same operations, so the value a stored row
 // without entries gets -- stored as contributions only; block 0 adds their
 // residual and dangling mass as count x value and records the common rank.
+// With the dangling mass leaked the common rank is (1 - d)/n in every sweep
+// (z = 0, no dangling term), so once both contribution buffers hold it
+// (sweeps 1 and 2) the stores are skipped.
 template <uint32_t KIND>
 __device__ __forceinline__ void combine_quiet(const CombineArgs& a, uint32_t q, float Df, double& err,
                                               double& dang) {
@@ -1289,7 +1292,8 @@ __device__ __forceinline__ void combine_quiet(const CombineArgs& a, uint32_t q, 
     const float u = s.dangling == kDanglingUniform ? s.un : 0.f;
     const float sq = fmaf(Df, u, 0.f);
     const float rn = fmaf(s.d_hi, sq, fmaf(s.d_lo, sq, s.base_u));
-    const uint32_t lo = s.nrows + q * kQuietRows, hi = min(s.n, lo + kQuietRows);
+    const uint32_t lo = s.nrows + q * kQuietRows,
+                   hi = s.dangling != kDanglingLeak || s.st->ctr[s.p] <= 2 ? min(s.n, lo + kQuietRows) : lo;
     constexpr uint32_t U = kQuietRows / kCombineThreads;  // one round of loads
     for (uint32_t v0 = lo + threadIdx.x; v0 < hi; v0 += U * kCombineThreads) {
       float dv[U];

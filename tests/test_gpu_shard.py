@@ -81,9 +81,12 @@ def test_each_share_alone_on_its_own_layout(shares):
     pg = PageRankGraph(g, deg)
     order, cuts = pg.gs_phases(shares)
     # the shares are the row partition a multi-GPU solve takes: about equal
-    # sweep bytes (8 per entry, 16 per row with entries, 8 per row without)
+    # sweep cost (8 per entry, 2.9 per fragment -- about min(entries, tiles)
+    # + entries // 256 of them -- 80 per row with entries, 1 per row without)
     counts = np.diff(g.offsets.astype(np.int64))[order]
-    cost = 8 * counts + np.where(counts > 0, 16, 8)
+    tiles = -(-g.n // 2 ** 15)
+    frags = np.minimum(counts, tiles) + counts // 256
+    cost = np.where(counts > 0, 8.0 * counts + 2.9 * frags + 80.0, 1.0)
     per = np.add.reduceat(cost, cuts[:-1].astype(np.int64))
     assert per.max() <= cost.sum() / shares + cost.max(), (per, cost.sum() / shares)
     out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
