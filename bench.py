@@ -429,7 +429,7 @@ def bench_delta(args):
 def bench_shard(args):
     """One GPU's share of a row-partitioned solve of R-MAT s22 ef16: the
     destination rows (in the handle's relabelled order) cut into `--shards`
-    consecutive ranges of about equal sweep bytes, each swept alone on a
+    consecutive ranges of about equal sweep cost, each swept alone on a
     tiled layout that holds only its entries and stores only its rows
     (nwgr_pr_share_run_device_f32), the other rows' contributions resident
     and fixed, as one GPU holds them between the all-gathers of its slice.
@@ -477,7 +477,7 @@ def bench_shard(args):
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"rmat s{args.scale} ef{args.edge_factor}, relabelled rows cut into {G} ranges of "
-                               f"equal sweep bytes, each swept alone on its own layout (only its entries, only its "
+                               f"equal sweep cost, each swept alone on its own layout (only its entries, only its "
                                f"rows stored), the other contributions resident; 20 sweeps per solve, LEAK",
                    "n": g.n, "m": g.m, "whole_graph_us_per_sweep": round(full_us, 2),
                    "ideal_us_per_sweep": round(full_us / G, 2), "shares": shares,

@@ -183,8 +183,9 @@ int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r
  * page_rank (v11, page_rank.hpp:124-139: each row's new rank and
  * contribution replace the old ones within the sweep) made deterministic.
  * The rows, in the handle's relabelled order (nwgr_pr_gs_phases), are cut
- * into `phases` consecutive phases of about equal sweep bytes (8 per entry,
- * 16 per row with entries, 8 per row without); inside a
+ * into `phases` consecutive phases of about equal sweep cost (8 per entry,
+ * 2.9 per fragment -- about min(entries, tiles) + entries / 256 -- 80 per
+ * row with entries, 1 per row without); inside a
  * phase every row gathers the contributions as they stood when the phase
  * began, and a phase's results replace the old ones before the next phase
  * gathers.  phases = 1 is the Jacobi solve of nwgr_pr_run_f32; more phases
