@@ -151,3 +151,15 @@ def test_rmat_s22_phases_cut_the_sweeps():
         r, res = gs_parity(pg, g, deg, phases, tol=1e-6, max_iters=100)
         say(f"  s22 phases={phases}: {res.iters} sweeps against {res_j.iters} Jacobi, {res.ms:.2f} ms")
         assert res.iters < res_j.iters - 4
+
+
+def test_big_rows_in_phase_layouts():
+    """Phase layouts holding rows of more than 2048 partials (items of their
+    own in the gather pass)."""
+    from tests.test_gpu_pagerank import big_row_graph
+    say("gs big rows")
+    g = big_row_graph()
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    for phases in (2, 5):
+        gs_parity(pg, g, deg, phases)

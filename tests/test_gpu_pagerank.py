@@ -106,6 +106,36 @@ def test_hub_rows_split_in_chunks():
     parity(g, deg, "leak", 1e-6, 100)
 
 
+def big_row_graph():
+    """Two rows whose partials (one per 256-entry run inside a source tile)
+    pass 2048 at any tile width: 650,000 and 560,000 in-neighbours, plus a
+    random background."""
+    n = 700_000
+    rng = np.random.default_rng(5)
+    src = np.concatenate([np.arange(1, 650_001), np.arange(50_000, 610_000), rng.integers(0, n, 1_000_000)])
+    dst = np.concatenate([np.zeros(650_000, np.int64), np.ones(560_000, np.int64), rng.integers(0, n, 1_000_000)])
+    g = nwgr_amd.build_adjacency(src, dst, n=n, dedup=True, drop_self_loops=True)
+    assert np.sort(np.diff(g.offsets))[-2] > 2048 * 256
+    return g
+
+
+@pytest.mark.parametrize("tuning", [{}, dict(engine=1, tile_bits=13, tile_combine=0)])
+def test_big_rows_added_by_the_whole_workgroup(tuning):
+    """Rows of more than 2048 partials are items of their own in the gather
+    pass (every thread adds a strided share, wavefronts in order)."""
+    say(f"big rows {tuning}")
+    g = big_row_graph()
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    pg.tuning = tuning
+    parity(g, deg, "uniform", 1e-6, 100, device_graph=pg)
+    parity(g, deg, "leak", 1e-6, 100, device_graph=pg)
+    tel = np.random.default_rng(6).random(g.n).astype(np.float32)
+    tel /= tel.sum()
+    parity(g, deg, "teleport", 1e-6, 100, teleport=tel, device_graph=pg)
+    assert pg.last_engine().startswith("tiled")
+
+
 @pytest.mark.parametrize("tuning", [{}, dict(engine=0), dict(engine=1, tile_combine=0),
                                     dict(engine=1, tile_combine=1)])
 @pytest.mark.parametrize("n", [1, 2, 5, 70000])
