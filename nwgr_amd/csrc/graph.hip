@@ -403,6 +403,7 @@ void replan(Graph& g, hipStream_t stream) {
 Graph::~Graph() {
   drop_graphs();
   if (hflag) (void)hipHostFree(hflag);
+  if (hist_h) (void)hipHostFree(hist_h);
   if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 

@@ -164,6 +164,8 @@ struct HostFlag {
   volatile double last_err;     // error of sweep `progress`
   volatile double prev_err;     // error of sweep `progress - 1` (0 if none)
   volatile double first_err;    // error of the initial iterate (kDelta: pushes of iteration 1)
+  volatile uint32_t finished;   // the output permutation enqueued before the stop was seen has run
+  volatile uint32_t pad2;
 };
 
 // Timing of the last solve, from events around each captured chunk.
@@ -451,6 +453,7 @@ struct Graph {
   DevBuf tel_h, init_h;  // f32[n] device copies of a host-output solve's teleport / initial vector
   DevBuf vflag;     // u32 result of the device check of the caller's vectors
   HostFlag* hflag = nullptr;   // pinned, mapped
+  double* hist_h = nullptr;    // pinned: the per-sweep errors copied back after a solve
   HostFlag* hflag_dev = nullptr;
 
   // captured chunk graphs: [kind or kExecGs][parity of the first sweep][length]
@@ -612,7 +615,16 @@ uint32_t combine_blocks_per_cu();
 // Item lists of the gather pass (tiled.hip), for g.grid_c workgroups.
 void plan_gather(Graph& g, Tiled& t, hipStream_t s);
 
-void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s);
+// The output a solve ends with: its ranks in caller order (quiet rows from
+// quiet_from on hold the quiet scalar).  solve() enqueues it as soon as the
+// launches issued are expected to include the stopping one (it runs only if
+// the solve has then stopped), and again after the stop when that was early.
+struct SolveFinish {
+  float* out;
+  uint32_t quiet_from;
+};
+void solve(Graph& g, const SolveSpec& spec, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s,
+           const SolveFinish* fin = nullptr);
 void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s);
 // block Gauss-Seidel solve with `phases` phases (LEAK dangling); with
 // only >= 0, the sweeps update that phase's rows alone (a row share)

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "nwgr_internal.hpp"
 
@@ -1492,10 +1493,14 @@ __global__ void k_reset_queue(State* st) { st->work[0] = 0; }
 
 // out[v] = r[iperm[v]] * scale.  With `st`, r is the final iterate
 // r_pair[st->iters & 1], and rows from `quiet_from` on hold its quiet rank.
+// With `hf`: only once the solve has stopped (else nothing), and reports that
+// it ran in the host flag.
 __global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t* __restrict__ iperm,
                                                       const float* __restrict__ r0, const float* __restrict__ r1,
                                                       const State* st, uint32_t n, uint32_t quiet_from, float scale,
-                                                      float* __restrict__ out) {
+                                                      float* __restrict__ out, HostFlag* hf) {
+  if (hf && !st->done) return;
+  if (hf && blockIdx.x == 0 && threadIdx.x == 0) hf->finished = 1;
   const float* r = (st && (st->iters & 1)) ? r1 : r0;
   const float rq = st ? st->rq[st->iters & 1] : 0.f;
   const uint32_t stride = gridDim.x * kBlock;
@@ -1869,7 +1874,15 @@ void empty_result(const nwgr_pr_params& p, nwgr_pr_result& res) {
   res.converged = (res.iters > 0 && p.tol > 0) ? 1u : 0u;
 }
 
-void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s) {
+void finish_launch(Graph& g, const SolveFinish& f, bool cond, hipStream_t s) {
+  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, f.quiet_from, 1.0f,
+      f.out, cond ? g.hflag_dev : nullptr);
+  NWGR_HIP(hipGetLastError());
+}
+
+void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_result& res, hipStream_t s,
+           const SolveFinish* fin) {
   // a traced solve captures one sweep per chunk, so the events around the
   // chunks time every sweep on its own (the reference's per-iteration time,
   // page_rank.hpp:123,142)
@@ -1988,14 +2001,20 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     prof.chunks += 1;
   };
   const uint32_t K0 = std::min<uint32_t>(K, 4);  // before the decay rate is known
+  // launch that is expected to detect convergence (0: no estimate yet)
+  auto predicted_stop = [&]() -> uint64_t {
+    if (pobs > 0 && eobs > 0 && rho > 0 && rho < 1 && std::isfinite(p.tol) && p.tol > 0) {
+      double more = eobs < p.tol ? 0.0 : std::ceil(std::log(p.tol / eobs) / std::log(rho));
+      if (!std::isfinite(more) || more > 1e6) more = 1e6;
+      return pobs + static_cast<uint64_t>(more) + 1;
+    }
+    return 0;
+  };
   auto plan = [&]() -> uint64_t {
     // launches still useful beyond `next - 1`
     if (next > limit) return 0;
     uint64_t want = K0;
-    if (pobs > 0 && eobs > 0 && rho > 0 && rho < 1 && std::isfinite(p.tol) && p.tol > 0) {
-      double more = eobs < p.tol ? 0.0 : std::ceil(std::log(p.tol / eobs) / std::log(rho));
-      if (!std::isfinite(more) || more > 1e6) more = 1e6;
-      const uint64_t stop_launch = pobs + static_cast<uint64_t>(more) + 1;  // launch that detects it
+    if (const uint64_t stop_launch = predicted_stop()) {
       want = stop_launch >= next ? stop_launch - next + 1 : 0;
     } else if (pobs > 0) {
       want = K;  // no usable decay estimate (stagnating or tol <= 0): full chunks
@@ -2011,12 +2030,45 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     prof.chunk_ms += cms;
     if (per_sweep) prof.launch_ms.push_back(cms);
   };
+  // progress as the device publishes it, sweep by sweep, into host-mapped
+  // memory (read while chunks run, so later chunks are sized before the
+  // earlier ones end)
+  auto observe = [&]() {
+    const uint32_t pr = g.hflag->progress;
+    if (pr > pobs) {
+      pobs = pr;
+      eobs = g.hflag->last_err;
+      const double pe = g.hflag->prev_err;
+      rho = (pr >= 2 && pe > 0 && eobs < pe) ? eobs / pe : 0.0;
+    }
+  };
+  // the output permutation, enqueued once the launches issued are expected
+  // to include the one that detects convergence: it does nothing unless the
+  // solve has stopped by the time it runs, and reports in the host flag
+  bool fin_enqueued = false;
+  auto maybe_finish = [&]() {
+    if (!fin || fin_enqueued) return;
+    const uint64_t stop_launch = predicted_stop();
+    if (next > limit || (stop_launch && next > stop_launch)) {
+      finish_launch(g, *fin, true, s);
+      fin_enqueued = true;
+    }
+  };
   launch(static_cast<uint32_t>(std::min<uint64_t>(K0, limit)));
   for (;;) {
     if (pending.size() - head < 2) {
+      // the second chunk waits, while the first runs, for the decay of its
+      // sweeps 2 and 3 (or for the first chunk's end)
+      if (prof.chunks == 1 && head < pending.size() && g.hflag->progress < 3 && !g.hflag->done &&
+          hipEventQuery(pending[head].b) == hipErrorNotReady) {
+        std::this_thread::yield();
+        continue;
+      }
+      observe();
       const uint64_t len = plan();
       if (len > 0) {
         launch(static_cast<uint32_t>(len));
+        maybe_finish();
         continue;
       }
     }
@@ -2028,18 +2080,23 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     }
     retire(pending[head++]);
     if (g.hflag->done) break;
-    const uint32_t pr = g.hflag->progress;
-    if (pr > pobs) {
-      pobs = pr;
-      eobs = g.hflag->last_err;
-      const double pe = g.hflag->prev_err;
-      rho = (pr >= 2 && pe > 0 && eobs < pe) ? eobs / pe : 0.0;
-    }
+    observe();
   }
-  // chunks still in flight see `done` and return at once
+  // chunks still in flight see `done` and return at once; the per-sweep
+  // errors come back with the same synchronisation
   for (; head < pending.size(); ++head) retire(pending[head]);
+  const uint32_t iters_done = g.hflag->iters;
+  const uint32_t nhist = std::min(iters_done, g.hist_cap);
+  if (!g.hist_h) NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&g.hist_h), sizeof(double) * g.hist_cap));
+  if (nhist) NWGR_HIP(hipMemcpyAsync(g.hist_h, g.hist.p, sizeof(double) * nhist, hipMemcpyDeviceToHost, s));
   NWGR_HIP(hipEventRecord(ev1, s));
   NWGR_HIP(hipEventSynchronize(ev1));
+  if (fin && !g.hflag->finished) {
+    // the early output permutation ran before the stop (or was not issued)
+    finish_launch(g, *fin, false, s);
+    NWGR_HIP(hipEventRecord(ev1, s));
+    NWGR_HIP(hipEventSynchronize(ev1));
+  }
   float ms = 0;
   NWGR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
   res.iters = g.hflag->iters;
@@ -2048,9 +2105,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   res.ms = ms;
   g.last_profile = prof;
   g.last_engine = !sp.tiled ? 0 : (sp.tiled->acc_on ? 2 : 1);
-  g.last_errs.assign(std::min(res.iters, g.hist_cap), 0.0);
-  if (!g.last_errs.empty())
-    NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
+  g.last_errs.assign(g.hist_h, g.hist_h + nhist);
   if (p.trace) {
     // the reference's trace table (page_rank.hpp:42-52, :109, :142), with
     // every sweep's own device time (seconds)
@@ -2093,7 +2148,7 @@ void prepare_quiet(Graph& g, hipStream_t s) {
 
 void unpermute(Graph& g, const float* src, float* out, float scale, hipStream_t s) {
   k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), src, src, nullptr, g.n, g.n, scale, out);
+      g.iperm.as<uint32_t>(), src, src, nullptr, g.n, g.n, scale, out, nullptr);
   NWGR_HIP(hipGetLastError());
 }
 
@@ -2114,23 +2169,8 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   // iterate uniform and the dangling mass leaked or spread uniformly
   sp.quiet = sp.tiled && !p.teleport && !p.init && p.dangling != kDanglingTeleport && g.nz < g.n;
   if (sp.quiet) prepare_quiet(g, s);
-  solve(g, sp, p, res, s);
-  hipEvent_t e0, e1;
-  NWGR_HIP(hipEventCreate(&e0));
-  NWGR_HIP(hipEventCreate(&e1));
-  NWGR_HIP(hipEventRecord(e0, s));
-  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n,
-      sp.quiet ? g.nz : g.n, 1.0f, d_out);
-  hipError_t le = hipGetLastError();
-  NWGR_HIP(hipEventRecord(e1, s));
-  NWGR_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  NWGR_HIP(le);
-  res.ms += ms;
+  const SolveFinish fin{d_out, sp.quiet ? g.nz : g.n};
+  solve(g, sp, p, res, s, &fin);
 }
 
 void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s,
@@ -2157,12 +2197,8 @@ void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nw
   // quiet rows are kept as the scalar only where the last phase refreshes them
   sp.quiet = !p.teleport && !p.init && g.nz < g.n && (only < 0 || only + 1 == static_cast<int>(phases));
   if (sp.quiet) prepare_quiet(g, s);
-  solve(g, sp, p, res, s);
-  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n,
-      sp.quiet ? g.nz : g.n, 1.0f, d_out);
-  NWGR_HIP(hipGetLastError());
-  NWGR_HIP(hipStreamSynchronize(s));
+  const SolveFinish fin{d_out, sp.quiet ? g.nz : g.n};
+  solve(g, sp, p, res, s, &fin);
   g.last_engine = 3;
 }
 
@@ -2192,22 +2228,8 @@ void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   nwgr_pr_params q = p;
   q.max_iters = p.max_iters - 1;
   q.tol = 0.5;
-  solve(g, sp, q, res, s);
-  hipEvent_t e0, e1;
-  NWGR_HIP(hipEventCreate(&e0));
-  NWGR_HIP(hipEventCreate(&e1));
-  NWGR_HIP(hipEventRecord(e0, s));
-  k_unpermute<<<std::min<uint32_t>((g.n + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      g.iperm.as<uint32_t>(), g.r[0].as<float>(), g.r[1].as<float>(), g.state.as<State>(), g.n, g.n, 1.0f, d_out);
-  hipError_t le = hipGetLastError();
-  NWGR_HIP(hipEventRecord(e1, s));
-  NWGR_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  NWGR_HIP(le);
-  res.ms += ms;
+  const SolveFinish fin{d_out, g.n};
+  solve(g, sp, q, res, s, &fin);
   // per-iteration push counts: iteration 1 (init), then one per sweep
   const double first = g.hflag->first_err;
   g.last_errs.insert(g.last_errs.begin(), first);
