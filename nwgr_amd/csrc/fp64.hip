@@ -262,10 +262,6 @@ void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_resul
   }
   if (p.teleport) ensure(g.tel64, sizeof(double) * n);
   ensure(g.hub64, sizeof(double) * std::max<uint32_t>(g.fwd.n_hub_units, 1));
-  if (!g.hist.p) {
-    g.hist_cap = kHistCap;
-    g.hist.alloc(sizeof(double) * g.hist_cap);
-  }
   const uint32_t grid = g.grid;  // the part slots hold at least this many pairs
   std::memset(const_cast<HostFlag*>(g.hflag), 0, sizeof(HostFlag));
   hipEvent_t e0, e1;
@@ -302,7 +298,7 @@ void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_resul
   a.tel = ia.tel;
   a.st = ia.st;
   a.hflag = g.hflag_dev;
-  a.hist = g.hist.as<double>();
+  a.hist = g.hist_dev;
   a.hist_cap = g.hist_cap;
   a.hub_units = g.fwd.hub_units.as<HubUnit>();
   a.hub_part = g.hub64.as<double>();
@@ -350,9 +346,7 @@ void run_plain64(Graph& g, const nwgr_pr_params& p, double* d_out, nwgr_pr_resul
   res.ms = ms;
   g.last_engine = 0;
   g.last_profile = SolveProfile{};
-  g.last_errs.assign(std::min(res.iters, g.hist_cap), 0.0);
-  if (!g.last_errs.empty())
-    NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
+  g.last_errs.assign(g.hist_h, g.hist_h + std::min(res.iters, g.hist_cap));  // written by the sweeps (mapped)
 }
 
 }  // namespace nwgr

@@ -624,7 +624,8 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   NWGR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g->hflag_dev), g->hflag, 0));
   std::memset(const_cast<HostFlag*>(g->hflag), 0, sizeof(HostFlag));
   g->hist_cap = kHistCap;
-  g->hist.alloc(sizeof(double) * g->hist_cap);
+  NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->hist_h), sizeof(double) * g->hist_cap, hipHostMallocMapped));
+  NWGR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g->hist_dev), g->hist_h, 0));
   // the layout the sweeps of the default tuning use is built here, so the
   // first solve costs what every later one does (less its graph captures)
   (void)select_tiled(*g, false, stream);

@@ -147,6 +147,13 @@ struct State {
   double dsum;        // tiled engine: dangling mass (or Q) of the sweep's input iterate
   double last_e;      // error of the latest completed sweep (the decay estimate past the history)
   float rq[2];        // quiet rows (no entries): their common rank in the iterate of parity p
+  // Output permutation written by the gather pass of the stopping launch
+  // (null: a launch of its own writes the output).  Rows from out_quiet_from
+  // on hold the quiet rank; fin_done is set by the first tile pass after the
+  // stop, so the later launches of the chunks in flight leave `out` alone.
+  float* out;
+  uint32_t out_quiet_from;
+  uint32_t fin_done;
 };
 
 // Per-sweep errors kept on the device for the trace (the first kHistCap
@@ -447,19 +454,22 @@ struct Graph {
   DevBuf tel;       // f32[n] permuted teleport (when given)
   DevBuf partials;  // double[2][grid][2]
   DevBuf state;     // State
-  DevBuf hist;      // double[hist_cap]
-  uint32_t hist_cap = 0;
+  uint32_t hist_cap = 0;  // per-sweep errors kept in hist_h
   DevBuf out_buf;   // f32[n] ranks of a host-output solve before the copy back
   DevBuf tel_h, init_h;  // f32[n] device copies of a host-output solve's teleport / initial vector
   DevBuf vflag;     // u32 result of the device check of the caller's vectors
   HostFlag* hflag = nullptr;   // pinned, mapped
-  double* hist_h = nullptr;    // pinned: the per-sweep errors copied back after a solve
+  double* hist_h = nullptr;    // pinned, mapped: the sweeps write their errors here directly
   HostFlag* hflag_dev = nullptr;
+  double* hist_dev = nullptr;  // hist_h as the device addresses it
 
   // captured chunk graphs: [kind or kExecGs][parity of the first sweep][length]
   static constexpr uint32_t kMaxChunk = 32;
   hipGraphExec_t exec[kExecSlots][2][kMaxChunk + 1] = {};
   uint64_t exec_key[kExecSlots] = {};
+  // sweeps of the latest solve per slot, with its graphs' key (first-chunk length of the next solve)
+  uint64_t hint_key[kExecSlots] = {};
+  uint32_t hint_iters[kExecSlots] = {};
 
   // block Gauss-Seidel (built on first use): the phases' layouts, their row
   // cuts (positions in the relabelled order, phases + 1 entries) and the
@@ -517,7 +527,7 @@ inline uint64_t bytes_of(const Graph& g) {
                           &g.fwd.hub_part, &g.fwd.hub_ticket, &g.fwd.rows, &g.t_offs, &g.t_idx, &g.t_pos,
                           &g.adj.hub_units, &g.adj.hub_part, &g.adj.hub_ticket, &g.adj.rows, &g.w_in,
                           &g.w_out, &g.s, &g.gsrc, &g.rsave, &g.coef_a, &g.coef_b, &g.r[0], &g.r[1], &g.c[0],
-                          &g.c[1], &g.tel, &g.partials, &g.state, &g.hist, &g.degu, &g.out_buf, &g.tel_h,
+                          &g.c[1], &g.tel, &g.partials, &g.state, &g.degu, &g.out_buf, &g.tel_h,
                           &g.init_h, &g.r64[0], &g.r64[1], &g.c64[0], &g.c64[1], &g.tel64, &g.hub64})
     b += d->bytes;
   return b;

@@ -105,6 +105,8 @@ struct InitArgs {
   float un;
   float thr, d, res0;  // push threshold, damping, initial residual 1 - d (kDelta)
   uint32_t npairs;     // pairs sweep 1 adds: this grid's, then zeros
+  uint32_t quiet_from; // rows from here on are quiet (their ranks are not stored)
+  float* out;          // caller-order output the stopping gather pass writes (null: none)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -342,7 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       if (send) sends += 1.0;
       continue;
     }
-    const uint32_t src = a.perm[i];
+    // (the caller-order index only when a caller vector is read)
+    const uint32_t src = a.tel_src || a.init ? a.perm[i] : 0u;
     float t = a.un;
     if (a.tel_src) {
       t = a.tel_src[src];
@@ -358,8 +361,10 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
     } else {
       const float r = a.init ? a.init[src] : a.un;
       const float dg = a.div[i];
-      a.r0[i] = r;
-      if (a.r1) a.r1[i] = r;
+      if (i < a.quiet_from) {  // quiet rows' ranks live in the scalar State::rq
+        a.r0[i] = r;
+        if (a.r1) a.r1[i] = r;
+      }
       a.c0[i] = r / dg;
       if (dg == 0.f) dang += static_cast<double>(r);
     }
@@ -386,6 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
       a.st->last_e = 0;
       a.st->rq[0] = a.un;  // quiet rows of the initial iterate (used without an init vector only)
       a.st->rq[1] = a.un;
+      a.st->out = a.out;
+      a.st->out_quiet_from = a.quiet_from;
+      a.st->fin_done = 0;
     }
   }
 }
@@ -590,7 +598,41 @@ struct CombineArgs {
   const uint32_t* __restrict__ items;
   const GatherDesc* __restrict__ desc;  // per list entry
   uint32_t nbig, nwin, nqb;
+  // the output permutation (State::out): caller order -> row, and the iterates
+  const uint32_t* __restrict__ iperm;
+  const float* rp0;
+  const float* rp1;
 };
+
+// The gather pass of the stopping launch writes the caller-order output
+// out[v] = r[iperm[v]], r the final iterate r_pair[st->iters & 1] (rows from
+// State::out_quiet_from on hold its quiet rank), when State::out is set and
+// the first tile pass after the stop has not yet marked it written.  NT
+// threads per workgroup, every workgroup of the grid.
+template <uint32_t NT>
+__device__ __forceinline__ void finish_output(const CombineArgs& a) {
+  const State* st = a.s.st;
+  float* out = st->out;
+  if (!out || st->fin_done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.s.hflag->finished = 1;
+  const uint32_t it = st->iters;
+  const float* __restrict__ r = (it & 1) ? a.rp1 : a.rp0;
+  const float rq = st->rq[it & 1];
+  const uint32_t qf = st->out_quiet_from, n = a.s.n;
+  constexpr uint32_t U = 4;  // caller positions in flight per thread
+  const uint32_t stride = gridDim.x * NT;
+  for (uint32_t v0 = blockIdx.x * NT + threadIdx.x; v0 < n; v0 += U * stride) {
+    uint32_t i[U];
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) i[j] = v0 + j * stride < n ? a.iperm[v0 + j * stride] : qf;
+    float x[U];
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j) x[j] = i[j] < qf ? r[i[j]] : rq;
+#pragma unroll
+    for (uint32_t j = 0; j < U; ++j)
+      if (v0 + j * stride < n) out[v0 + j * stride] = x[j];
+  }
+}
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -1003,7 +1045,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
         D += v[j].y;
       }
     }
-    if (tid == 0) *s_flag = a.st->done;
+    if (tid == 0) {
+      const uint32_t dn = a.st->done;
+      *s_flag = dn;
+      // a launch past the stop: the stopping launch's gather pass has written
+      // the output (workgroup 0 of the stopping launch reads 0 here: it sets
+      // `done` itself, later)
+      if (dn && blockIdx.x == 0) a.st->fin_done = 1;
+    }
     __syncthreads();
     if (*s_flag) return;
     E = wave_sum(E);
@@ -1322,7 +1371,10 @@ __global__ __launch_bounds__(kCombineThreads) __attribute__((amdgpu_waves_per_eu
     L.nh = 0;
   }
   __syncthreads();
-  if (L.flag) return;
+  if (L.flag) {
+    if constexpr (KIND != kCoef) finish_output<kCombineThreads>(a);
+    return;
+  }
   const float Df = KIND != kCoef ? static_cast<float>(a.s.st->dsum) : 0.f;
   double err = 0, dang = 0;
   const uint32_t i0 = a.ptr[blockIdx.x], i1 = a.ptr[blockIdx.x + 1];
@@ -1442,7 +1494,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_accum(CombineArgs a, AccArgs 
   if constexpr (KIND != kCoef) {
     if (tid == 0) s_flag = a.s.st->done;
     __syncthreads();
-    if (s_flag) return;
+    if (s_flag) {
+      finish_output<kSweepThreads>(a);
+      return;
+    }
     Df = static_cast<float>(a.s.st->dsum);
   }
   double err = 0, dang = 0;
@@ -1567,7 +1622,7 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
   }
   a.st = g.state.as<State>();
   a.hflag = g.hflag_dev;
-  a.hist = g.hist.as<double>();
+  a.hist = g.hist_dev;
   a.hist_cap = g.hist_cap;
   a.hub_units = sp.plan->hub_units.as<HubUnit>();
   a.hub_part = sp.plan->hub_part.as<uint32_t>();
@@ -1692,6 +1747,9 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ca.nbig = t.nbig;
   ca.nwin = gp.nwin;
   ca.nqb = kind == kPlain ? gp.nqb : 0u;
+  ca.iperm = g.iperm.as<uint32_t>();
+  ca.rp0 = g.r[0].as<float>();
+  ca.rp1 = g.r[1].as<float>();
   const size_t shm = tile_lds_bytes(t.bits);
   const uint32_t nb = t.nblocks;
   if (t.acc_on) {
@@ -1888,11 +1946,6 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   // page_rank.hpp:123,142)
   const bool per_sweep = p.trace != 0;
   const uint32_t K = per_sweep ? 1u : std::clamp<uint32_t>(p.chunk ? p.chunk : g.tune.chunk, 1, Graph::kMaxChunk);
-  if (!g.hist.p) {
-    g.hist_cap = kHistCap;
-    g.hist.alloc(sizeof(double) * g.hist_cap);
-    g.drop_graphs();
-  }
   const bool has_tel = sp.tel_src != nullptr;
   if (has_tel && !g.tel.p) {
     g.tel.alloc(sizeof(float) * g.n);
@@ -1970,6 +2023,11 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   ia.d = static_cast<float>(p.damping);
   ia.res0 = static_cast<float>(1.0 - static_cast<double>(ia.d));
   ia.npairs = npairs;
+  ia.quiet_from = sp.quiet ? g.nz : g.n;
+  // the stopping launch's gather pass writes the output (tiled sweeps without
+  // phases); otherwise a launch of its own does
+  const bool fused_fin = fin && sp.tiled && !sp.phases;
+  ia.out = fused_fin ? fin->out : nullptr;
   k_init<<<std::min(sp.tiled ? g.grid_c : g.grid, npairs), kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());
 
@@ -2000,9 +2058,16 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     prof.launches += len;
     prof.chunks += 1;
   };
-  const uint32_t K0 = std::min<uint32_t>(K, 4);  // before the decay rate is known
+  // Before the decay rate is known: four sweeps, or -- when the handle's
+  // previous solve had the same captured arguments and the same uniform start
+  // (no teleport or initial vector) -- that solve's sweeps plus the stopping
+  // launch.  Launch planning only: every launch past the stop returns at once.
+  const bool hinted = !p.init && !p.teleport && !per_sweep && g.hint_key[slot] == key && g.hint_iters[slot] > 0;
+  const uint64_t hint_stop = hinted ? static_cast<uint64_t>(g.hint_iters[slot]) + 1 : 0;
+  const uint32_t K0 = hinted ? static_cast<uint32_t>(std::min<uint64_t>(K, hint_stop)) : std::min<uint32_t>(K, 4);
   // launch that is expected to detect convergence (0: no estimate yet)
   auto predicted_stop = [&]() -> uint64_t {
+    if (pobs == 0 && hint_stop) return hint_stop;
     if (pobs > 0 && eobs > 0 && rho > 0 && rho < 1 && std::isfinite(p.tol) && p.tol > 0) {
       double more = eobs < p.tol ? 0.0 : std::ceil(std::log(p.tol / eobs) / std::log(rho));
       if (!std::isfinite(more) || more > 1e6) more = 1e6;
@@ -2047,7 +2112,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   // solve has stopped by the time it runs, and reports in the host flag
   bool fin_enqueued = false;
   auto maybe_finish = [&]() {
-    if (!fin || fin_enqueued) return;
+    if (!fin || fused_fin || fin_enqueued) return;
     const uint64_t stop_launch = predicted_stop();
     if (next > limit || (stop_launch && next > stop_launch)) {
       finish_launch(g, *fin, true, s);
@@ -2055,11 +2120,13 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     }
   };
   launch(static_cast<uint32_t>(std::min<uint64_t>(K0, limit)));
+  maybe_finish();  // a first chunk sized from the previous solve covers the stop
   for (;;) {
     if (pending.size() - head < 2) {
       // the second chunk waits, while the first runs, for the decay of its
-      // sweeps 2 and 3 (or for the first chunk's end)
-      if (prof.chunks == 1 && head < pending.size() && g.hflag->progress < 3 && !g.hflag->done &&
+      // sweeps 2 and 3 (or for the first chunk's end); after a hinted first
+      // chunk nothing more is planned until it ends
+      if (!hinted && prof.chunks == 1 && head < pending.size() && g.hflag->progress < 3 && !g.hflag->done &&
           hipEventQuery(pending[head].b) == hipErrorNotReady) {
         std::this_thread::yield();
         continue;
@@ -2086,9 +2153,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   // errors come back with the same synchronisation
   for (; head < pending.size(); ++head) retire(pending[head]);
   const uint32_t iters_done = g.hflag->iters;
-  const uint32_t nhist = std::min(iters_done, g.hist_cap);
-  if (!g.hist_h) NWGR_HIP(hipHostMalloc(reinterpret_cast<void**>(&g.hist_h), sizeof(double) * g.hist_cap));
-  if (nhist) NWGR_HIP(hipMemcpyAsync(g.hist_h, g.hist.p, sizeof(double) * nhist, hipMemcpyDeviceToHost, s));
+  const uint32_t nhist = std::min(iters_done, g.hist_cap);  // written into mapped memory by the sweeps
   NWGR_HIP(hipEventRecord(ev1, s));
   NWGR_HIP(hipEventSynchronize(ev1));
   if (fin && !g.hflag->finished) {
@@ -2106,6 +2171,8 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   g.last_profile = prof;
   g.last_engine = !sp.tiled ? 0 : (sp.tiled->acc_on ? 2 : 1);
   g.last_errs.assign(g.hist_h, g.hist_h + nhist);
+  g.hint_key[slot] = key;
+  g.hint_iters[slot] = res.iters;
   if (p.trace) {
     // the reference's trace table (page_rank.hpp:42-52, :109, :142), with
     // every sweep's own device time (seconds)

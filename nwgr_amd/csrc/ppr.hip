@@ -446,11 +446,6 @@ void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_se
   const uint32_t K4 = (k + 3) & ~3u;
   ppr_alloc(g, ps, K4);
   const uint32_t n = g.n;
-  if (!g.hist.p) {
-    g.hist_cap = kHistCap;
-    g.hist.alloc(sizeof(double) * g.hist_cap);
-    g.drop_graphs();
-  }
 
   hipEvent_t ev0, ev1;
   NWGR_HIP(hipEventCreate(&ev0));
@@ -496,7 +491,7 @@ void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_se
   ra.colerr = ps.colerr.as<double>();
   ra.st = g.state.as<State>();
   ra.hflag = g.hflag_dev;
-  ra.hist = g.hist.as<double>();
+  ra.hist = g.hist_dev;
   ra.queue = ps.queue.as<uint32_t>();
   ra.hist_cap = g.hist_cap;
   ra.nb = ps.nb;
@@ -555,9 +550,7 @@ void run_ppr(Graph& g, const nwgr_pr_params& p, uint32_t k, const uint32_t* d_se
   res.error = p.max_iters == 0 ? 0.0 : st.err;
   res.converged = (iters > 0 && res.error < p.tol) ? 1u : 0u;
   res.ms = ms;
-  g.last_errs.assign(std::min(iters, g.hist_cap), 0.0);
-  if (!g.last_errs.empty())
-    NWGR_HIP(hipMemcpy(g.last_errs.data(), g.hist.p, sizeof(double) * g.last_errs.size(), hipMemcpyDeviceToHost));
+  g.last_errs.assign(g.hist_h, g.hist_h + std::min(iters, g.hist_cap));  // written by the sweeps (mapped)
   g.ppr_cols = k;
 }
 
