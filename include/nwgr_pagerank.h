@@ -290,7 +290,7 @@ typedef struct nwgr_pr_tuning {
   uint32_t chunk;              /* sweeps per captured hipGraph launch */
   uint32_t engine;             /* NWGR_ENGINE_BINNED, NWGR_ENGINE_TILED or NWGR_ENGINE_AUTO */
   uint32_t tile_bits;          /* tiled engine: source tile width 2^tile_bits, 13 .. 15 */
-  uint32_t tile_fill_cost;     /* tiled engine: chunks of 512 entries one tile fill is
+  uint32_t tile_fill_cost;     /* tiled engine: plain chunks (1024 entries) one tile fill is
                                   charged when the chunks are cut into workgroup ranges */
   uint32_t tile_combine;       /* tiled engine, second pass: NWGR_COMBINE_GATHER,
                                   NWGR_COMBINE_ACCUMULATE or NWGR_COMBINE_AUTO */

@@ -630,6 +630,8 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
   // first solve costs what every later one does (less its graph captures)
   (void)select_tiled(*g, false, stream);
   prepare_quiet(*g, stream);
+  // and its tile pass cut by measured workgroup times (large layouts only)
+  calibrate_tile_pass(*g, stream);
   NWGR_HIP(hipStreamSynchronize(stream));
   g->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return g.release();

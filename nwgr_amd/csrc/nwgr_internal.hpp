@@ -201,7 +201,7 @@ struct Tuning {
   uint32_t unit_edges = 2048;      // target edges per non-hub unit (one wavefront)
   uint32_t zero_rows_per_unit = 2048;
   uint32_t blocks_per_cu = 1;
-  uint32_t chunk = 16;             // sweeps per captured graph
+  uint32_t chunk = 32;             // sweeps per captured graph (R-MAT s22: 766 against 751 GTEPS at 16)
   uint32_t engine = kEngineAuto;   // kEngineBinned, kEngineTiled or kEngineAuto
   uint32_t tile_bits = 15;         // tiled engine: log2 of the source tile width (13 .. 15)
   uint32_t tile_fill_cost = 40;    // tiled engine: chunk-equivalents charged per tile fill
@@ -310,6 +310,15 @@ struct Tiled {
   DevBuf tasks;      // TiledTask[ntasks]
   DevBuf wg_task;    // u32[nblocks + 1] tasks [wg_task[b], wg_task[b + 1]) of workgroup b
   DevBuf wg_tile0;   // u32[nblocks] tile of workgroup b's first task (kNoTile: it has none)
+  // The cut on the host: every chunk's cost (the fitted model, then scaled by
+  // measured workgroup times), the tasks and each workgroup's first task; and
+  // the workgroups' measured tile-pass times (10 ns ticks, kCalibSweeps
+  // sweeps by done-sweep count modulo kCalibSweeps, then workgroup)
+  std::vector<double> ccost;
+  std::vector<TiledTask> tasks_h;
+  std::vector<uint32_t> wg_h;
+  DevBuf wg_time;
+  uint32_t calib_rounds = 0;  // measured re-cuts kept
   // Big rows are left out of the windows of the gather pass: lpp and lslot are
   // pair_ptr and rslot over the other rows only (a big row's range empty), and
   // heavy marks the big rows (a bit per row).  Without big rows (light_split
@@ -470,6 +479,8 @@ struct Graph {
   // sweeps of the latest solve per slot, with its graphs' key (first-chunk length of the next solve)
   uint64_t hint_key[kExecSlots] = {};
   uint32_t hint_iters[kExecSlots] = {};
+  // the tile passes record their workgroups' times (Tiled::wg_time)
+  bool calib_on = false;
 
   // block Gauss-Seidel (built on first use): the phases' layouts, their row
   // cuts (positions in the relabelled order, phases + 1 entries) and the
@@ -515,7 +526,7 @@ inline uint64_t bytes_of(const Tiled& t) {
   for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
                           &t.wg_task, &t.wg_tile0, &t.gplan[0].ptr, &t.gplan[0].items,
                           &t.gplan[1].ptr, &t.gplan[1].items, &t.gplan[0].desc, &t.gplan[1].desc, &t.ablo, &t.aseg, &t.afrow, &t.lpp,
-                          &t.lslot, &t.heavy})
+                          &t.lslot, &t.heavy, &t.wg_time})
     b += d->bytes;
   return b;
 }
@@ -584,6 +595,14 @@ void ensure_gs(Graph& g, uint32_t phases, hipStream_t s);
 void build_accum(Graph& g, Tiled& t, hipStream_t s);
 // cut the chunks of `t` into per-workgroup task ranges (tiled.hip)
 void plan_tasks(Graph& g, Tiled& t, hipStream_t s);
+// sweeps whose tile-pass workgroup times one calibration solve records
+constexpr uint32_t kCalibSweeps = 8;
+// scale the chunk costs of `t` by the measured-to-modelled time of the
+// workgroup that added them (us per workgroup of the current cut) and cut
+// again (tiled.hip); returns false when there is nothing to scale
+bool recut_tasks(Graph& g, Tiled& t, const std::vector<double>& wg_us, hipStream_t s);
+// measured re-cuts of the default layout's tile pass (pagerank.hip)
+void calibrate_tile_pass(Graph& g, hipStream_t s);
 
 // workgroups of the tile pass for the current tile width
 uint32_t tile_blocks(const Graph& g);

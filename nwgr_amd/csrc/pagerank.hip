@@ -580,6 +580,9 @@ struct TileArgs {
   uint32_t check;   // 1: this launch takes the stop decision (the first tile pass of a sweep); 0: it only
                     // returns when the sweep's first pass stopped the solve (later block Gauss-Seidel phases)
   double tol;
+  // calibration (null: off): each workgroup's time, 10 ns ticks, at
+  // [(done sweeps % kCalibSweeps) * gridDim.x + workgroup]
+  uint32_t* __restrict__ wg_time;
 };
 
 struct CombineArgs {
@@ -1016,6 +1019,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
   const uint32_t nq = (mask + 1) / 4;  // float4 per tile
   constexpr uint32_t kJ = kMaxTile / 4 / kSweepThreads;
   const uint32_t tile0 = a.wg_tile0[vb];
+  const uint64_t t_start = a.wg_time ? wall_clock64() : 0;
+  uint32_t swept = 0;  // sweeps done before this one (calibration record)
   float4 f0[kJ];
   if (tile0 != kNoTile) {
     const float4* src = reinterpret_cast<const float4*>(a.c_in + (static_cast<size_t>(tile0) << a.bits));
@@ -1086,6 +1091,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
       }
       return;
     }
+    swept = done_sweeps;
     if (blockIdx.x == 0 && tid == 0) {
       a.st->ctr[a.p] = done_sweeps + 1;
       a.st->dsum = D;  // read by this sweep's combine pass
@@ -1126,6 +1132,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_tile(TileArgs a) {
     // slower (R-MAT s22: 61.0 against 58.5 us per launch)
     run_chunks<kW, 1>(a, lds0, lds0 + (mask + 1 + wv * kStageW) * 4u, lds0 + (mask + 1 + kNW * kStageW + wl) * 4u,
                       mask, task.chunk_lo + wv, task.chunk_hi, wl);
+  }
+  if (a.wg_time) {
+    __syncthreads();  // every wave is done
+    if (tid == 0) a.wg_time[(swept % kCalibSweeps) * G + vb] = static_cast<uint32_t>(wall_clock64() - t_start);
   }
 }
 
@@ -1731,6 +1741,8 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   ta.max_iters = sa.max_iters;
   ta.check = check;
   ta.tol = sa.tol;
+  ta.wg_time = g.calib_on && t.wg_time.bytes >= sizeof(uint32_t) * kCalibSweeps * t.nblocks
+                   ? t.wg_time.as<uint32_t>() : nullptr;
   CombineArgs ca{};
   ca.s = sa;
   ca.pair_ptr = t.pair_ptr.as<uint32_t>();
@@ -1991,6 +2003,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   key = fnv_mix(key, g.gs_stride);
   key = fnv_mix(key, sp.phases ? sp.phases->size() : 0);
   key = fnv_mix(key, static_cast<uint64_t>(sp.only + 1));
+  key = fnv_mix(key, g.calib_on);
   const uint32_t slot = sp.phases ? kExecGs : sp.kind;
   if (g.exec_key[slot] != key) {
     g.drop_graphs(static_cast<int>(slot));
@@ -2238,6 +2251,74 @@ void run_plain(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& 
   if (sp.quiet) prepare_quiet(g, s);
   const SolveFinish fin{d_out, sp.quiet ? g.nz : g.n};
   solve(g, sp, p, res, s, &fin);
+}
+
+// Measured re-cuts of the default layout's tile pass.  The cut model leaves
+// workgroups whose times differ by several microseconds, the same workgroups
+// sweep after sweep (R-MAT s22: slowest 4.5 us past the mean, sweep-to-sweep
+// correlation of the workgroups' times 0.9 to 0.96).  A calibration solve
+// (uniform start, UNIFORM dangling, tol 0) records every workgroup's
+// tile-pass time over kCalibSweeps sweeps; recut_tasks scales the chunk costs
+// by them and cuts again.  kCalibRounds rounds; the cut whose slowest
+// workgroup measured shortest is kept.  The cut decides only which workgroup
+// adds which chunk: every partial sum goes to the same slot, so results do not
+// change.  Layouts with fewer than kCalibMinChunks chunks per workgroup keep
+// the model's cut.
+void calibrate_tile_pass(Graph& g, hipStream_t s) {
+  constexpr uint32_t kCalibRounds = 3;
+  constexpr uint32_t kCalibMinChunks = 64;
+  Tiled* t = select_tiled(g, false, s);
+  if (!t || !t->built || t->nblocks == 0 || t->nchunks < kCalibMinChunks * t->nblocks) return;
+  const uint32_t G = t->nblocks;
+  t->wg_time.alloc(sizeof(uint32_t) * kCalibSweeps * G);
+  nwgr_pr_params p{};
+  p.damping = 0.85;
+  p.tol = 0.0;
+  p.max_iters = kCalibSweeps + 1;  // sweeps 2 .. kCalibSweeps + 1 are recorded
+  p.dangling = kDanglingUniform;
+  SolveSpec sp;
+  sp.kind = kPlain;
+  sp.plan = &g.fwd;
+  sp.offs = g.offs.as<uint32_t>();
+  sp.idx = g.idx.as<uint32_t>();
+  sp.div = g.degf.as<float>();
+  sp.tiled = t;
+  sp.quiet = g.nz < g.n;
+  if (sp.quiet) prepare_quiet(g, s);
+  std::vector<uint32_t> ticks(static_cast<size_t>(kCalibSweeps) * G);
+  std::vector<double> us(G), best_cost;
+  double best = std::numeric_limits<double>::infinity();
+  g.calib_on = true;
+  for (uint32_t round = 0;; ++round) {
+    NWGR_HIP(hipMemsetAsync(t->wg_time.p, 0, t->wg_time.bytes, s));
+    nwgr_pr_result res{};
+    solve(g, sp, p, res, s);
+    NWGR_HIP(hipMemcpyAsync(ticks.data(), t->wg_time.p, t->wg_time.bytes, hipMemcpyDeviceToHost, s));
+    NWGR_HIP(hipStreamSynchronize(s));
+    double worst = 0;
+    for (uint32_t b = 0; b < G; ++b) {
+      double sum = 0;
+      uint32_t cnt = 0;
+      for (uint32_t k = 0; k < kCalibSweeps; ++k)
+        if (const uint32_t v = ticks[static_cast<size_t>(k) * G + b]) {
+          sum += v;
+          cnt += 1;
+        }
+      us[b] = cnt ? sum / cnt / 100.0 : 0.0;  // 100 MHz wall clock
+      worst = std::max(worst, us[b]);
+    }
+    if (worst < best) {
+      best = worst;
+      best_cost = t->ccost;
+    }
+    if (round == kCalibRounds || !recut_tasks(g, *t, us, s)) break;
+  }
+  g.calib_on = false;
+  if (!best_cost.empty() && best_cost != t->ccost) {
+    t->ccost = std::move(best_cost);
+    plan_tasks(g, *t, s);
+  }
+  g.drop_graphs();
 }
 
 void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s,

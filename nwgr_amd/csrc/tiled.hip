@@ -331,9 +331,13 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
   const uint32_t G = layout_blocks(g, t);
   const double F = g.tune.tile_fill_cost;
   // chunk costs from the fragments of every chunk (a chunk starts a group,
-  // so its first lane starts a fragment)
+  // so its first lane starts a fragment), unless measured re-cuts scaled them
   std::vector<double> pc(static_cast<size_t>(t.nchunks) + 1, 0.0);
-  if (t.nchunks) {
+  if (t.ccost.size() != t.nchunks) {
+    t.ccost.assign(t.nchunks, 0.0);
+    t.calib_rounds = 0;
+  }
+  if (t.nchunks && t.calib_rounds == 0) {
     std::vector<uint32_t> fr(t.nchunks);
     DevBuf d;
     d.alloc(sizeof(uint32_t) * t.nchunks);
@@ -342,8 +346,9 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
     NWGR_HIP(hipGetLastError());
     NWGR_HIP(hipMemcpyAsync(fr.data(), d.p, sizeof(uint32_t) * t.nchunks, hipMemcpyDeviceToHost, s));
     NWGR_HIP(hipStreamSynchronize(s));
-    for (uint32_t c = 0; c < t.nchunks; ++c) pc[c + 1] = pc[c] + chunk_cost(fr[c]);
+    for (uint32_t c = 0; c < t.nchunks; ++c) t.ccost[c] = chunk_cost(fr[c]);
   }
+  for (uint32_t c = 0; c < t.nchunks; ++c) pc[c + 1] = pc[c] + t.ccost[c];
   uint32_t busy = 0;
   for (uint32_t tile = 0; tile < t.ntiles; ++tile) busy += t.tile_chunk0[tile + 1] > t.tile_chunk0[tile];
   // one range holding everything always fits
@@ -377,6 +382,44 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
     NWGR_HIP(hipMemcpyAsync(t.tasks.p, tasks.data(), sizeof(TiledTask) * tasks.size(), hipMemcpyHostToDevice, s));
   NWGR_HIP(hipMemcpyAsync(t.wg_task.p, wg.data(), sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, s));
   NWGR_HIP(hipStreamSynchronize(s));
+  t.tasks_h = tasks;
+  t.wg_h = std::move(wg);
+}
+
+// A workgroup's modelled cost is its tasks' fills and chunk costs; its
+// measured time over that is its ratio, relative to the ratio of all
+// workgroups together.  Every chunk of the workgroup takes that factor
+// (clamped to [1/2, 2]), so a re-cut moves chunks out of the slow ranges.
+bool recut_tasks(Graph& g, Tiled& t, const std::vector<double>& wg_us, hipStream_t s) {
+  const uint32_t G = t.nblocks;
+  if (wg_us.size() != G || t.wg_h.size() != G + 1 || t.ccost.size() != t.nchunks) return false;
+  const double F = t.fill_cost;
+  std::vector<double> model(G, 0.0);
+  double sm = 0, su = 0;
+  for (uint32_t b = 0; b < G; ++b) {
+    for (uint32_t k = t.wg_h[b]; k < t.wg_h[b + 1]; ++k) {
+      const TiledTask& task = t.tasks_h[k];
+      model[b] += F;
+      for (uint32_t c = task.chunk_lo; c < task.chunk_hi; ++c) model[b] += t.ccost[c];
+    }
+    if (model[b] > 0 && wg_us[b] > 0) {
+      sm += model[b];
+      su += wg_us[b];
+    }
+  }
+  if (sm <= 0 || su <= 0) return false;
+  for (uint32_t b = 0; b < G; ++b) {
+    if (model[b] <= 0 || wg_us[b] <= 0) continue;
+    const double f = std::clamp((wg_us[b] / model[b]) / (su / sm), 0.5, 2.0);
+    for (uint32_t k = t.wg_h[b]; k < t.wg_h[b + 1]; ++k) {
+      const TiledTask& task = t.tasks_h[k];
+      for (uint32_t c = task.chunk_lo; c < task.chunk_hi; ++c) t.ccost[c] *= f;
+    }
+  }
+  t.calib_rounds += 1;
+  plan_tasks(g, t, s);
+  g.drop_graphs();
+  return true;
 }
 
 void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, uint64_t m, uint32_t row_lo,
