@@ -238,8 +238,8 @@ struct Plan {
 // copies the tile's contributions into LDS and gathers from there.  Each lane
 // reduces its entries; runs crossing lanes are joined by a segmented scan
 // over the 16 lanes of a DPP row (no LDS traffic), and every fragment's sum
-// is stored at the fragment's ordinal (the chunk's first ordinal, `lane_sp0`
-// at its lane 0, plus the fragment starts before it in the chunk), so a
+// is stored at the fragment's ordinal (the chunk's first ordinal,
+// `chunk_sp0`, plus the fragment starts before it in the chunk), so a
 // chunk's partials are written contiguously.  The combine pass adds each row's partials in tile order
 // (`rslot` lists them row by row) and applies the row epilogue.
 constexpr uint32_t kMaxTileBits = 15;          // local ids use 15 bits of a u16
@@ -303,7 +303,7 @@ struct Tiled {
   DevBuf eloc;       // u16[nchunks * kChunkEdges]
   DevBuf epos;       // u32[nchunks * kChunkEdges] CSR edge index (padding: m)
   DevBuf w;          // f32[nchunks * kChunkEdges] weights in tiled order (weighted kinds)
-  DevBuf lane_sp0;   // u32[nchunks * 64] ordinal of the fragment holding each lane's first entry
+  DevBuf chunk_sp0;  // u32[nchunks] ordinal of the fragment holding each chunk's first entry
   DevBuf pair_ptr;   // u32[n + 1] each row's range in rslot
   DevBuf rslot;      // u32[nsub] fragment ordinals, row by row, tiles ascending
   DevBuf partial;    // f32[nsub] one partial sum per fragment ordinal
@@ -523,7 +523,7 @@ struct Graph {
 
 inline uint64_t bytes_of(const Tiled& t) {
   uint64_t b = 0;
-  for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.lane_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
+  for (const DevBuf* d : {&t.eloc, &t.epos, &t.w, &t.chunk_sp0, &t.pair_ptr, &t.rslot, &t.partial, &t.tasks,
                           &t.wg_task, &t.wg_tile0, &t.gplan[0].ptr, &t.gplan[0].items,
                           &t.gplan[1].ptr, &t.gplan[1].items, &t.gplan[0].desc, &t.gplan[1].desc, &t.ablo, &t.aseg, &t.afrow, &t.lpp,
                           &t.lslot, &t.heavy, &t.wg_time})

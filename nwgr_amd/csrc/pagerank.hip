@@ -562,7 +562,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
 struct TileArgs {
   const uint16_t* __restrict__ eloc;
   const float* __restrict__ w;  // weights in tiled order (weighted kinds)
-  const uint32_t* __restrict__ lane_sp0;  // read at lane 0 of each chunk: the chunk's first ordinal
+  const uint32_t* __restrict__ chunk_sp0;  // each chunk's first ordinal
   const TiledTask* __restrict__ tasks;
   const uint32_t* __restrict__ wg_task;
   const uint32_t* __restrict__ wg_tile0;  // tile of each workgroup's first task (kNoTile: no task)
@@ -664,7 +664,7 @@ __device__ __forceinline__ ChunkIn load_chunk(const TileArgs& a, uint32_t c, uin
     in.w2 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + kChunkEdges / 2 + wl * 4));
     in.w3 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.w + cb + 3 * kChunkEdges / 4 + wl * 4));
   }
-  in.base = a.lane_sp0[static_cast<uint64_t>(c) * kWave];  // lane 0's entry, one address for the wave
+  in.base = a.chunk_sp0[c];  // one address for the wave
   return in;
 }
 
@@ -1724,7 +1724,7 @@ void launch_layout(Graph& g, uint32_t kind, const Tiled& t, const float* tw, con
   TileArgs ta{};
   ta.eloc = t.eloc.as<uint16_t>();
   ta.w = tw;
-  ta.lane_sp0 = t.lane_sp0.as<uint32_t>();
+  ta.chunk_sp0 = t.chunk_sp0.as<uint32_t>();
   ta.tasks = t.tasks.as<TiledTask>();
   ta.wg_task = t.wg_task.as<uint32_t>();
   ta.wg_tile0 = t.wg_tile0.as<uint32_t>();

@@ -121,11 +121,11 @@ __global__ void k_subpairs(const uint32_t* __restrict__ flagv, const uint32_t* _
   }
 }
 
-// Ordinal of the fragment holding a lane's first entry: the exclusive count
+// Ordinal of the fragment holding a chunk's first entry: the exclusive count
 // of starts before it, minus one when that entry continues a fragment.
-__global__ void k_lane_sp0(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ flagv, uint64_t nlanes,
-                           uint32_t* lane_sp0) {
-  GRID_STRIDE(l, nlanes) lane_sp0[l] = ord[l * kLaneEdges] + flagv[l * kLaneEdges] - 1u;
+__global__ void k_chunk_sp0(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ flagv, uint64_t nchunks,
+                            uint32_t* chunk_sp0) {
+  GRID_STRIDE(c, nchunks) chunk_sp0[c] = ord[c * kChunkEdges] + flagv[c * kChunkEdges] - 1u;
 }
 
 // (row, tile) runs of a CSR with rows sorted by source: one wavefront per row,
@@ -197,11 +197,11 @@ __global__ void k_frag_local(const uint32_t* __restrict__ sprow, const uint32_t*
 
 // First fragment ordinal of every tile (a chunk starts a fragment), nsub
 // after the last; an empty tile gets its successor's.
-__global__ void k_tile_ord(const uint32_t* __restrict__ lane_sp0, const uint32_t* __restrict__ chunk0,
+__global__ void k_tile_ord(const uint32_t* __restrict__ chunk_sp0, const uint32_t* __restrict__ chunk0,
                            uint32_t ntiles, uint32_t nchunks, uint32_t nsub, uint32_t* __restrict__ tord) {
   GRID_STRIDE(t, static_cast<uint64_t>(ntiles) + 1) {
     const uint32_t c = t < ntiles ? chunk0[t] : nchunks;
-    tord[t] = c < nchunks ? lane_sp0[static_cast<size_t>(c) * 64] : nsub;
+    tord[t] = c < nchunks ? chunk_sp0[c] : nsub;
   }
 }
 
@@ -317,11 +317,11 @@ Cut greedy_cut(const Tiled& t, const std::vector<double>& pc, double F, double t
   return r;
 }
 
-__global__ void k_chunk_frags(const uint32_t* __restrict__ lane_sp0, uint32_t nchunks, uint32_t nsub,
+__global__ void k_chunk_frags(const uint32_t* __restrict__ chunk_sp0, uint32_t nchunks, uint32_t nsub,
                               uint32_t* __restrict__ frags) {
   GRID_STRIDE(c, nchunks) {
-    const uint32_t a = lane_sp0[c * 64];
-    const uint32_t b = c + 1 < nchunks ? lane_sp0[(c + 1) * 64] : nsub;
+    const uint32_t a = chunk_sp0[c];
+    const uint32_t b = c + 1 < nchunks ? chunk_sp0[c + 1] : nsub;
     frags[c] = b - a;
   }
 }
@@ -341,7 +341,7 @@ void plan_tasks(Graph& g, Tiled& t, hipStream_t s) {
     std::vector<uint32_t> fr(t.nchunks);
     DevBuf d;
     d.alloc(sizeof(uint32_t) * t.nchunks);
-    k_chunk_frags<<<grid_for(t.nchunks), kT, 0, s>>>(t.lane_sp0.as<uint32_t>(), t.nchunks, t.nsub,
+    k_chunk_frags<<<grid_for(t.nchunks), kT, 0, s>>>(t.chunk_sp0.as<uint32_t>(), t.nchunks, t.nsub,
                                                       d.as<uint32_t>());
     NWGR_HIP(hipGetLastError());
     NWGR_HIP(hipMemcpyAsync(fr.data(), d.p, sizeof(uint32_t) * t.nchunks, hipMemcpyDeviceToHost, s));
@@ -504,11 +504,11 @@ void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, 
   }
   t.nsub = last[0] + last[1];
   const uint32_t nsub = t.nsub;
-  const uint64_t nlanes = mp / kLaneEdges;
-  t.lane_sp0.alloc(sizeof(uint32_t) * std::max<uint64_t>(nlanes, 1));
-  if (nlanes) {
-    k_lane_sp0<<<grid_for(nlanes), kT, 0, s>>>(ord.as<uint32_t>(), flagv.as<uint32_t>(), nlanes,
-                                               t.lane_sp0.as<uint32_t>());
+  const uint64_t nch = mp / kChunkEdges;
+  t.chunk_sp0.alloc(sizeof(uint32_t) * std::max<uint64_t>(nch, 1));
+  if (nch) {
+    k_chunk_sp0<<<grid_for(nch), kT, 0, s>>>(ord.as<uint32_t>(), flagv.as<uint32_t>(), nch,
+                                             t.chunk_sp0.as<uint32_t>());
     NWGR_HIP(hipGetLastError());
   }
   DevBuf sprow, rowcnt;
@@ -743,7 +743,7 @@ void build_accum(Graph& g, Tiled& t, hipStream_t s) {
   chunk0.alloc(sizeof(uint32_t) * (static_cast<size_t>(nt) + 1));
   NWGR_HIP(hipMemcpyAsync(chunk0.p, t.tile_chunk0.data(), sizeof(uint32_t) * (static_cast<size_t>(nt) + 1),
                           hipMemcpyHostToDevice, s));
-  k_tile_ord<<<grid_for(static_cast<uint64_t>(nt) + 1), kT, 0, s>>>(t.lane_sp0.as<uint32_t>(), chunk0.as<uint32_t>(),
+  k_tile_ord<<<grid_for(static_cast<uint64_t>(nt) + 1), kT, 0, s>>>(t.chunk_sp0.as<uint32_t>(), chunk0.as<uint32_t>(),
                                                                      nt, t.nchunks, nsub, tord.as<uint32_t>());
   NWGR_HIP(hipGetLastError());
   const uint64_t nseg = static_cast<uint64_t>(t.nab + 1) * nt;
