@@ -151,3 +151,30 @@ def test_weighted_adjoint_per_engine(ename):
 
     assert rel2(wd.grad.cpu().numpy(), dw64) <= 1e-4
     assert rel2(td.grad.cpu().numpy(), dt64) <= 1e-4
+
+
+def test_tile_pass_cut_leaves_results_bit_identical():
+    """The tile pass's workgroup ranges (cut at setup from measured workgroup
+    times, at this size) decide only which workgroup adds which chunk: every
+    partial sum goes to the same slot.  Cuts made with other fill costs give
+    bit-identical ranks, sweep counts and errors, and match the FP64 oracle."""
+    say("tile pass cuts, R-MAT s21")
+    g = nwgr_amd.rmat(21, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    assert pg.tuning["tile_fill_cost"] == 40
+    base = {}
+    for mode in ("uniform", "leak"):
+        r, res = pg.run(tol=1e-6, max_iters=100, dangling=mode)
+        assert pg.last_engine() == "tiled"
+        base[mode] = (r, res.iters, res.error)
+    for fill in (0, 1, 400, 40):
+        pg.tuning = {"tile_fill_cost": fill}
+        for mode in ("uniform", "leak"):
+            r, res = pg.run(tol=1e-6, max_iters=100, dangling=mode)
+            r0, it0, e0 = base[mode]
+            assert res.iters == it0 and res.error == e0, (fill, mode)
+            assert np.array_equal(r, r0), (fill, mode)
+    r64, k64 = oracle.jacobi(g.offsets, g.indices, deg, 0.85, 1e-6, 100, dangling=oracle.UNIFORM, nthreads=4)
+    assert abs(base["uniform"][1] - k64) <= 1
+    assert l1(base["uniform"][0], r64) <= 1e-6
