@@ -93,8 +93,6 @@ void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t 
   if (page_rank.size() < n) throw std::runtime_error("nwgr: page_rank must hold graph.size() entries");
   if (degrees_size < n) throw std::runtime_error("nwgr: degrees must hold graph.size() entries");
   if (n == 0) return;  // page_rank.hpp:92-147 over an empty graph writes nothing
-  nwgr_pr_graph* g = nullptr;  // owned by the library's resident-graph cache
-  check(nwgr_pr_graph_cached(offsets, indices, n, m, degrees, device(), &g, nullptr));
   nwgr_pr_params p;
   nwgr_pr_params_default(&p);
   p.damping = static_cast<double>(damping_factor);
@@ -102,22 +100,26 @@ void run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t 
   p.max_iters = max_iters > 0xffffffffu ? 0xffffffffu : static_cast<uint32_t>(max_iters);
   p.dangling = NWGR_DANGLING_LEAK;
   nwgr_pr_result res;
+  // the device graph is the library cache's resident one (nwgr_pr_cached_run)
+  auto solve = [&](uint32_t which, void* out) {
+    check(nwgr_pr_cached_run(offsets, indices, n, m, degrees, device(), which, &p, out, &res, nullptr));
+  };
   if (kind == solver::residual) {
     // the residual push is FP32 on the device
     if constexpr (std::is_same_v<Real, float>) {
-      check(nwgr_pr_delta_run_f32(g, &p, page_rank.data(), &res));
+      solve(NWGR_SOLVE_DELTA_F32, page_rank.data());
     } else {
       std::vector<float> tmp(n);
-      check(nwgr_pr_delta_run_f32(g, &p, tmp.data(), &res));
+      solve(NWGR_SOLVE_DELTA_F32, tmp.data());
       std::copy(tmp.begin(), tmp.end(), page_rank.begin());
     }
   } else if constexpr (std::is_same_v<Real, float>) {
-    check(nwgr_pr_run_f32(g, &p, page_rank.data(), &res));
+    solve(NWGR_SOLVE_F32, page_rank.data());
   } else if constexpr (std::is_same_v<Real, double>) {
-    check(nwgr_pr_run_f64(g, &p, page_rank.data(), &res));
+    solve(NWGR_SOLVE_F64, page_rank.data());
   } else {
     std::vector<double> tmp(n);
-    check(nwgr_pr_run_f64(g, &p, tmp.data(), &res));
+    solve(NWGR_SOLVE_F64, tmp.data());
     std::copy(tmp.begin(), tmp.end(), page_rank.begin());
   }
 }

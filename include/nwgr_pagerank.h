@@ -152,6 +152,23 @@ void nwgr_pr_graph_destroy(nwgr_pr_graph* g);
  * used by one thread at a time. */
 int nwgr_pr_graph_cached(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
                          const uint32_t* degrees, int device, nwgr_pr_graph** out, uint32_t* hit);
+/* One whole drop-in call on the resident graph of the arrays: the solve
+ * nw::graph::page_rank performs (page_rank.hpp:89-91; NWGR_SOLVE_F32 as
+ * nwgr_pr_run_f32 into float r_out, NWGR_SOLVE_F64 as nwgr_pr_run_f64 into
+ * double r_out) or the residual push of page_rank_v13
+ * (experimental/algorithms/page_rank.hpp:632-634; NWGR_SOLVE_DELTA_F32 as
+ * nwgr_pr_delta_run_f32), with host vectors.  The graph is found or made as
+ * by nwgr_pr_graph_cached, except that a key match is confirmed while the
+ * solve runs: if the contents changed, that result is discarded, the graph
+ * rebuilt and the solve run again.  n = 0 writes nothing and returns
+ * NWGR_OK.  Holds the cache lock for the whole call.  *hit as for
+ * nwgr_pr_graph_cached (`hit` may be NULL). */
+#define NWGR_SOLVE_F32 0u
+#define NWGR_SOLVE_F64 1u
+#define NWGR_SOLVE_DELTA_F32 2u
+int nwgr_pr_cached_run(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,
+                       const uint32_t* degrees, int device, uint32_t solver, const nwgr_pr_params* p, void* r_out,
+                       nwgr_pr_result* res, uint32_t* hit);
 /* Destroys every cached handle. */
 void nwgr_pr_cache_clear(void);
 /* Most handles kept (1 .. 64); the least recently used beyond it are destroyed. */

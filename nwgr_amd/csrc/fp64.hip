@@ -394,8 +394,7 @@ int nwgr_pr_run_f64(nwgr_pr_graph* g, const nwgr_pr_params* p, double* r_out, nw
     if (g->out_buf.bytes < 2 * nb) g->out_buf.alloc(2 * nb);
     double* out = static_cast<double*>(g->out_buf.p);
     nwgr::run_plain64(*g, q, out, *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, out, 2 * nb, hipMemcpyDeviceToHost, s));
-    NWGR_HIP(hipStreamSynchronize(s));
+    nwgr::copy_out(*g, r_out, out, 2 * nb, s);
   });
 }
 

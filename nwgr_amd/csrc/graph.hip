@@ -404,7 +404,26 @@ Graph::~Graph() {
   drop_graphs();
   if (hflag) (void)hipHostFree(hflag);
   if (hist_h) (void)hipHostFree(hist_h);
+  if (pin_out) (void)hipHostFree(pin_out);
   if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+// A copy into pageable memory goes through the runtime's own staging at about
+// half the rate of one into pinned memory and a host copy after it (4 M
+// floats: 1.0 ms against 0.31 + 0.22 ms), and ran 1.4 ms slower still for the
+// first dozen calls after a second large graph was built.
+void copy_out(Graph& g, void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  if (g.pin_bytes < bytes) {
+    if (g.pin_out) NWGR_HIP(hipHostFree(g.pin_out));
+    g.pin_out = nullptr;
+    g.pin_bytes = 0;
+    NWGR_HIP(hipHostMalloc(&g.pin_out, bytes, hipHostMallocDefault));
+    g.pin_bytes = bytes;
+  }
+  NWGR_HIP(hipMemcpyAsync(g.pin_out, src, bytes, hipMemcpyDeviceToHost, s));
+  NWGR_HIP(hipStreamSynchronize(s));
+  std::memcpy(dst, g.pin_out, bytes);
 }
 
 Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -471,6 +472,8 @@ struct Graph {
   double* hist_h = nullptr;    // pinned, mapped: the sweeps write their errors here directly
   HostFlag* hflag_dev = nullptr;
   double* hist_dev = nullptr;  // hist_h as the device addresses it
+  void* pin_out = nullptr;     // pinned staging of host-output results (copy_out)
+  size_t pin_bytes = 0;
 
   // captured chunk graphs: [kind or kExecGs][parity of the first sweep][length]
   static constexpr uint32_t kMaxChunk = 32;
@@ -567,6 +570,14 @@ Graph* cache_acquire(const uint32_t* offs, const uint32_t* idx, uint32_t n, uint
                      int device, bool& hit);
 // drop `g` from the cache without destroying it; false if it is not cached
 bool cache_forget(Graph* g);
+// run(g) on the resident graph of the arrays, their contents compared while
+// it runs (run again on a rebuilt graph if they changed)
+void cache_run(const uint32_t* offs, const uint32_t* idx, uint32_t n, uint64_t m, const uint32_t* deg, int device,
+               const std::function<void(Graph*)>& run, bool& hit);
+
+// device-to-host copy of a result through the handle's pinned staging buffer
+// (graph.hip); returns once `dst` holds it
+void copy_out(Graph& g, void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // graph construction (graph.hip)
 Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n, uint64_t m,

@@ -2319,6 +2319,11 @@ void calibrate_tile_pass(Graph& g, hipStream_t s) {
     plan_tasks(g, *t, s);
   }
   g.drop_graphs();
+  // the calibration solves are not the caller's: no last solve to report
+  g.last_errs.clear();
+  g.last_profile = SolveProfile{};
+  g.last_engine = -1;
+  for (uint32_t k = 0; k < kExecSlots; ++k) g.hint_key[k] = 0;
 }
 
 void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nwgr_pr_result& res, hipStream_t s,
@@ -2546,8 +2551,7 @@ int nwgr_pr_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_out, nwg
       NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.init), p->init, nb, hipMemcpyHostToDevice, s));
     }
     nwgr::run_plain(*g, q, out, *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
-    NWGR_HIP(hipStreamSynchronize(s));
+    nwgr::copy_out(*g, r_out, out, nb, s);
   });
 }
 
@@ -2591,8 +2595,7 @@ int nwgr_pr_gs_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t phase
       NWGR_HIP(hipMemcpyAsync(const_cast<float*>(q.init), p->init, nb, hipMemcpyHostToDevice, s));
     }
     nwgr::run_gs(*g, q, phases, out, *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, out, nb, hipMemcpyDeviceToHost, s));
-    NWGR_HIP(hipStreamSynchronize(s));
+    nwgr::copy_out(*g, r_out, out, nb, s);
   });
 }
 
@@ -2670,8 +2673,7 @@ int nwgr_pr_delta_run_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* r_ou
     const size_t nb = sizeof(float) * g->n;
     if (g->out_buf.bytes < nb) g->out_buf.alloc(nb);
     nwgr::run_delta(*g, *p, g->out_buf.as<float>(), *res, s);
-    NWGR_HIP(hipMemcpyAsync(r_out, g->out_buf.p, nb, hipMemcpyDeviceToHost, s));
-    NWGR_HIP(hipStreamSynchronize(s));
+    nwgr::copy_out(*g, r_out, g->out_buf.p, nb, s);
   });
 }
 

@@ -382,6 +382,28 @@ def _resident(graph: CSRGraph, degrees, device: int) -> PageRankGraph:
     return g
 
 
+_SOLVE_F32, _SOLVE_F64, _SOLVE_DELTA_F32 = 0, 1, 2
+
+
+def _cached_run(graph: CSRGraph, degrees, device: int, solver: int, p: PrParams, out: np.ndarray) -> bool:
+    """One drop-in call on the cache's resident graph (nwgr_pr_cached_run:
+    the arrays' contents are confirmed while the solve runs).  False when
+    the arrays are not the contiguous uint32 ones the cache keys on."""
+    off = np.ascontiguousarray(graph.offsets, dtype=np.uint32)
+    ind = np.ascontiguousarray(graph.indices, dtype=np.uint32)
+    deg = None if degrees is None else np.ascontiguousarray(degrees, dtype=np.uint32)
+    if off is not graph.offsets or ind is not graph.indices or (degrees is not None and deg is not degrees):
+        return False
+    if deg is not None and len(deg) != graph.n:
+        raise ValueError("degrees must have one entry per vertex")
+    res = PrResult()
+    check(pagerank_lib().nwgr_pr_cached_run(off.ctypes.data if len(off) else None,
+                                            ind.ctypes.data if len(ind) else None, graph.n, graph.m,
+                                            None if deg is None else deg.ctypes.data, device, solver,
+                                            ctypes.byref(p), out.ctypes.data, ctypes.byref(res), None))
+    return True
+
+
 def _output(page_rank, n: int, dtype=np.float32):
     """(array of `dtype` the solve writes, whether it is page_rank itself)."""
     if isinstance(page_rank, np.ndarray) and page_rank.dtype == dtype and page_rank.flags.c_contiguous \
@@ -408,10 +430,12 @@ def page_rank(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, threshol
     if graph.n == 0:
         return
     fp64 = getattr(page_rank, "dtype", None) == np.float64
-    with _resident(graph, degrees, device) as g:
-        out, direct = _output(page_rank, graph.n, np.float64 if fp64 else np.float32)
-        solve = g.run64 if fp64 else g.run
-        solve(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak", out=out)
+    out, direct = _output(page_rank, graph.n, np.float64 if fp64 else np.float32)
+    p = _params(damping_factor, threshold, max_iters, "leak", 0, False)
+    if not _cached_run(graph, degrees, device, _SOLVE_F64 if fp64 else _SOLVE_F32, p, out):
+        with _resident(graph, degrees, device) as g:
+            solve = g.run64 if fp64 else g.run
+            solve(damping=damping_factor, tol=threshold, max_iters=max_iters, dangling="leak", out=out)
     if not direct:
         page_rank[:] = out
 
@@ -429,9 +453,11 @@ def page_rank_delta(graph: CSRGraph, degrees, page_rank, damping_factor=0.85, th
         raise ValueError("page_rank must be pre-sized to graph.size()")
     if graph.n == 0:
         return
-    with _resident(graph, degrees, device) as g:
-        out, direct = _output(page_rank, graph.n)
-        g.run_delta(damping=damping_factor, threshold=threshold, max_iters=max_iters, out=out)
+    out, direct = _output(page_rank, graph.n)
+    p = _params(damping_factor, threshold, max_iters, "leak", 0, False)  # the threshold as tol (run_delta)
+    if not _cached_run(graph, degrees, device, _SOLVE_DELTA_F32, p, out):
+        with _resident(graph, degrees, device) as g:
+            g.run_delta(damping=damping_factor, threshold=threshold, max_iters=max_iters, out=out)
     if not direct:
         page_rank[:] = out
 
