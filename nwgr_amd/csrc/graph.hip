@@ -3,7 +3,8 @@
 // Input is the reference's in-CSR (adjacency<1>, build.hpp:209-265) plus the
 // caller's degree array (bench/common.hpp:109-120).  On the device the graph
 // is relabelled so that rows appear in non-increasing in-degree order (ties:
-// ascending original id).  Degree bins then become contiguous row ranges and
+// ascending original id; among the rows without entries, the sources of some
+// entry before the isolated vertices).  Degree bins then become contiguous row ranges and
 // hub rows come first in the work queue.  Row entries are renamed through
 // the same permutation and sorted ascending.  `perm` (new -> original) and
 // `iperm` (original -> new) map results back to the caller's order.
@@ -48,12 +49,32 @@ __global__ void k_out_degree(const uint32_t* __restrict__ idx, uint64_t m, uint3
     atomicAdd(&deg[idx[k]], 1u);
 }
 
-__global__ void k_in_degree(const uint32_t* __restrict__ offs, uint32_t n, uint32_t* key, uint32_t* val) {
+// Rows that are a source of some entry (every writer stores the same 1).
+__global__ void k_mark_sources(const uint32_t* __restrict__ idx, uint64_t m, uint32_t* has_out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < m; k += stride)
+    has_out[idx[k]] = 1u;
+}
+
+// Relabelling key: in-degree + 1 for a row with entries; for a row without,
+// 1 when it is a source, 0 when the vertex is isolated -- so the
+// non-increasing order ends with the isolated vertices, whose contributions
+// no sweep reads.
+__global__ void k_relabel_key(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ has_out, uint32_t n,
+                              uint32_t* key, uint32_t* val) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n; v += stride) {
-    key[v] = offs[v + 1] - offs[v];
+    const uint32_t d = offs[v + 1] - offs[v];
+    key[v] = d ? d + 1 : has_out[v];
     val[v] = static_cast<uint32_t>(v);
   }
+}
+
+// Sorted keys back to in-degrees.
+__global__ void k_key_to_indeg(uint32_t* key, uint32_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n; v += stride)
+    key[v] = key[v] >= 2 ? key[v] - 1 : 0u;
 }
 
 __global__ void k_invert(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* iperm) {
@@ -508,15 +529,25 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
     d_deg = in_deg.as<uint32_t>();
   }
 
-  // --- relabel rows by non-increasing in-degree (stable => ties by id)
+  // --- relabel rows by non-increasing in-degree (stable => ties by id; among
+  // the rows without entries the sources first, then the isolated vertices)
   DevBuf key_in, val_in, key_out;
   key_in.alloc(nb);
   val_in.alloc(nb);
   key_out.alloc(nb);
   g->perm.alloc(nb);
   g->iperm.alloc(nb);
-  k_in_degree<<<grid_for(n), kT, 0, stream>>>(d_offs, n, key_in.as<uint32_t>(), val_in.as<uint32_t>());
-  NWGR_HIP(hipGetLastError());
+  {
+    DevBuf has_out;
+    has_out.alloc(nb);
+    NWGR_HIP(hipMemsetAsync(has_out.p, 0, nb, stream));
+    if (m) k_mark_sources<<<grid_for(m), kT, 0, stream>>>(d_idx, m, has_out.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    k_relabel_key<<<grid_for(n), kT, 0, stream>>>(d_offs, has_out.as<uint32_t>(), n, key_in.as<uint32_t>(),
+                                                  val_in.as<uint32_t>());
+    NWGR_HIP(hipGetLastError());
+    NWGR_HIP(hipStreamSynchronize(stream));  // has_out is freed
+  }
   {
     size_t tmp = 0;
     NWGR_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, key_in.as<uint32_t>(),
@@ -531,6 +562,17 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
                                                          32, stream));
   }
   k_invert<<<grid_for(n), kT, 0, stream>>>(g->perm.as<uint32_t>(), n, g->iperm.as<uint32_t>());
+  NWGR_HIP(hipGetLastError());
+  // the isolated vertices: the zero keys at the end
+  {
+    std::vector<uint32_t> keys_h(n);
+    NWGR_HIP(hipMemcpyAsync(keys_h.data(), key_out.p, nb, hipMemcpyDeviceToHost, stream));
+    NWGR_HIP(hipStreamSynchronize(stream));
+    uint32_t act = n;
+    while (act > 0 && keys_h[act - 1] == 0) --act;
+    g->n_act = act;
+  }
+  k_key_to_indeg<<<grid_for(n), kT, 0, stream>>>(key_out.as<uint32_t>(), n);
   NWGR_HIP(hipGetLastError());
   // new offsets = exclusive scan of the sorted in-degrees
   g->offs.alloc(nb + sizeof(uint32_t));
@@ -625,6 +667,7 @@ Graph* create_graph(const uint32_t* offsets, const uint32_t* indices, uint32_t n
         break;
       }
     g->nz = nz;
+    g->n_act = std::max(g->n_act, nz);
   }
   replan(*g, stream);  // plans and sweep grid for the default tuning
 

@@ -456,6 +456,10 @@ struct Graph {
   // common rank, so the tiled plain solve keeps it as a scalar (State::rq) and
   // never stores their ranks (their contributions rq / degree are stored).
   uint32_t nz = 0;
+  // rows from n_act on are isolated vertices (no entries, no out-edges): no
+  // sweep reads their contributions, and a solve that keeps the quiet rows
+  // as a scalar stores none for them (n_act >= nz)
+  uint32_t n_act = 0;
   uint32_t nq_dangling = 0;  // quiet rows whose divisor is 0 (their rank is dangling mass)
   bool quiet_ready = false;  // nq_dangling counted
 

@@ -78,6 +78,7 @@ struct SweepArgs {
   uint32_t nrows;      // rows the sweep stores: n, or nz with quiet rows
   uint32_t nq_dangling;  // quiet rows with divisor 0
   uint32_t quiet;      // 1: rows [nrows, n) are quiet (common rank State::rq)
+  uint32_t q_end;      // quiet rows from here on are isolated: their contributions are not stored
   float thr;           // push threshold (kDelta)
   double tol;
   // gradient coefficients (kCoef): a = d r / s, b = a * (P lambda)
@@ -107,6 +108,7 @@ struct InitArgs {
   uint32_t npairs;     // pairs sweep 1 adds: this grid's, then zeros
   uint32_t quiet_from; // rows from here on are quiet (their ranks are not stored)
   float* out;          // caller-order output the stopping gather pass writes (null: none)
+  uint32_t c_from;     // contributions stored below this row (the rest are isolated vertices')
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
         a.r0[i] = r;
         if (a.r1) a.r1[i] = r;
       }
-      a.c0[i] = r / dg;
+      if (i < a.c_from) a.c0[i] = r / dg;
       if (dg == 0.f) dang += static_cast<double>(r);
     }
   }
@@ -1151,7 +1153,7 @@ __device__ __forceinline__ void quiet_rows(const SweepArgs& a, float Df, double&
       const float u = a.dangling == kDanglingUniform ? a.un : 0.f;
       const float sq = fmaf(Df, u, 0.f);
       const float rn = fmaf(a.d_hi, sq, fmaf(a.d_lo, sq, a.base_u));
-      for (uint32_t v = a.nrows + blockIdx.x * NT + threadIdx.x; v < a.n; v += gridDim.x * NT)
+      for (uint32_t v = a.nrows + blockIdx.x * NT + threadIdx.x; v < a.q_end; v += gridDim.x * NT)
         a.c_out[v] = rn / a.div[v];
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         const float ro = a.st->rq[a.p ^ 1];
@@ -1353,7 +1355,8 @@ __device__ __forceinline__ void combine_quiet(const CombineArgs& a, uint32_t q, 
     const float sq = fmaf(Df, u, 0.f);
     const float rn = fmaf(s.d_hi, sq, fmaf(s.d_lo, sq, s.base_u));
     const uint32_t lo = s.nrows + q * kQuietRows,
-                   hi = s.dangling != kDanglingLeak || s.st->ctr[s.p] <= 2 ? min(s.n, lo + kQuietRows) : lo;
+                   hi = s.dangling != kDanglingLeak || s.st->ctr[s.p] <= 2 ? max(lo, min(s.q_end, lo + kQuietRows))
+                                                                          : lo;
     constexpr uint32_t U = kQuietRows / kCombineThreads;  // one round of loads
     for (uint32_t v0 = lo + threadIdx.x; v0 < hi; v0 += U * kCombineThreads) {
       float dv[U];
@@ -1651,6 +1654,7 @@ SweepArgs sweep_args(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, uin
   a.un = static_cast<float>(1.0 / g.n);
   a.quiet = sp.quiet ? 1u : 0u;
   a.nrows = sp.quiet ? g.nz : g.n;
+  a.q_end = sp.quiet ? g.n_act : g.n;
   a.nq_dangling = g.nq_dangling;
   a.thr = sp.thr;
   a.tol = p.tol;
@@ -2040,6 +2044,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
   // the stopping launch's gather pass writes the output (tiled sweeps without
   // phases); otherwise a launch of its own does
   const bool fused_fin = fin && sp.tiled && !sp.phases;
+  ia.c_from = sp.quiet ? g.n_act : g.n;
   ia.out = fused_fin ? fin->out : nullptr;
   k_init<<<std::min(sp.tiled ? g.grid_c : g.grid, npairs), kBlock, 0, s>>>(ia);
   NWGR_HIP(hipGetLastError());

@@ -624,7 +624,10 @@ void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
     const uint32_t stop = mode ? std::min(g.nz, t.row_hi) : t.row_hi;
     gp.nwin = static_cast<uint32_t>(std::lower_bound(t.win_h.begin(), t.win_h.begin() + t.nwin, stop) -
                                     t.win_h.begin());
-    gp.nqb = mode && t.quiet_tail && quiet_rows ? (quiet_rows + kQuietRows - 1) / kQuietRows : 0;
+    // (the stored quiet rows are [nz, n_act): the isolated vertices after them
+    // store nothing; one block always holds the common rank)
+    const uint32_t stored = std::max(g.n_act, std::min(g.nz, g.n)) - std::min(g.nz, g.n);
+    gp.nqb = mode && t.quiet_tail && quiet_rows ? std::max(1u, (stored + kQuietRows - 1) / kQuietRows) : 0;
     const uint32_t nitems = t.nbig + gp.nwin + gp.nqb;
     std::vector<float> cost(nitems);
     double total = 0;
@@ -634,7 +637,7 @@ void plan_gather(Graph& g, Tiled& t, hipStream_t s) {
       else if (i - t.nbig < gp.nwin)
         cost[i] = t.win_cost[i - t.nbig];
       else
-        cost[i] = quiet_block_cost(std::min(kQuietRows, quiet_rows - (i - t.nbig - gp.nwin) * kQuietRows));
+        cost[i] = quiet_block_cost(std::min(kQuietRows, stored - std::min(stored, (i - t.nbig - gp.nwin) * kQuietRows)));
       total += cost[i];
     }
     std::vector<uint32_t> home(nitems);
