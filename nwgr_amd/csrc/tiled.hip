@@ -791,11 +791,13 @@ std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
   constexpr double kCutEntry = 8.0, kCutFrag = 2.9, kCutRow = 80.0, kCutQuiet = 1.0;
   std::vector<uint32_t> cuts(phases + 1, 0);
   const auto& oh = g.offs_h;
-  const double tiles = static_cast<double>((static_cast<uint64_t>(g.n) + (uint64_t{1} << g.tune.tile_bits) - 1) >>
-                                           g.tune.tile_bits);
+  // the tiles that hold sources (the isolated vertices, last, are none)
+  const double tiles = static_cast<double>((static_cast<uint64_t>(std::max(g.n_act, 1u)) +
+                                            (uint64_t{1} << g.tune.tile_bits) - 1) >> g.tune.tile_bits);
   auto cost = [&](uint32_t r) -> double {
     const double e = static_cast<double>(oh[r + 1] - oh[r]);
-    return e > 0 ? kCutEntry * e + kCutFrag * (std::min(e, tiles) + std::floor(e / 256)) + kCutRow : kCutQuiet;
+    if (e > 0) return kCutEntry * e + kCutFrag * (std::min(e, tiles) + std::floor(e / 256)) + kCutRow;
+    return r < g.n_act ? kCutQuiet : 0.0;  // an isolated vertex is never stored
   };
   double total = 0;
   for (uint32_t r = 0; r < g.n; ++r) total += cost(r);

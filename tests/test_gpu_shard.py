@@ -81,12 +81,16 @@ def test_each_share_alone_on_its_own_layout(shares):
     pg = PageRankGraph(g, deg)
     order, cuts = pg.gs_phases(shares)
     # the shares are the row partition a multi-GPU solve takes: about equal
-    # sweep cost (8 per entry, 2.9 per fragment -- about min(entries, tiles)
-    # + entries // 256 of them -- 80 per row with entries, 1 per row without)
+    # sweep cost (8 per entry, 2.9 per fragment -- about min(entries, tiles
+    # holding sources) + entries // 256 of them -- 80 per row with entries, 1
+    # per row without that is a source, 0 per isolated vertex)
     counts = np.diff(g.offsets.astype(np.int64))[order]
-    tiles = -(-g.n // 2 ** 15)
+    source = (deg[order] > 0)
+    n_act = int(np.count_nonzero((counts > 0) | source))
+    assert np.all(((counts > 0) | source)[:n_act])  # the isolated vertices are last
+    tiles = -(-max(n_act, 1) // 2 ** 15)
     frags = np.minimum(counts, tiles) + counts // 256
-    cost = np.where(counts > 0, 8.0 * counts + 2.9 * frags + 80.0, 1.0)
+    cost = np.where(counts > 0, 8.0 * counts + 2.9 * frags + 80.0, np.where(source, 1.0, 0.0))
     per = np.add.reduceat(cost, cuts[:-1].astype(np.int64))
     assert per.max() <= cost.sum() / shares + cost.max(), (per, cost.sum() / shares)
     out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
