@@ -426,6 +426,7 @@ Graph::~Graph() {
   if (hflag) (void)hipHostFree(hflag);
   if (hist_h) (void)hipHostFree(hist_h);
   if (pin_out) (void)hipHostFree(pin_out);
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
   if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 

@@ -478,6 +478,8 @@ struct Graph {
   double* hist_dev = nullptr;  // hist_h as the device addresses it
   void* pin_out = nullptr;     // pinned staging of host-output results (copy_out)
   size_t pin_bytes = 0;
+  // timing events the solves reuse (one solve at a time per handle)
+  std::vector<hipEvent_t> events;
 
   // captured chunk graphs: [kind or kExecGs][parity of the first sweep][length]
   static constexpr uint32_t kMaxChunk = 32;

@@ -1878,15 +1878,27 @@ hipGraphExec_t chunk_graph(Graph& g, const SolveSpec& sp, const nwgr_pr_params& 
   return e;
 }
 
+// The handle's events, taken in turn by one solve (created once and kept by
+// the handle, up to kKeptEvents: creating and destroying them per solve cost
+// host time between solves); a traced solve's further ones are its own.
 struct Events {
-  std::vector<hipEvent_t> v;
+  static constexpr size_t kKeptEvents = 64;
+  Graph& g;
+  size_t next = 0;
+  std::vector<hipEvent_t> own;
   ~Events() {
-    for (auto e : v) (void)hipEventDestroy(e);
+    for (hipEvent_t e : own) (void)hipEventDestroy(e);
   }
   hipEvent_t make() {
+    if (next < g.events.size()) return g.events[next++];
     hipEvent_t e;
     NWGR_HIP(hipEventCreate(&e));
-    v.push_back(e);
+    if (g.events.size() < kKeptEvents) {
+      g.events.push_back(e);
+      ++next;
+    } else {
+      own.push_back(e);
+    }
     return e;
   }
 };
@@ -2014,7 +2026,7 @@ void solve(Graph& g, const SolveSpec& sp, const nwgr_pr_params& p, nwgr_pr_resul
     g.exec_key[slot] = key;
   }
 
-  Events evs;
+  Events evs{g};
   hipEvent_t ev0 = evs.make(), ev1 = evs.make();
   std::memset(const_cast<HostFlag*>(g.hflag), 0, sizeof(HostFlag));
   NWGR_HIP(hipEventRecord(ev0, s));
