@@ -75,6 +75,8 @@ def parse():
                          "(page_rank_v13, threshold 1e-4) on R-MAT s22; shard: each GPU's row share of R-MAT s22 "
                          "swept alone on its own layout (--shards shares)")
     ap.add_argument("--shards", type=int, default=8, help="shard workload: number of row slices")
+    ap.add_argument("--balance-rounds", type=int, default=2,
+                    help="shard workload: re-cuts of the shares by their measured sweep times (0: model cut only)")
     ap.add_argument("--fwd-adjoint", type=int, default=1,
                     help="1: also time the weighted forward and forward + adjoint on R-MAT s20 (the metric's "
                          "fwd+adjoint vs fwd ratio); 0: skip")
@@ -429,7 +431,8 @@ def bench_delta(args):
 def bench_shard(args):
     """One GPU's share of a row-partitioned solve of R-MAT s22 ef16: the
     destination rows (in the handle's relabelled order) cut into `--shards`
-    consecutive ranges of about equal sweep cost, each swept alone on a
+    consecutive ranges of about equal sweep cost (the cost model's cut,
+    then re-cut by measured share times: nwgr_pr_share_balance), each swept alone on a
     tiled layout that holds only its entries and stores only its rows
     (nwgr_pr_share_run_device_f32), the other rows' contributions resident
     and fixed, as one GPU holds them between the all-gathers of its slice.
@@ -458,17 +461,31 @@ def bench_shard(args):
 
     full_us = per_sweep_us(lambda: pg.run_device(out.data_ptr(), dangling="leak", **kw))
     log(f"whole graph: {full_us:.1f} us per sweep ({pg.last_engine()})")
-    order, cuts = pg.gs_phases(G)
-    indeg = np.diff(g.offsets.astype(np.int64))[order]
-    shares = []
-    for k in range(G):
-        lo, hi = int(cuts[k]), int(cuts[k + 1])
-        us = per_sweep_us(lambda: pg.run_share_device(out.data_ptr(), G, k, **kw))
-        m_k, n_k = int(indeg[lo:hi].sum()), hi - lo
-        b_k = 8 * m_k + 16 * n_k
-        shares.append({"rows": [lo, hi], "m": m_k, "us_per_sweep": round(us, 2),
-                       "gbs": round(b_k / (us / 1e6) / 1e9, 1)})
-        log(f"share {k}: rows [{lo}, {hi}) m={m_k} {us:.1f} us per sweep")
+    indeg_caller = np.diff(g.offsets.astype(np.int64))
+
+    def measure_shares(tag):
+        order, cuts = pg.gs_phases(G)
+        indeg = indeg_caller[order]
+        shares = []
+        for k in range(G):
+            lo, hi = int(cuts[k]), int(cuts[k + 1])
+            us = per_sweep_us(lambda: pg.run_share_device(out.data_ptr(), G, k, **kw))
+            m_k, n_k = int(indeg[lo:hi].sum()), hi - lo
+            b_k = 8 * m_k + 16 * n_k
+            shares.append({"rows": [lo, hi], "m": m_k, "us_per_sweep": round(us, 2),
+                           "gbs": round(b_k / (us / 1e6) / 1e9, 1)})
+            log(f"{tag} share {k}: rows [{lo}, {hi}) m={m_k} {us:.1f} us per sweep")
+        return shares
+
+    model = measure_shares("model cut")
+    model_worst = max(s["us_per_sweep"] for s in model)
+    shares = model
+    if args.balance_rounds > 0:
+        t0 = time.perf_counter()
+        est = pg.share_balance(G, rounds=args.balance_rounds)
+        log(f"measured balance: {args.balance_rounds} rounds in {time.perf_counter() - t0:.2f} s, "
+            f"slowest share {est:.1f} us per sweep while balancing")
+        shares = measure_shares("balanced")
     worst = max(shares, key=lambda s: s["us_per_sweep"])
     b_w = 8 * worst["m"] + 16 * (worst["rows"][1] - worst["rows"][0])
     line = {
@@ -477,11 +494,14 @@ def bench_shard(args):
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"rmat s{args.scale} ef{args.edge_factor}, relabelled rows cut into {G} ranges of "
-                               f"equal sweep cost, each swept alone on its own layout (only its entries, only its "
-                               f"rows stored), the other contributions resident; 20 sweeps per solve, LEAK",
+                               f"equal sweep cost (modelled, then re-cut by measured share times over "
+                               f"{args.balance_rounds} rounds), each swept alone on its own layout (only its "
+                               f"entries, only its rows stored), the other contributions resident; 20 sweeps per "
+                               f"solve, LEAK",
                    "n": g.n, "m": g.m, "whole_graph_us_per_sweep": round(full_us, 2),
                    "ideal_us_per_sweep": round(full_us / G, 2), "shares": shares,
-                   "sum_of_shares_us": round(sum(s["us_per_sweep"] for s in shares), 2)},
+                   "sum_of_shares_us": round(sum(s["us_per_sweep"] for s in shares), 2),
+                   "model_cut_slowest_us": model_worst, "balance_rounds": args.balance_rounds},
         "roofline": {"bound": "hbm", "achieved": worst["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(worst["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_tile+k_combine (one share's layout)", "algorithmic_bytes_per_sweep": b_w},

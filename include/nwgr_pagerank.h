@@ -201,8 +201,9 @@ int nwgr_pr_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, float* d_r
  * contribution replace the old ones within the sweep) made deterministic.
  * The rows, in the handle's relabelled order (nwgr_pr_gs_phases), are cut
  * into `phases` consecutive phases of about equal sweep cost (8 per entry,
- * 2.9 per fragment -- about min(entries, tiles) + entries / 256 -- 80 per
- * row with entries, 1 per row without); inside a
+ * 2.9 per fragment -- about min(entries, tiles holding sources) + entries /
+ * 256 -- 80 per row with entries, 1 per source row without, 0 per isolated
+ * vertex; or the measured cuts of nwgr_pr_share_balance); inside a
  * phase every row gathers the contributions as they stood when the phase
  * began, and a phase's results replace the old ones before the next phase
  * gathers.  phases = 1 is the Jacobi solve of nwgr_pr_run_f32; more phases
@@ -226,10 +227,23 @@ int nwgr_pr_gs_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_
  * their initial values).  LEAK dangling; shares in [1, 64]. */
 int nwgr_pr_share_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint32_t shares, uint32_t share,
                                  float* d_r_out, nwgr_pr_result* res, void* stream);
+/* Measured balance of the row shares (no reference counterpart: the
+ * partition a multi-GPU solve would take).  Each of the `shares` shares is
+ * swept alone; the rows are re-cut with every row of a share costing its
+ * share's measured-to-mean sweep time (clamped to [1/2, 2]) times more, and
+ * the layouts rebuilt, `rounds` times (<= 8); the cuts whose slowest share
+ * measured shortest are kept -- for the shares and for the block Gauss-Seidel
+ * solve with phases = shares, whose results follow its phase cuts -- until
+ * the phases are rebuilt (another phase count, another tile width).
+ * *us_max (may be NULL): that share's device time per sweep (us). */
+int nwgr_pr_share_balance(nwgr_pr_graph* g, uint32_t shares, uint32_t rounds, double* us_max, void* stream);
 /* The sweep order of the block Gauss-Seidel solve: order[i] (n entries,
  * caller ids; may be NULL) is the i-th row visited -- in-degree
- * non-increasing, ties by ascending id -- and cuts[0 .. phases] (may be
- * NULL) the positions where the phases begin (cuts[phases] = n). */
+ * non-increasing, ties by ascending id, the isolated vertices last -- and
+ * cuts[0 .. phases] (may be NULL) the positions where the phases begin
+ * (cuts[phases] = n): the cuts of the phases' layouts when they are built
+ * for this phase count (balanced ones after nwgr_pr_share_balance), else the
+ * model's. */
 int nwgr_pr_gs_phases(const nwgr_pr_graph* g, uint32_t phases, uint32_t* order, uint32_t* cuts);
 
 /* FP64 PageRank: the iteration and stop rule of nwgr_pr_run_f32 with double

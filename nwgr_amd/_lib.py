@@ -92,6 +92,8 @@ def _declare_pagerank(L):
     L.nwgr_pr_set_sort_batch.argtypes = [u64]
     L.nwgr_pr_graph_cached.restype = i32
     L.nwgr_pr_graph_cached.argtypes = [vp, vp, u32, u64, vp, i32, P(vp), P(u32)]
+    L.nwgr_pr_share_balance.restype = i32
+    L.nwgr_pr_share_balance.argtypes = [vp, u32, u32, P(ctypes.c_double), vp]
     L.nwgr_pr_cached_run.restype = i32
     L.nwgr_pr_cached_run.argtypes = [vp, vp, u32, u64, vp, i32, u32, P(PrParams), vp, P(PrResult), P(u32)]
     L.nwgr_pr_cache_clear.restype = None

@@ -603,8 +603,13 @@ void replan(Graph& g, hipStream_t s);
 void build_tiled(Graph& g, Tiled& t, const uint32_t* offs, const uint32_t* idx, uint64_t m, uint32_t row_lo,
                  uint32_t row_hi, hipStream_t s);
 // phase cuts of block Gauss-Seidel: `phases` consecutive ranges of the
-// relabelled rows with about equal entry counts (positions, phases + 1)
-std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases);
+// relabelled rows of about equal modelled sweep cost (positions, phases + 1);
+// with old_cuts and scale, every row of old range k costs scale[k] times more
+std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases, const std::vector<uint32_t>* old_cuts = nullptr,
+                                  const std::vector<double>* scale = nullptr);
+// the phases' layouts for the row cuts `cuts` (phases + 1 entries), replacing
+// any built before (tiled.hip)
+void build_gs(Graph& g, const std::vector<uint32_t>& cuts, hipStream_t s);
 // the phases' layouts for `phases` phases (tiled.hip); no-op once built
 void ensure_gs(Graph& g, uint32_t phases, hipStream_t s);
 // row blocks and segment table of the accumulate form of the second pass

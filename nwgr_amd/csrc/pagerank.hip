@@ -2372,6 +2372,70 @@ void run_gs(Graph& g, const nwgr_pr_params& p, uint32_t phases, float* d_out, nw
   g.last_engine = 3;
 }
 
+// Measured balance of the row shares (the phases with phases = shares): each
+// share swept alone (LEAK, tol 0, kBalanceSweeps sweeps, the second of two
+// solves timed), then the rows re-cut with every row of share k costing its
+// share's measured-to-mean sweep time (clamped to [1/2, 2]) times more, the
+// layouts rebuilt and measured again; `rounds` re-cuts, the cuts with the
+// shortest slowest share kept.  Returns that share's time per sweep (us).
+// The cut model leaves shares 2.5 us apart at R-MAT s22 in a way no linear
+// fit of their entries, fragments and rows explains.
+double balance_shares(Graph& g, uint32_t shares, uint32_t rounds, hipStream_t s) {
+  constexpr uint32_t kBalanceSweeps = 10;
+  ensure_gs(g, shares, s);
+  nwgr_pr_params p{};
+  p.damping = 0.85;
+  p.tol = 0.0;
+  p.max_iters = kBalanceSweeps;
+  p.dangling = kDanglingLeak;
+  auto measure = [&](std::vector<double>& t) {
+    t.assign(shares, 0.0);
+    for (uint32_t q = 0; q < shares; ++q) {
+      SolveSpec sp;
+      sp.kind = kPlain;
+      sp.plan = &g.fwd;
+      sp.offs = g.offs.as<uint32_t>();
+      sp.idx = g.idx.as<uint32_t>();
+      sp.div = g.degf.as<float>();
+      sp.phases = &g.gs;
+      sp.only = static_cast<int>(q);
+      sp.tiled = &g.gs[q];
+      sp.quiet = g.nz < g.n && q + 1 == shares;
+      if (sp.quiet) prepare_quiet(g, s);
+      for (int rep = 0; rep < 2; ++rep) {
+        nwgr_pr_result res{};
+        solve(g, sp, p, res, s);
+        t[q] = g.last_profile.chunk_ms * 1e3 / std::max(1u, res.iters);
+      }
+    }
+    return *std::max_element(t.begin(), t.end());
+  };
+  std::vector<double> t;
+  double best = measure(t);
+  std::vector<uint32_t> best_cuts = g.gs_cuts;
+  for (uint32_t round = 0; round < rounds; ++round) {
+    double mean = 0;
+    for (double x : t) mean += x / shares;
+    std::vector<double> f(shares);
+    for (uint32_t q = 0; q < shares; ++q) f[q] = std::clamp(t[q] / mean, 0.5, 2.0);
+    const std::vector<uint32_t> old = g.gs_cuts;
+    build_gs(g, gs_cut_rows(g, shares, &old, &f), s);
+    const double worst = measure(t);
+    if (worst < best) {
+      best = worst;
+      best_cuts = g.gs_cuts;
+    }
+  }
+  if (best_cuts != g.gs_cuts) build_gs(g, best_cuts, s);
+  g.drop_graphs();
+  // the balancing solves are not the caller's: no last solve to report
+  g.last_errs.clear();
+  g.last_profile = SolveProfile{};
+  g.last_engine = -1;
+  for (uint32_t k = 0; k < kExecSlots; ++k) g.hint_key[k] = 0;
+  return best;
+}
+
 void run_delta(Graph& g, const nwgr_pr_params& p, float* d_out, nwgr_pr_result& res, hipStream_t s) {
   validate_params(p);
   NWGR_REQUIRE(!p.teleport && !p.init && p.dangling == kDanglingLeak,
@@ -2631,6 +2695,20 @@ int nwgr_pr_share_run_device_f32(nwgr_pr_graph* g, const nwgr_pr_params* p, uint
   });
 }
 
+int nwgr_pr_share_balance(nwgr_pr_graph* g, uint32_t shares, uint32_t rounds, double* us_max, void* stream) {
+  return guarded([&] {
+    NWGR_REQUIRE(g != nullptr, "null argument");
+    NWGR_REQUIRE(shares >= 1 && shares <= nwgr::kMaxGsPhases, "shares must be in [1, 64]");
+    NWGR_REQUIRE(rounds <= 8, "rounds must be <= 8");
+    if (us_max) *us_max = 0.0;
+    if (g->n == 0) return;
+    nwgr::DeviceGuard dg(g->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
+    const double w = nwgr::balance_shares(*g, shares, rounds, s);
+    if (us_max) *us_max = w;
+  });
+}
+
 int nwgr_pr_gs_phases(const nwgr_pr_graph* g, uint32_t phases, uint32_t* order, uint32_t* cuts) {
   return guarded([&] {
     NWGR_REQUIRE(g != nullptr, "null argument");
@@ -2638,6 +2716,8 @@ int nwgr_pr_gs_phases(const nwgr_pr_graph* g, uint32_t phases, uint32_t* order, 
     if (cuts) {
       if (g->n == 0) {
         std::fill(cuts, cuts + phases + 1, 0u);
+      } else if (g->gs.size() == phases && g->gs_cuts.size() == phases + 1) {
+        std::copy(g->gs_cuts.begin(), g->gs_cuts.end(), cuts);  // the layouts' cuts (measured ones after a balance)
       } else {
         const std::vector<uint32_t> c = nwgr::gs_cut_rows(*g, phases);
         std::copy(c.begin(), c.end(), cuts);

@@ -234,16 +234,13 @@ void cub_call(Fn&& fn, hipStream_t s) {
 }  // namespace
 
 // A row-range layout (a phase or a row share) holds a fraction of the entries
-// but every workgroup of its tile pass fills a whole tile: fewer workgroups
-// fill fewer tiles.
-#ifndef NWGR_SUB_LAYOUT_DIV
-#define NWGR_SUB_LAYOUT_DIV 1
-#endif
-constexpr uint32_t kSubLayoutDiv = NWGR_SUB_LAYOUT_DIV;
-
+// but every workgroup of its tile pass fills a whole tile.  It still gets the
+// whole layout's workgroups: with half or a quarter of them (fewer tile
+// fills) the slowest of eight R-MAT s22 shares took 43.0 / 57.3 us against
+// 37.0 us.
 uint32_t layout_blocks(const Graph& g, const Tiled& t) {
-  const bool sub = t.row_lo > 0 || t.row_hi < g.n;
-  return sub ? std::max(1u, tile_blocks(g) / kSubLayoutDiv) : tile_blocks(g);
+  (void)t;
+  return tile_blocks(g);
 }
 
 uint32_t tile_blocks(const Graph& g) {
@@ -779,7 +776,8 @@ __global__ void k_sub_offsets(const uint32_t* __restrict__ offs, uint32_t n, uin
 }
 }  // namespace
 
-std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
+std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases, const std::vector<uint32_t>* old_cuts,
+                                  const std::vector<double>* scale) {
   // Ranges of about equal sweep cost.  A row with e > 0 entries costs
   // kCutEntry * e + kCutFrag * (its fragments, about min(e, tiles) + e / 256)
   // + kCutRow, a row without entries kCutQuiet (the quiet scalar leaves it
@@ -799,13 +797,22 @@ std::vector<uint32_t> gs_cut_rows(const Graph& g, uint32_t phases) {
     if (e > 0) return kCutEntry * e + kCutFrag * (std::min(e, tiles) + std::floor(e / 256)) + kCutRow;
     return r < g.n_act ? kCutQuiet : 0.0;  // an isolated vertex is never stored
   };
+  // (a measured re-cut: every row of old range k costs scale[k] times more)
+  const bool scaled = old_cuts && scale && scale->size() + 1 == old_cuts->size();
+  std::vector<double> rc(g.n);
+  uint32_t k = 0;
+  for (uint32_t r = 0; r < g.n; ++r) {
+    if (scaled)
+      while (k + 1 < scale->size() && r >= (*old_cuts)[k + 1]) ++k;
+    rc[r] = cost(r) * (scaled ? (*scale)[k] : 1.0);
+  }
   double total = 0;
-  for (uint32_t r = 0; r < g.n; ++r) total += cost(r);
+  for (uint32_t r = 0; r < g.n; ++r) total += rc[r];
   double acc = 0;
   uint32_t q = 1;
   for (uint32_t r = 0; r < g.n && q < phases; ++r) {
     while (q < phases && acc >= total * q / phases) cuts[q++] = r;
-    acc += cost(r);
+    acc += rc[r];
   }
   for (; q < phases; ++q) cuts[q] = g.n;
   cuts[phases] = g.n;
@@ -824,9 +831,13 @@ void ensure_gs(Graph& g, uint32_t phases, hipStream_t s) {
     if (replanned) g.drop_graphs(static_cast<int>(kExecGs));
     return;
   }
+  build_gs(g, gs_cut_rows(g, phases), s);
+}
+
+void build_gs(Graph& g, const std::vector<uint32_t>& cuts, hipStream_t s) {
+  const uint32_t phases = static_cast<uint32_t>(cuts.size() - 1);
   g.gs.clear();
   g.drop_graphs(static_cast<int>(kExecGs));
-  const std::vector<uint32_t> cuts = gs_cut_rows(g, phases);
   DevBuf sub;
   sub.alloc(sizeof(uint32_t) * (static_cast<size_t>(g.n) + 1));
   std::vector<Tiled> layouts(phases);

@@ -243,6 +243,15 @@ class PageRankGraph:
                                                           ctypes.byref(res), stream))
         return RunResult(int(res.iters), bool(res.converged), float(res.error), float(res.ms))
 
+    def share_balance(self, shares=8, rounds=2, stream=None) -> float:
+        """Re-cut the rows of `shares` shares by their measured sweep times
+        (nwgr_pr_share_balance); the shares and the block Gauss-Seidel solve
+        with phases = shares then use these cuts.  Returns the slowest share's
+        device time per sweep (us)."""
+        us = ctypes.c_double()
+        check(pagerank_lib().nwgr_pr_share_balance(self._h, int(shares), int(rounds), ctypes.byref(us), stream))
+        return float(us.value)
+
     def gs_phases(self, phases=8):
         """(order uint32[n]: the rows in the order the block Gauss-Seidel solve
         visits them, caller ids; cuts uint32[phases + 1]: where each phase

@@ -124,3 +124,31 @@ def test_share_reference_fixed_point_dense():
     want = np.linalg.solve(M, rhs)
     r, _ = share_reference(g, deg.astype(np.uint32), rows, 1e-15, 5000)
     assert np.abs(r[rows] - want).sum() < 1e-12
+
+
+def test_balanced_shares_keep_parity():
+    """nwgr_pr_share_balance: the rows re-cut by measured share times.  The
+    cuts stay a partition of the relabelled rows; every share swept alone
+    matches the FP64 restatement of its rows' sweeps, and the block
+    Gauss-Seidel solve with phases = shares follows the new cuts (the FP64
+    block Gauss-Seidel restatement on those cuts pins it)."""
+    import torch
+    from tests.test_gpu_gs import gs_parity
+    say("balanced shares, rmat s15")
+    g = nwgr_amd.rmat(15, 16)
+    deg = nwgr_amd.build_degrees(g)
+    pg = PageRankGraph(g, deg)
+    _, model_cuts = pg.gs_phases(4)
+    us = pg.share_balance(4, rounds=2)
+    assert us > 0
+    order, cuts = pg.gs_phases(4)
+    assert cuts[0] == 0 and cuts[-1] == g.n and np.all(np.diff(cuts.astype(np.int64)) >= 0)
+    say(f"  model cuts {model_cuts.tolist()} balanced {cuts.tolist()} slowest {us:.1f} us")
+    out = torch.empty(g.n, dtype=torch.float32, device="cuda:0")
+    for q in range(4):
+        rows = order[cuts[q]:cuts[q + 1]].astype(np.int64)
+        res = pg.run_share_device(out.data_ptr(), 4, q, tol=1e-6, max_iters=200)
+        r = out.cpu().numpy()
+        r_m, _ = share_reference(g, deg, rows, 0.0, res.iters)
+        assert l1(r[rows], r_m[rows]) <= 1e-6, q
+    gs_parity(pg, g, deg, 4)
