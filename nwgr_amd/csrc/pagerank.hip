@@ -2426,13 +2426,14 @@ double balance_shares(Graph& g, uint32_t shares, uint32_t rounds, hipStream_t s)
       best_cuts = g.gs_cuts;
     }
   }
+  // a rebuild drops the sweeps captured for the other cuts
   if (best_cuts != g.gs_cuts) build_gs(g, best_cuts, s);
-  g.drop_graphs();
-  // the balancing solves are not the caller's: no last solve to report
+  // the balancing solves are not the caller's: no last solve to report, no
+  // sweep count to start the next share or phase solve from
   g.last_errs.clear();
   g.last_profile = SolveProfile{};
   g.last_engine = -1;
-  for (uint32_t k = 0; k < kExecSlots; ++k) g.hint_key[k] = 0;
+  g.hint_key[kExecGs] = 0;
   return best;
 }
 
