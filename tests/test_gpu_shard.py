@@ -152,3 +152,22 @@ def test_balanced_shares_keep_parity():
         r_m, _ = share_reference(g, deg, rows, 0.0, res.iters)
         assert l1(r[rows], r_m[rows]) <= 1e-6, q
     gs_parity(pg, g, deg, 4)
+
+
+def test_balance_on_a_tiny_graph(karate):
+    """The measured balance on karate (34 rows, 8 shares: some shares hold
+    a row or two): the cuts stay a partition, zero rounds keeps the model's
+    cuts, and the phased solve on the balanced cuts keeps parity with the
+    block Gauss-Seidel restatement on those cuts."""
+    from tests.test_gpu_gs import gs_parity
+    _, g, deg = karate
+    pg = PageRankGraph(g, deg)
+    _, model_cuts = pg.gs_phases(8)
+    assert pg.share_balance(8, rounds=0) > 0
+    np.testing.assert_array_equal(pg.gs_phases(8)[1], model_cuts)
+    pg.share_balance(8, rounds=1)
+    _, cuts = pg.gs_phases(8)
+    assert cuts[0] == 0 and cuts[-1] == g.n and np.all(np.diff(cuts.astype(np.int64)) >= 0)
+    gs_parity(pg, g, deg, 8, topk=34)
+    with pytest.raises(nwgr_amd.NwgrError):
+        pg.share_balance(8, rounds=9)
